@@ -1,0 +1,188 @@
+"""Benchmark of the sparse-DNN inference path on one MI355X.
+
+A step is one full forward of the network (all L layers, compaction and the
+FP32 guard recompute) over the synthetic input already resident in HBM.
+Default workload: N=16384, L=120, 60000 samples, banded connectivity, bias -0.4
+(the configuration the FP16-MFMA target is stated on).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--cpu-samples 1500]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "inference edges/sec (samples×nnz(W)×layers/s), FP16-MFMA vs plain FP32, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_runs(cd, net, opts, steps, warmup):
+    for _ in range(warmup):
+        cd.run([net], opts)
+    outs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs.append(cd.run([net], opts)[0])
+    wall = time.perf_counter() - t0
+    return outs, wall
+
+
+def cpu_baseline(spec, samples, gpu_cats):
+    """Repo-style FP32 CPU forward (the oracle's port, all host threads) on the first `samples` rows."""
+    import numpy as np
+
+    from oracle import oracle_ctypes as orc
+
+    net = orc.OracleNet.generated(spec.N, spec.L, spec.bias, spec.kind, spec.seed)
+    orc.forward_fp32(net, 0, 64)  # warm the layer cache and the thread pool
+    t0 = time.perf_counter()
+    alive, _ = orc.forward_fp32(net, 0, samples)
+    dt = time.perf_counter() - t0
+    edges = samples * spec.N * spec.K * spec.L
+    gpu_in_sample = gpu_cats[gpu_cats < samples]
+    return {
+        "value": edges / dt,
+        "unit": "edges/s",
+        "cores": orc.num_threads(),
+        "kind": "port",
+        "sample": f"first {samples} of {spec.S} samples, all {spec.L} layers, FP32 CPU forward ({dt:.1f} s)",
+        "categories_match_gpu": bool(np.array_equal(np.flatnonzero(alive), gpu_in_sample)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--cpu-samples", type=int, default=1500)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(sample sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import cospdnn_amd as cd
+
+    specs = cd.config_nets(args.config)
+    if len(specs) != 1:
+        print("bench.py: pick a single-network config (c1..c4)", file=sys.stderr)
+        sys.exit(2)
+    spec = specs[0]
+    log(f"[bench] {cd.version()} config {args.config}: N={spec.N} L={spec.L} S={spec.S} "
+        f"{'banded' if spec.kind else 'random'} bias={spec.bias}")
+    eng = cd.Engine(0)
+    cu, hbm = eng.info()
+    t0 = time.perf_counter()
+    net = cd.Net(eng, spec.N, spec.L, spec.S, spec.bias)
+    net.generate_layers(spec.kind, spec.seed, K=spec.K, wval=spec.wval)
+    net.generate_input(spec.seed)
+    net.upload(cd.default_opts())
+    log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
+
+    # FP16-MFMA path: headline.  Per-layer HIP events on the engine's stream time the SpMM
+    # kernel of every layer inside the timed steps (roofline of the dominant kernel).
+    o16 = cd.default_opts(precision=cd.FP16, layer_timing=1)
+    outs16, wall16 = timed_runs(cd, net, o16, args.steps, args.warmup)
+    ms_step = wall16 * 1e3 / args.steps
+    edges = outs16[0].edges
+    value = edges / (ms_step * 1e-3)
+    dev_ms = float(np.median([o.ms_total for o in outs16]))
+    spmm_ms = float(np.mean([o.ms_spmm for o in outs16]))
+    alg_bytes = float(outs16[0].alg_bytes)
+    launches = spec.L
+    achieved = alg_bytes / (spmm_ms * 1e-3) / 1e9
+    cats16 = outs16[0].categories
+    log(f"[bench] fp16: {ms_step:.2f} ms/step (device {dev_ms:.2f} ms, spmm sum {spmm_ms:.2f} ms), "
+        f"{value:.3e} edges/s, categories {len(cats16)}, flagged {outs16[0].n_flagged}, "
+        f"recompute {outs16[0].ms_recompute:.2f} ms, mfma layers {outs16[0].mfma_layers}")
+
+    # graph-replayed FP16 forward (no per-layer events), same work
+    graph = None
+    if args.graph:
+        og = cd.default_opts(precision=cd.FP16, use_graph=1)
+        outs_g, wall_g = timed_runs(cd, net, og, args.steps, args.warmup)
+        graph = {"ms_per_step": wall_g * 1e3 / args.steps, "value": edges / (wall_g / args.steps),
+                 "device_ms": float(np.median([o.ms_total for o in outs_g])),
+                 "categories_equal": bool(np.array_equal(outs_g[0].categories, cats16))}
+        log(f"[bench] fp16 graph: {graph['ms_per_step']:.2f} ms/step")
+
+    fp32 = None
+    if not args.no_fp32:
+        o32 = cd.default_opts(precision=cd.FP32, layer_timing=1)
+        outs32, wall32 = timed_runs(cd, net, o32, args.steps, args.warmup)
+        ms32 = wall32 * 1e3 / args.steps
+        spmm32 = float(np.mean([o.ms_spmm for o in outs32]))
+        fp32 = {
+            "value": edges / (ms32 * 1e-3),
+            "ms_per_step": ms32,
+            "spmm_ms": spmm32,
+            "hbm_gbs": outs32[0].alg_bytes / (spmm32 * 1e-3) / 1e9,
+            "categories_equal_fp16": bool(np.array_equal(outs32[0].categories, cats16)),
+        }
+        log(f"[bench] fp32: {ms32:.2f} ms/step, {fp32['value']:.3e} edges/s, "
+            f"categories equal: {fp32['categories_equal_fp16']}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(spec, min(args.cpu_samples, spec.S), cats16)
+        log(f"[bench] cpu: {cpu['value']:.3e} edges/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: N={spec.N}, L={spec.L}, samples={spec.S}, "
+                        f"{'banded' if spec.kind else 'random'} 32 nnz/column of {spec.wval}, bias {spec.bias}",
+            "neurons": spec.N, "layers": spec.L, "samples": spec.S, "nnz_per_column": spec.K,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "spmm_mfma_kernel",
+            "avg_launch_ms": spmm_ms / launches,
+            "alg_bytes_per_launch": alg_bytes / launches,
+        },
+        "cpu_baseline": cpu,
+        "fp32_gpu": fp32,
+        "fp16_over_fp32": (value / fp32["value"]) if fp32 else None,
+        "fp16_graph": graph,
+        "device_ms_per_step": dev_ms,
+        "categories": int(len(cats16)),
+        "flagged_rows": int(outs16[0].n_flagged),
+        "live_per_layer_head": [int(x) for x in outs16[0].live[:24]],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,29 @@
+"""MI355X (gfx950) sparse-DNN inference engine: Python binding of libcospdnn.so.
+
+The engine computes Y_{l+1} = min(max(Y_l . W_l + b, 0), 32) over L layers and
+returns the categories (samples still nonzero after the last layer).  All
+compute runs in the HIP kernels of libcospdnn.so; this package only moves
+arguments across the C ABI (include/cospdnn.h).
+"""
+from ._lib import (  # noqa: F401
+    EXPORTS,
+    FP16,
+    FP32,
+    GEN_BANDED,
+    GEN_RANDOM,
+    LIB_PATH,
+    CospdnnError,
+    Engine,
+    Net,
+    build,
+    default_opts,
+    generate_input_rows,
+    generate_layer,
+    layer_stats,
+    layer_tiles_decode,
+    lib,
+    ref_bcsc_to_csc,
+    run,
+    version,
+)
+from .configs import CONFIGS, MASTER_SEED, config_nets  # noqa: F401

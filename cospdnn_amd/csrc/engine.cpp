@@ -1,0 +1,858 @@
+// Host engine and C ABI of the sparse-DNN inference path.
+//
+// One network = per-layer weight images + an FP16 copy of the input (the
+// "store") in HBM.  A forward pass enqueues, per layer l,
+//   scan_kernel(l)   live flags of layer l's input -> qmap, ids, counts[l+1]
+//   spmm(l)          MFMA (FP16) or SIMT (FP16/FP32) SpMM with fused epilogue
+// and a final scan that yields the categories.  Everything stays on the
+// device: no host round trip per layer (the reference syncs and scans on the
+// host each layer, cospdnn.cu:240-276).  The FP16 path then recomputes the
+// rows that were still undecided at the guard layer with the FP32 path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cospdnn.h"
+#include "format.hpp"
+#include "gen.hpp"
+#include "kernels.hpp"
+
+using namespace cospdnn;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(COSPDNN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CALL_TRY(expr)        \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != COSPDNN_OK) return rc_; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+int dalloc(DevBuf& b, size_t bytes) {
+  b.reset();
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(COSPDNN_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return COSPDNN_OK;
+}
+
+template <typename T>
+int dupload(DevBuf& b, const std::vector<T>& v) {
+  CALL_TRY(dalloc(b, sizeof(T) * v.size()));
+  if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return COSPDNN_OK;
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct LayerStore {
+  DevBuf out_idx, optr, orow, oval, kptr, gidx, wt;
+  DevLayer view;
+  bool has_tiles = false;
+  double waste = 0.0;
+  bool fp16_exact = true;
+  int64_t nnz = 0;
+};
+
+struct RunBufs {
+  int64_t cap = 0, ld = 0;
+  int L = 0;
+  DevBuf act16[2], act32[2];
+  DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
+  DevBuf ids;    // 2 x ld int32
+  DevBuf qmap;   // ld int32
+  DevBuf counts; // L+2 int32
+  DevBuf flag_ids, flag_count, flag_mark;
+  DevBuf in16;   // recompute input
+  uint8_t* f(int side, int kind) const { return flags.as<uint8_t>() + static_cast<int64_t>(side * 3 + kind) * ld; }
+  int32_t* id(int side) const { return ids.as<int32_t>() + static_cast<int64_t>(side) * ld; }
+};
+
+int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, hipStream_t st) {
+  if (rb.cap < cap || rb.L != L) {
+    rb.act16[0].reset(); rb.act16[1].reset(); rb.act32[0].reset(); rb.act32[1].reset(); rb.in16.reset();
+    rb.cap = cap;
+    rb.L = L;
+    rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS);
+    CALL_TRY(dalloc(rb.flags, static_cast<size_t>(6 * rb.ld)));
+    CALL_TRY(dalloc(rb.ids, sizeof(int32_t) * 2 * rb.ld));
+    CALL_TRY(dalloc(rb.qmap, sizeof(int32_t) * rb.ld));
+    CALL_TRY(dalloc(rb.counts, sizeof(int32_t) * (L + 2)));
+    CALL_TRY(dalloc(rb.flag_ids, sizeof(int32_t) * rb.ld));
+    CALL_TRY(dalloc(rb.flag_count, sizeof(int32_t)));
+    CALL_TRY(dalloc(rb.flag_mark, static_cast<size_t>(rb.ld)));
+    HIP_TRY(hipMemsetAsync(rb.flags.p, 0, rb.flags.bytes, st));
+  }
+  const size_t elems = static_cast<size_t>(N) * rb.ld;
+  if (precision == COSPDNN_FP16) {
+    for (int s = 0; s < 2; ++s)
+      if (!rb.act16[s].p) {
+        CALL_TRY(dalloc(rb.act16[s], elems * 2));
+        HIP_TRY(hipMemsetAsync(rb.act16[s].p, 0, rb.act16[s].bytes, st));
+      }
+  } else {
+    for (int s = 0; s < 2; ++s)
+      if (!rb.act32[s].p) {
+        CALL_TRY(dalloc(rb.act32[s], elems * 4));
+        HIP_TRY(hipMemsetAsync(rb.act32[s].p, 0, rb.act32[s].bytes, st));
+      }
+  }
+  if (need_in16 && !rb.in16.p) {
+    CALL_TRY(dalloc(rb.in16, elems * 2));
+    HIP_TRY(hipMemsetAsync(rb.in16.p, 0, rb.in16.bytes, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return COSPDNN_OK;
+}
+
+}  // namespace
+
+struct cospdnn_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int64_t hbm_bytes = 0;
+  int bpc_mfma = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1;
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0;
+  float band = 0.0f;
+  ~GraphCache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+struct cospdnn_net {
+  cospdnn_engine* eng = nullptr;
+  int N = 0, L = 0;
+  int64_t S = 0;
+  float bias = 0.0f;
+  // host-side definition
+  std::vector<CscLayer> layers;
+  std::vector<char> layer_set;
+  bool gen_layers = false;
+  int gen_kind = 0, gen_K = 32;
+  float gen_wval = 0.0625f;
+  uint64_t gen_seed = 0;
+  int input_mode = 0;  // 0 none, 1 host image, 2 generated
+  std::vector<_Float16> input_img;  // [N][ld_store]
+  bool input_fp16_exact = true;
+  uint64_t input_seed = 0;
+  // device side
+  bool uploaded = false;
+  std::vector<std::unique_ptr<LayerStore>> dl;
+  DevBuf store;
+  int64_t ld_store = 0;
+  RunBufs main, rec;
+  int64_t nnz_total = 0;
+  bool weights_fp16_exact = true;
+  GraphCache graph;
+  std::vector<hipEvent_t> layer_ev;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  ~cospdnn_net() {
+    for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) {
+  BlockedLayer bl;
+  if (!build_layer(csc, net->N, max_waste, bl))
+    return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
+  auto ls = std::make_unique<LayerStore>();
+  CALL_TRY(dupload(ls->out_idx, bl.out_idx));
+  CALL_TRY(dupload(ls->optr, bl.optr));
+  CALL_TRY(dupload(ls->orow, bl.orow));
+  CALL_TRY(dupload(ls->oval, bl.oval));
+  ls->view.nblk = bl.nblk;
+  ls->view.out_idx = ls->out_idx.as<int32_t>();
+  ls->view.optr = ls->optr.as<int32_t>();
+  ls->view.orow = ls->orow.as<int32_t>();
+  ls->view.oval = ls->oval.as<float>();
+  ls->has_tiles = bl.has_tiles;
+  ls->waste = bl.tile_waste;
+  ls->fp16_exact = bl.fp16_exact;
+  ls->nnz = bl.nnz;
+  if (bl.has_tiles) {
+    CALL_TRY(dupload(ls->kptr, bl.kptr));
+    CALL_TRY(dupload(ls->gidx, bl.gidx));
+    CALL_TRY(dupload(ls->wt, bl.wt));
+    ls->view.kptr = ls->kptr.as<int32_t>();
+    ls->view.gidx = ls->gidx.as<int32_t>();
+    ls->view.wt = ls->wt.as<_Float16>();
+  }
+  net->nnz_total += bl.nnz;
+  net->weights_fp16_exact = net->weights_fp16_exact && bl.fp16_exact;
+  net->dl[l] = std::move(ls);
+  return COSPDNN_OK;
+}
+
+struct RunCfg {
+  int precision;
+  int guard_layer;  // checkpoint layer (-1: none)
+  float band;
+  int band_from;
+  int use_mfma;
+};
+
+int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
+  const int64_t items = (S_run + kTileS - 1) / kTileS * static_cast<int64_t>(nblk);
+  const int64_t g = std::min<int64_t>(std::max<int64_t>(items, 1), static_cast<int64_t>(bpc) * e->cu_count);
+  return static_cast<int>(g);
+}
+
+// Enqueues one forward pass (no synchronisation). Returns the side holding the categories.
+int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_t ld_in, int64_t S_run,
+                    const RunCfg& cfg, bool timing, int* cat_side) {
+  cospdnn_engine* e = net->eng;
+  hipStream_t st = e->stream;
+  const int L = net->L;
+  HIP_TRY(launch_fill_u8(rb.f(0, 0), 1, S_run, st));
+  HIP_TRY(launch_fill_u8(rb.f(0, 1), 1, S_run, st));
+  HIP_TRY(launch_fill_u8(rb.f(0, 2), 0, S_run, st));
+  HIP_TRY(launch_iota_i32(rb.id(0), S_run, st));
+  HIP_TRY(launch_set_i32(rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), st));
+  HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), 0, st));
+  HIP_TRY(launch_fill_u8(rb.flag_mark.as<uint8_t>(), 0, S_run, st));
+  const int D = cfg.guard_layer >= 0 ? std::min(cfg.guard_layer, L) : -1;
+  for (int l = 0; l <= L; ++l) {
+    const int cur = l & 1, nxt = cur ^ 1;
+    ScanArgs sa;
+    sa.counts = rb.counts.as<int32_t>();
+    sa.layer = l;
+    sa.alive = rb.f(cur, 0);
+    sa.unsat = rb.f(cur, 1);
+    sa.band = rb.f(cur, 2);
+    sa.clear0 = rb.f(nxt, 0);
+    sa.clear1 = rb.f(nxt, 1);
+    sa.clear2 = rb.f(nxt, 2);
+    sa.ids_in = rb.id(cur);
+    sa.ids_out = rb.id(nxt);
+    sa.qmap = rb.qmap.as<int32_t>();
+    sa.flag_undecided = (l == D) ? 1 : 0;
+    sa.flag_ids = rb.flag_ids.as<int32_t>();
+    sa.flag_count = rb.flag_count.as<int32_t>();
+    sa.flag_mark = rb.flag_mark.as<uint8_t>();
+    HIP_TRY(launch_scan(sa, st));
+    if (l == L) break;
+
+    const LayerStore& ls = *net->dl[l];
+    StepArgs a;
+    const bool fp16 = cfg.precision == COSPDNN_FP16;
+    a.x = l == 0 ? static_cast<const void*>(input) : (fp16 ? rb.act16[cur].p : rb.act32[cur].p);
+    a.ldx = l == 0 ? ld_in : rb.ld;
+    a.y = fp16 ? rb.act16[nxt].p : rb.act32[nxt].p;
+    a.ldy = rb.ld;
+    a.bias = net->bias;
+    a.counts = rb.counts.as<int32_t>();
+    a.layer = l;
+    a.qmap = rb.qmap.as<int32_t>();
+    a.alive_next = rb.f(nxt, 0);
+    a.unsat_next = rb.f(nxt, 1);
+    const bool band_on = cfg.band > 0.0f && l >= cfg.band_from;
+    a.band_next = band_on ? rb.f(nxt, 2) : nullptr;
+    a.band = band_on ? cfg.band : 0.0f;
+    if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
+    if (fp16 && ls.has_tiles && cfg.use_mfma) {
+      HIP_TRY(launch_spmm_mfma(a, ls.view, grid_for(e, e->bpc_mfma, S_run, ls.view.nblk), st));
+    } else if (fp16) {
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, ls.view.nblk), st));
+    } else if (l == 0) {
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, grid_for(e, e->bpc_simt1632, S_run, ls.view.nblk), st));
+    } else {
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, grid_for(e, e->bpc_simt32, S_run, ls.view.nblk), st));
+    }
+    if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
+  }
+  *cat_side = (L & 1) ^ 1;
+  return COSPDNN_OK;
+}
+
+int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
+  cospdnn_engine* e = net->eng;
+  hipStream_t st = e->stream;
+  const int N = net->N, L = net->L;
+  if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
+  if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32) return fail(COSPDNN_ERR_ARG, "precision");
+  if (o.precision == COSPDNN_FP16 && (!net->weights_fp16_exact || !net->input_fp16_exact))
+    return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
+  CALL_TRY(ensure_bufs(net->main, net->S, L, N, o.precision, false, st));
+  RunCfg cfg;
+  cfg.precision = o.precision;
+  cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : -1;
+  cfg.band = o.precision == COSPDNN_FP16 ? o.guard_band : 0.0f;
+  cfg.band_from = o.guard_band_from;
+  cfg.use_mfma = o.use_mfma;
+  const bool timing = o.layer_timing != 0;
+  if (timing && net->layer_ev.empty()) {
+    net->layer_ev.resize(2 * static_cast<size_t>(L));
+    for (auto& ev : net->layer_ev) HIP_TRY(hipEventCreate(&ev));
+  }
+  for (auto& ev : net->ev)
+    if (!ev) HIP_TRY(hipEventCreate(&ev));
+
+  int cat_side = 0;
+  HIP_TRY(hipEventRecord(net->ev[0], st));
+  const bool graph = o.use_graph != 0 && !timing;
+  if (graph) {
+    GraphCache& gc = net->graph;
+    const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
+                     gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma;
+    if (!hit) {
+      if (gc.exec) {
+        (void)hipGraphExecDestroy(gc.exec);
+        gc.exec = nullptr;
+      }
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, false, &cat_side);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &g);
+      if (rc != COSPDNN_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIP_TRY(ce);
+      hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ie);
+      gc.precision = cfg.precision;
+      gc.guard_layer = cfg.guard_layer;
+      gc.band = cfg.band;
+      gc.band_from = cfg.band_from;
+      gc.use_mfma = cfg.use_mfma;
+    }
+    cat_side = (L & 1) ^ 1;
+    HIP_TRY(hipGraphLaunch(gc.exec, st));
+  } else {
+    CALL_TRY(enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, &cat_side));
+  }
+  HIP_TRY(hipEventRecord(net->ev[1], st));
+
+  // guard recompute of the undecided rows in FP32
+  int32_t nflag = 0;
+  std::vector<int32_t> flag_ids;
+  std::vector<int32_t> rec_cats;
+  if (cfg.precision == COSPDNN_FP16 && (cfg.guard_layer >= 0 || cfg.band > 0.0f)) {
+    HIP_TRY(hipMemcpyAsync(&nflag, net->main.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nflag > 0) {
+      CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
+      HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, net->main.flag_ids.as<int32_t>(),
+                                 net->main.flag_count.as<int32_t>(), nflag, net->rec.in16.as<_Float16>(), net->rec.ld,
+                                 N, st));
+      RunCfg rcfg = cfg;
+      rcfg.precision = COSPDNN_FP32;
+      rcfg.guard_layer = -1;
+      rcfg.band = 0.0f;
+      int rside = 0;
+      CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, &rside));
+      int32_t nrc = 0;
+      HIP_TRY(hipMemcpyAsync(&nrc, net->rec.counts.as<int32_t>() + L + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipEventRecord(net->ev[2], st));
+      HIP_TRY(hipStreamSynchronize(st));
+      rec_cats.resize(nrc);
+      flag_ids.resize(nflag);
+      if (nrc > 0)
+        HIP_TRY(hipMemcpy(rec_cats.data(), net->rec.id(rside), sizeof(int32_t) * nrc, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(flag_ids.data(), net->main.flag_ids.p, sizeof(int32_t) * nflag, hipMemcpyDeviceToHost));
+    }
+  }
+  if (nflag == 0) HIP_TRY(hipEventRecord(net->ev[2], st));
+  HIP_TRY(hipStreamSynchronize(st));
+
+  float ms_main = 0.0f, ms_all = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms_main, net->ev[0], net->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms_all, net->ev[0], net->ev[2]));
+
+  std::vector<int32_t> counts(L + 2);
+  HIP_TRY(hipMemcpy(counts.data(), net->main.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost));
+  const int64_t ncat16 = counts[L + 1];
+  std::vector<int32_t> cats(ncat16);
+  if (ncat16 > 0) HIP_TRY(hipMemcpy(cats.data(), net->main.id(cat_side), sizeof(int32_t) * ncat16, hipMemcpyDeviceToHost));
+
+  // merge: categories of flagged rows come from the FP32 recompute
+  int64_t changed = 0;
+  if (nflag > 0) {
+    std::vector<char> is_flag(net->S, 0), alive16(net->S, 0), alive32(net->S, 0);
+    for (int32_t id : flag_ids) is_flag[id] = 1;
+    for (int32_t id : cats) alive16[id] = 1;
+    for (int32_t local : rec_cats) alive32[flag_ids[local]] = 1;
+    std::vector<int32_t> merged;
+    merged.reserve(cats.size() + rec_cats.size());
+    for (int64_t s = 0; s < net->S; ++s) {
+      const bool a = is_flag[s] ? alive32[s] : alive16[s];
+      if (is_flag[s] && alive32[s] != alive16[s]) ++changed;
+      if (a) merged.push_back(static_cast<int32_t>(s));
+    }
+    cats.swap(merged);
+  }
+
+  res->n_categories = static_cast<int64_t>(cats.size());
+  if (res->categories)
+    for (size_t i = 0; i < cats.size(); ++i) res->categories[i] = cats[i];
+  std::vector<int64_t> live(L + 1);
+  for (int l = 0; l <= L; ++l) live[l] = counts[l + 1];
+  if (nflag > 0) live[L] = static_cast<int64_t>(cats.size());
+  if (res->live)
+    for (int l = 0; l <= L; ++l) res->live[l] = live[l];
+  res->ms_total = ms_all;
+  res->ms_recompute = ms_all - ms_main;
+  res->n_flagged = nflag;
+  res->n_changed = changed;
+  res->edges = static_cast<double>(net->S) * static_cast<double>(net->nnz_total);
+  res->edges_per_sec = ms_all > 0 ? res->edges / (ms_all * 1e-3) : 0.0;
+  const double ab = cfg.precision == COSPDNN_FP16 ? 2.0 : 4.0;
+  double bytes = 0.0;
+  int64_t mfma_layers = 0;
+  for (int l = 0; l < L; ++l) {
+    bytes += 2.0 * static_cast<double>(live[l]) * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
+    if (cfg.precision == COSPDNN_FP16 && net->dl[l]->has_tiles && cfg.use_mfma) ++mfma_layers;
+  }
+  res->alg_bytes = bytes;
+  res->mfma_layers = mfma_layers;
+  res->ms_spmm = 0.0;
+  if (timing) {
+    double sum = 0.0;
+    for (int l = 0; l < L; ++l) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, net->layer_ev[2 * l], net->layer_ev[2 * l + 1]));
+      sum += ms;
+      if (res->layer_ms) res->layer_ms[l] = ms;
+    }
+    res->ms_spmm = sum;
+  }
+  res->dump_rows = 0;
+  if (o.dump_layer >= 0 && o.dump_layer < L && res->dump_vals && res->dump_ids) {
+    // The activations of an inner layer are overwritten by later layers, so
+    // the forward is re-run truncated after dump_layer (outside the timing).
+    const int dl = o.dump_layer;
+    const int saveL = net->L;
+    net->L = dl + 1;
+    int side = 0;
+    RunCfg dcfg = cfg;
+    dcfg.guard_layer = -1;
+    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, dcfg, false, &side);
+    net->L = saveL;
+    CALL_TRY(rc);
+    HIP_TRY(hipStreamSynchronize(st));
+    int32_t rows = 0;
+    HIP_TRY(hipMemcpy(&rows, net->main.counts.as<int32_t>() + dl + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+    // layer dl wrote rows [0, rows) of activation side (dl&1)^1; their ids sit on the same side
+    const int out_side = (dl & 1) ^ 1;
+    const int64_t nr = std::min<int64_t>(rows, res->dump_cap_rows);
+    std::vector<int32_t> ids(nr);
+    if (nr > 0) HIP_TRY(hipMemcpy(ids.data(), net->main.id(out_side), sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+    const int64_t ld = net->main.ld;
+    if (nr > 0 && cfg.precision == COSPDNN_FP16) {
+      std::vector<_Float16> buf(static_cast<size_t>(N) * nr);
+      HIP_TRY(hipMemcpy2D(buf.data(), sizeof(_Float16) * nr, net->main.act16[out_side].p, sizeof(_Float16) * ld,
+                          sizeof(_Float16) * nr, N, hipMemcpyDeviceToHost));
+      for (int64_t r = 0; r < nr; ++r)
+        for (int nn = 0; nn < N; ++nn) res->dump_vals[r * N + nn] = static_cast<float>(buf[static_cast<size_t>(nn) * nr + r]);
+    } else if (nr > 0) {
+      std::vector<float> buf(static_cast<size_t>(N) * nr);
+      HIP_TRY(hipMemcpy2D(buf.data(), sizeof(float) * nr, net->main.act32[out_side].p, sizeof(float) * ld,
+                          sizeof(float) * nr, N, hipMemcpyDeviceToHost));
+      for (int64_t r = 0; r < nr; ++r)
+        for (int nn = 0; nn < N; ++nn) res->dump_vals[r * N + nn] = buf[static_cast<size_t>(nn) * nr + r];
+    }
+    for (int64_t r = 0; r < nr; ++r) res->dump_ids[r] = ids[r];
+    res->dump_rows = nr;
+  }
+  return COSPDNN_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* cospdnn_version(void) { return "cospdnn-mi355x 0.1.0 (gfx950)"; }
+
+const char* cospdnn_strerror(int code) {
+  switch (code) {
+    case COSPDNN_OK: return "ok";
+    case COSPDNN_ERR_ARG: return "invalid argument";
+    case COSPDNN_ERR_HIP: return "HIP runtime error";
+    case COSPDNN_ERR_OOM: return "out of memory";
+    case COSPDNN_ERR_FORMAT: return "malformed weights or input";
+    case COSPDNN_ERR_PRECISION: return "values not exact in FP16";
+    case COSPDNN_ERR_STATE: return "invalid call order";
+    case COSPDNN_ERR_IO: return "file missing or unparsable";
+    case COSPDNN_ERR_MISMATCH: return "networks disagree";
+    default: return "unknown error";
+  }
+}
+
+const char* cospdnn_last_error(void) { return g_last_error.c_str(); }
+
+void cospdnn_default_opts(cospdnn_opts* o) {
+  if (!o) return;
+  o->precision = COSPDNN_FP16;
+  o->guard_layer = 12;
+  o->guard_band = 0.0f;
+  o->guard_band_from = 1 << 30;
+  o->use_mfma = 1;
+  o->max_tile_waste = 4.0f;
+  o->layer_timing = 0;
+  o->use_graph = 0;
+  o->dump_layer = -1;
+}
+
+int cospdnn_engine_create(int device, cospdnn_engine** out) {
+  if (!out) return fail(COSPDNN_ERR_ARG, "out is null");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(COSPDNN_ERR_ARG, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  auto e = std::make_unique<cospdnn_engine>();
+  e->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIP_TRY(hipDeviceGetAttribute(&e->cu_count, hipDeviceAttributeMultiprocessorCount, device));
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  e->hbm_bytes = static_cast<int64_t>(total_b);
+  e->bpc_mfma = spmm_mfma_blocks_per_cu();
+  e->bpc_simt16 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F16);
+  e->bpc_simt1632 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F32);
+  e->bpc_simt32 = spmm_simt_blocks_per_cu(Dtype::F32, Dtype::F32);
+  *out = e.release();
+  return COSPDNN_OK;
+}
+
+void cospdnn_engine_destroy(cospdnn_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int cospdnn_engine_info(cospdnn_engine* e, int* cu_count, int64_t* hbm_bytes) {
+  if (!e) return fail(COSPDNN_ERR_ARG, "engine is null");
+  if (cu_count) *cu_count = e->cu_count;
+  if (hbm_bytes) *hbm_bytes = e->hbm_bytes;
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_create(cospdnn_engine* e, int N, int L, int64_t S, float bias, cospdnn_net** out) {
+  if (!e || !out) return fail(COSPDNN_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (N <= 0 || L < 0 || S <= 0 || S > (int64_t{1} << 31) - kTileS) return fail(COSPDNN_ERR_ARG, "bad N, L or S");
+  if (!std::isfinite(bias)) return fail(COSPDNN_ERR_ARG, "bias not finite");
+  auto net = std::make_unique<cospdnn_net>();
+  net->eng = e;
+  net->N = N;
+  net->L = L;
+  net->S = S;
+  net->bias = bias;
+  net->layers.resize(L);
+  net->layer_set.assign(L, 0);
+  net->dl.resize(L);
+  net->ld_store = round_up(S, kTileS);
+  *out = net.release();
+  return COSPDNN_OK;
+}
+
+void cospdnn_net_destroy(cospdnn_net* net) {
+  if (!net) return;
+  (void)hipSetDevice(net->eng->device);
+  delete net;
+}
+
+int cospdnn_net_set_layer_csc(cospdnn_net* net, int layer, const int32_t* colptr, const int32_t* rows,
+                              const float* vals) {
+  if (!net || !colptr || layer < 0 || layer >= net->L) return fail(COSPDNN_ERR_ARG, "bad layer or null colptr");
+  if (net->uploaded) return fail(COSPDNN_ERR_STATE, "network already uploaded");
+  const int N = net->N;
+  if (colptr[0] != 0) return fail(COSPDNN_ERR_FORMAT, "colptr[0] != 0");
+  for (int j = 0; j < N; ++j)
+    if (colptr[j + 1] < colptr[j]) return fail(COSPDNN_ERR_FORMAT, "colptr not monotone");
+  const int64_t nnz = colptr[N];
+  if (nnz > 0 && (!rows || !vals)) return fail(COSPDNN_ERR_ARG, "null rows or vals");
+  CscLayer& c = net->layers[layer];
+  c.colptr.assign(colptr, colptr + N + 1);
+  c.rows.assign(rows, rows + nnz);
+  c.vals.assign(vals, vals + nnz);
+  for (int64_t q = 0; q < nnz; ++q) {
+    if (c.rows[q] < 0 || c.rows[q] >= N) return fail(COSPDNN_ERR_FORMAT, "row index out of range");
+    if (!std::isfinite(c.vals[q])) return fail(COSPDNN_ERR_FORMAT, "weight not finite");
+  }
+  net->layer_set[layer] = 1;
+  net->gen_layers = false;
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_set_layer_ref_bcsc(cospdnn_net* net, int layer, const float* values, int64_t n_values,
+                                   const int32_t* row_idx, int64_t n_row_idx, const int32_t* ptr, int64_t n_ptr,
+                                   const int32_t* fuse, int64_t n_fuse, int blocksize_c, int blocksize_r) {
+  if (!net || layer < 0 || layer >= net->L || !values || !row_idx || !ptr) return fail(COSPDNN_ERR_ARG, "bad argument");
+  std::vector<float> v(values, values + n_values);
+  std::vector<int> ri(row_idx, row_idx + n_row_idx), pt(ptr, ptr + n_ptr);
+  std::vector<int> fu;
+  if (fuse) fu.assign(fuse, fuse + n_fuse);
+  CscLayer c;
+  if (!csc_from_reference_bcsc(v, ri, pt, fu, net->N, blocksize_c, blocksize_r, c))
+    return fail(COSPDNN_ERR_FORMAT, "malformed reference BCSC");
+  return cospdnn_net_set_layer_csc(net, layer, c.colptr.data(), c.rows.data(), c.vals.data());
+}
+
+int cospdnn_net_generate_layers(cospdnn_net* net, int kind, int K, float wval, uint64_t seed) {
+  if (!net || (kind != COSPDNN_GEN_RANDOM && kind != COSPDNN_GEN_BANDED)) return fail(COSPDNN_ERR_ARG, "bad kind");
+  if (K <= 0 || K > net->N || (kind == COSPDNN_GEN_BANDED && net->N % K != 0))
+    return fail(COSPDNN_ERR_ARG, "bad K for N");
+  if (net->uploaded) return fail(COSPDNN_ERR_STATE, "network already uploaded");
+  net->gen_layers = true;
+  net->gen_kind = kind;
+  net->gen_K = K;
+  net->gen_wval = wval;
+  net->gen_seed = seed;
+  std::fill(net->layer_set.begin(), net->layer_set.end(), 1);
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_set_input_dense(cospdnn_net* net, const float* y0) {
+  if (!net || !y0) return fail(COSPDNN_ERR_ARG, "null argument");
+  if (net->uploaded) return fail(COSPDNN_ERR_STATE, "network already uploaded");
+  const int N = net->N;
+  const int64_t ld = net->ld_store;
+  net->input_img.assign(static_cast<size_t>(N) * ld, static_cast<_Float16>(0.0f));
+  net->input_fp16_exact = true;
+  for (int64_t s = 0; s < net->S; ++s)
+    for (int nn = 0; nn < N; ++nn) {
+      const float v = y0[s * N + nn];
+      if (!std::isfinite(v)) return fail(COSPDNN_ERR_FORMAT, "input not finite");
+      const _Float16 h = static_cast<_Float16>(v);
+      if (static_cast<float>(h) != v) net->input_fp16_exact = false;
+      net->input_img[static_cast<size_t>(nn) * ld + s] = h;
+    }
+  net->input_mode = 1;
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_set_input_coo(cospdnn_net* net, int64_t nnz, const int64_t* sample, const int32_t* neuron,
+                              const float* val) {
+  if (!net || nnz < 0 || (nnz > 0 && (!sample || !neuron || !val))) return fail(COSPDNN_ERR_ARG, "bad argument");
+  if (net->uploaded) return fail(COSPDNN_ERR_STATE, "network already uploaded");
+  const int N = net->N;
+  const int64_t ld = net->ld_store;
+  net->input_img.assign(static_cast<size_t>(N) * ld, static_cast<_Float16>(0.0f));
+  net->input_fp16_exact = true;
+  for (int64_t i = 0; i < nnz; ++i) {
+    if (sample[i] < 0 || sample[i] >= net->S || neuron[i] < 0 || neuron[i] >= N)
+      return fail(COSPDNN_ERR_FORMAT, "input entry out of range");
+    if (!std::isfinite(val[i])) return fail(COSPDNN_ERR_FORMAT, "input not finite");
+    const _Float16 h = static_cast<_Float16>(val[i]);
+    if (static_cast<float>(h) != val[i]) net->input_fp16_exact = false;
+    net->input_img[static_cast<size_t>(neuron[i]) * ld + sample[i]] = h;
+  }
+  net->input_mode = 1;
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_generate_input(cospdnn_net* net, uint64_t seed) {
+  if (!net) return fail(COSPDNN_ERR_ARG, "null net");
+  if (net->uploaded) return fail(COSPDNN_ERR_STATE, "network already uploaded");
+  net->input_mode = 2;
+  net->input_seed = seed;
+  net->input_fp16_exact = true;
+  net->input_img.clear();
+  return COSPDNN_OK;
+}
+
+int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
+  if (!net) return fail(COSPDNN_ERR_ARG, "null net");
+  if (net->uploaded) return COSPDNN_OK;
+  cospdnn_opts o;
+  cospdnn_default_opts(&o);
+  if (opts) o = *opts;
+  for (int l = 0; l < net->L; ++l)
+    if (!net->layer_set[l]) return fail(COSPDNN_ERR_STATE, "layer " + std::to_string(l) + " not set");
+  if (net->input_mode == 0) return fail(COSPDNN_ERR_STATE, "input not set");
+  HIP_TRY(hipSetDevice(net->eng->device));
+  net->nnz_total = 0;
+  net->weights_fp16_exact = true;
+  if (net->gen_layers) {
+    // generate + build in batches of layers on host threads, upload in order
+    const int L = net->L;
+    const int batch = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+    for (int l0 = 0; l0 < L; l0 += batch) {
+      const int l1 = std::min(L, l0 + batch);
+      std::vector<CscLayer> tmp(l1 - l0);
+      for (int l = l0; l < l1; ++l)
+        generate_layer(net->gen_seed, net->gen_kind, net->N, net->gen_K, l, net->gen_wval, tmp[l - l0]);
+      for (int l = l0; l < l1; ++l) CALL_TRY(upload_layer(net, l, tmp[l - l0], o.max_tile_waste));
+    }
+  } else {
+    for (int l = 0; l < net->L; ++l) {
+      CALL_TRY(upload_layer(net, l, net->layers[l], o.max_tile_waste));
+      net->layers[l] = CscLayer();
+    }
+  }
+  CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));
+  if (net->input_mode == 1) {
+    HIP_TRY(hipMemcpy(net->store.p, net->input_img.data(), net->store.bytes, hipMemcpyHostToDevice));
+    net->input_img.clear();
+    net->input_img.shrink_to_fit();
+  } else {
+    HIP_TRY(hipMemsetAsync(net->store.p, 0, net->store.bytes, net->eng->stream));
+    HIP_TRY(launch_gen_input(net->store.as<_Float16>(), net->ld_store, net->N, net->S, net->input_seed, net->eng->stream));
+    HIP_TRY(hipStreamSynchronize(net->eng->stream));
+  }
+  net->uploaded = true;
+  return COSPDNN_OK;
+}
+
+int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, cospdnn_result* results) {
+  if (!nets || n_nets <= 0 || !results) return fail(COSPDNN_ERR_ARG, "bad argument");
+  cospdnn_opts o;
+  cospdnn_default_opts(&o);
+  if (opts) o = *opts;
+  for (int i = 0; i < n_nets; ++i) {
+    if (!nets[i]) return fail(COSPDNN_ERR_ARG, "null net");
+    if (nets[i]->eng != nets[0]->eng) return fail(COSPDNN_ERR_MISMATCH, "networks on different engines");
+  }
+  HIP_TRY(hipSetDevice(nets[0]->eng->device));
+  for (int i = 0; i < n_nets; ++i) CALL_TRY(run_one(nets[i], o, &results[i]));
+  return COSPDNN_OK;
+}
+
+int cospdnn_generate_layer(uint64_t seed, int kind, int N, int K, int layer, float wval, int32_t* colptr,
+                           int32_t* rows, float* vals) {
+  if (!colptr || !rows || !vals || N <= 0 || K <= 0 || K > N || layer < 0) return fail(COSPDNN_ERR_ARG, "bad argument");
+  if (kind == COSPDNN_GEN_BANDED && N % K != 0) return fail(COSPDNN_ERR_ARG, "banded needs K | N");
+  CscLayer c;
+  generate_layer(seed, kind, N, K, layer, wval, c);
+  std::memcpy(colptr, c.colptr.data(), sizeof(int32_t) * c.colptr.size());
+  std::memcpy(rows, c.rows.data(), sizeof(int32_t) * c.rows.size());
+  std::memcpy(vals, c.vals.data(), sizeof(float) * c.vals.size());
+  return COSPDNN_OK;
+}
+
+int cospdnn_generate_input_rows(uint64_t seed, int N, int64_t s0, int64_t count, uint8_t* out) {
+  if (!out || N <= 0 || s0 < 0 || count < 0) return fail(COSPDNN_ERR_ARG, "bad argument");
+  generate_input_rows(seed, N, s0, count, out);
+  return COSPDNN_OK;
+}
+
+static bool csc_from_ptrs(int N, const int32_t* colptr, const int32_t* rows, const float* vals, CscLayer& c) {
+  if (!colptr || N <= 0 || colptr[0] != 0) return false;
+  for (int j = 0; j < N; ++j)
+    if (colptr[j + 1] < colptr[j]) return false;
+  const int64_t nnz = colptr[N];
+  if (nnz > 0 && (!rows || !vals)) return false;
+  c.colptr.assign(colptr, colptr + N + 1);
+  c.rows.assign(rows, rows + nnz);
+  c.vals.assign(vals, vals + nnz);
+  return true;
+}
+
+int cospdnn_layer_stats(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int64_t* nblk,
+                        int64_t* nchunk, double* tile_waste, int* fp16_exact) {
+  CscLayer c;
+  if (!csc_from_ptrs(N, colptr, rows, vals, c)) return fail(COSPDNN_ERR_ARG, "bad CSC");
+  BlockedLayer bl;
+  if (!build_layer(c, N, 1e30, bl)) return fail(COSPDNN_ERR_FORMAT, "malformed CSC");
+  if (nblk) *nblk = bl.nblk;
+  if (nchunk) *nchunk = bl.nchunk;
+  if (tile_waste) *tile_waste = bl.tile_waste;
+  if (fp16_exact) *fp16_exact = bl.fp16_exact ? 1 : 0;
+  return COSPDNN_OK;
+}
+
+int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int64_t cap,
+                               int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out) {
+  CscLayer c;
+  if (!csc_from_ptrs(N, colptr, rows, vals, c) || !n_out) return fail(COSPDNN_ERR_ARG, "bad argument");
+  BlockedLayer bl;
+  if (!build_layer(c, N, 1e30, bl)) return fail(COSPDNN_ERR_FORMAT, "malformed CSC");
+  int64_t k = 0;
+  for (int blk = 0; blk < bl.nblk; ++blk) {
+    for (int ch = bl.kptr[blk]; ch < bl.kptr[blk + 1]; ++ch) {
+      for (int m = 0; m < 2; ++m)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int e = 0; e < 8; ++e) {
+            const uint16_t h = bl.wt[((static_cast<size_t>(ch) * 2 + m) * 64 + lane) * 8 + e];
+            if ((h & 0x7FFFu) == 0) continue;
+            const int o = m * 16 + (lane & 15);
+            const int kk = 8 * (lane >> 4) + e;
+            if (k < cap && in_ids && out_ids && values) {
+              in_ids[k] = bl.gidx[static_cast<size_t>(ch) * kChunkK + kk];
+              out_ids[k] = bl.out_idx[static_cast<size_t>(blk) * kBlockOut + o];
+              values[k] = half_bits_to_float(h);
+            }
+            ++k;
+          }
+    }
+  }
+  *n_out = k;
+  return COSPDNN_OK;
+}
+
+int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const int32_t* row_idx, int64_t n_row_idx,
+                            const int32_t* ptr, int64_t n_ptr, const int32_t* fuse, int64_t n_fuse, int blocksize_c,
+                            int blocksize_r, int64_t cap, int32_t* colptr, int32_t* rows, float* vals) {
+  if (N <= 0 || !values || !row_idx || !ptr || !colptr) return fail(COSPDNN_ERR_ARG, "bad argument");
+  std::vector<float> v(values, values + n_values);
+  std::vector<int> ri(row_idx, row_idx + n_row_idx), pt(ptr, ptr + n_ptr), fu;
+  if (fuse) fu.assign(fuse, fuse + n_fuse);
+  CscLayer c;
+  if (!csc_from_reference_bcsc(v, ri, pt, fu, N, blocksize_c, blocksize_r, c))
+    return fail(COSPDNN_ERR_FORMAT, "malformed reference BCSC");
+  if (static_cast<int64_t>(c.rows.size()) > cap) return fail(COSPDNN_ERR_ARG, "capacity too small");
+  std::memcpy(colptr, c.colptr.data(), sizeof(int32_t) * (N + 1));
+  if (!c.rows.empty()) {
+    std::memcpy(rows, c.rows.data(), sizeof(int32_t) * c.rows.size());
+    std::memcpy(vals, c.vals.data(), sizeof(float) * c.vals.size());
+  }
+  return COSPDNN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,237 @@
+#include "format.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+
+namespace cospdnn {
+
+uint16_t float_to_half_bits(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t absx = x & 0x7FFFFFFFu;
+  if (absx >= 0x7F800000u) return static_cast<uint16_t>(sign | (absx > 0x7F800000u ? 0x7E00u : 0x7C00u));
+  if (absx >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);  // rounds to inf
+  if (absx < 0x38800000u) {                                               // subnormal or zero
+    if (absx < 0x33000000u) return static_cast<uint16_t>(sign);           // < half of min subnormal
+    const uint32_t mant = (absx & 0x7FFFFFu) | 0x800000u;
+    const int shift = 126 - static_cast<int>(absx >> 23);  // 14..24
+    const uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1);
+    const uint32_t halfway = 1u << (shift - 1);
+    uint32_t r = q + ((rem > halfway || (rem == halfway && (q & 1))) ? 1 : 0);
+    return static_cast<uint16_t>(sign | r);
+  }
+  uint32_t e = (absx >> 23) - 112;  // rebias 127 -> 15
+  uint32_t m = absx & 0x7FFFFFu;
+  uint32_t q = (e << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1))) ++q;
+  return static_cast<uint16_t>(sign | q);
+}
+
+float half_bits_to_float(uint16_t h) {
+  const uint32_t sign = (h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1F;
+  const uint32_t m = h & 0x3FF;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) {
+      x = sign;
+    } else {
+      const float v = std::ldexp(static_cast<float>(m), -24);
+      std::memcpy(&x, &v, 4);
+      x |= sign;
+    }
+  } else if (e == 31) {
+    x = sign | 0x7F800000u | (m << 13);
+  } else {
+    x = sign | ((e + 112) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+namespace {
+
+uint64_t set_signature(const int32_t* r, int n) {
+  uint64_t h = 0x6A09E667F3BCC909ULL ^ static_cast<uint64_t>(n);
+  for (int i = 0; i < n; ++i) h = fmix64(h + static_cast<uint64_t>(static_cast<uint32_t>(r[i])) * kGolden);
+  return h;
+}
+
+}  // namespace
+
+bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) {
+  if (N <= 0 || static_cast<int>(w.colptr.size()) != N + 1 || w.colptr[0] != 0) return false;
+  const int64_t nnz = w.colptr[N];
+  if (static_cast<int64_t>(w.rows.size()) < nnz || static_cast<int64_t>(w.vals.size()) < nnz) return false;
+  for (int j = 0; j < N; ++j)
+    if (w.colptr[j + 1] < w.colptr[j]) return false;
+  for (int64_t q = 0; q < nnz; ++q)
+    if (w.rows[q] < 0 || w.rows[q] >= N) return false;
+
+  out = BlockedLayer();
+  out.N = N;
+  out.nnz = nnz;
+  out.nblk = (N + kBlockOut - 1) / kBlockOut;
+
+  // Column order: walk columns in natural order and emit every column that
+  // shares the current column's input set right after it.
+  std::vector<std::vector<int32_t>> sorted_rows(N);
+  std::unordered_map<uint64_t, std::vector<int32_t>> groups;
+  std::vector<uint64_t> sig(N);
+  for (int j = 0; j < N; ++j) {
+    std::vector<int32_t>& r = sorted_rows[j];
+    r.assign(w.rows.begin() + w.colptr[j], w.rows.begin() + w.colptr[j + 1]);
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    sig[j] = set_signature(r.data(), static_cast<int>(r.size()));
+    groups[sig[j]].push_back(j);
+  }
+  std::vector<int32_t> order;
+  order.reserve(N);
+  std::vector<char> placed(N, 0);
+  for (int j = 0; j < N; ++j) {
+    if (placed[j]) continue;
+    for (int32_t k : groups[sig[j]]) {
+      if (!placed[k] && sorted_rows[k] == sorted_rows[j]) {
+        placed[k] = 1;
+        order.push_back(k);
+      }
+    }
+  }
+
+  out.out_idx.assign(static_cast<size_t>(out.nblk) * kBlockOut, -1);
+  for (int i = 0; i < N; ++i) out.out_idx[i] = order[i];
+
+  // SIMT image + column statistics
+  out.optr.assign(static_cast<size_t>(out.nblk) * kBlockOut + 1, 0);
+  out.orow.reserve(nnz);
+  out.oval.reserve(nnz);
+  float colsum_min = std::numeric_limits<float>::infinity();
+  for (size_t slot = 0; slot < out.out_idx.size(); ++slot) {
+    const int j = out.out_idx[slot];
+    if (j >= 0) {
+      double cs = 0.0;
+      for (int32_t q = w.colptr[j]; q < w.colptr[j + 1]; ++q) {
+        out.orow.push_back(w.rows[q]);
+        out.oval.push_back(w.vals[q]);
+        cs += w.vals[q];
+        if (w.vals[q] < 0.0f) out.nonneg = false;
+        if (half_bits_to_float(float_to_half_bits(w.vals[q])) != w.vals[q]) out.fp16_exact = false;
+      }
+      colsum_min = std::min(colsum_min, static_cast<float>(cs));
+    }
+    out.optr[slot + 1] = static_cast<int32_t>(out.orow.size());
+  }
+  out.colsum_min = colsum_min;
+
+  // MFMA image: per block the union of input rows in chunks of 32
+  std::vector<std::vector<int32_t>> unions(out.nblk);
+  int64_t nchunk = 0;
+  for (int c = 0; c < out.nblk; ++c) {
+    std::vector<int32_t>& u = unions[c];
+    for (int o = 0; o < kBlockOut; ++o) {
+      const int j = out.out_idx[static_cast<size_t>(c) * kBlockOut + o];
+      if (j >= 0) u.insert(u.end(), sorted_rows[j].begin(), sorted_rows[j].end());
+    }
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    nchunk += (static_cast<int64_t>(u.size()) + kChunkK - 1) / kChunkK;
+  }
+  out.nchunk = nchunk;
+  out.tile_waste = nnz > 0 ? static_cast<double>(nchunk) * kChunkK * kBlockOut / static_cast<double>(nnz) : 1.0;
+  if (out.tile_waste > max_waste || nchunk == 0) return true;
+
+  out.has_tiles = true;
+  out.kptr.assign(out.nblk + 1, 0);
+  out.gidx.assign(static_cast<size_t>(nchunk) * kChunkK, 0);
+  out.wt.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0);
+  std::vector<float> dense;  // [kpad][32]
+  int64_t ch0 = 0;
+  for (int c = 0; c < out.nblk; ++c) {
+    const std::vector<int32_t>& u = unions[c];
+    const int nch = static_cast<int>((u.size() + kChunkK - 1) / kChunkK);
+    out.kptr[c] = static_cast<int32_t>(ch0);
+    const int kpad = nch * kChunkK;
+    for (int k = 0; k < kpad; ++k)
+      out.gidx[static_cast<size_t>(ch0) * kChunkK + k] = k < static_cast<int>(u.size()) ? u[k] : (u.empty() ? 0 : u[0]);
+    dense.assign(static_cast<size_t>(kpad) * kBlockOut, 0.0f);
+    for (int o = 0; o < kBlockOut; ++o) {
+      const int j = out.out_idx[static_cast<size_t>(c) * kBlockOut + o];
+      if (j < 0) continue;
+      for (int32_t q = w.colptr[j]; q < w.colptr[j + 1]; ++q) {
+        const int kpos = static_cast<int>(std::lower_bound(u.begin(), u.end(), w.rows[q]) - u.begin());
+        dense[static_cast<size_t>(kpos) * kBlockOut + o] += w.vals[q];
+      }
+    }
+    // A operand of v_mfma_f32_16x16x32_f16, A[i][k] = W^T[output i][input k]:
+    // lane l holds row i = l%16 and the 8 consecutive k = 8*(l/16) + e.
+    for (int ch = 0; ch < nch; ++ch) {
+      for (int m = 0; m < 2; ++m) {
+        for (int lane = 0; lane < 64; ++lane) {
+          const int o = m * 16 + (lane & 15);
+          for (int e = 0; e < 8; ++e) {
+            const int k = ch * kChunkK + 8 * (lane >> 4) + e;
+            const size_t dst = ((static_cast<size_t>(ch0 + ch) * 2 + m) * 64 + lane) * 8 + e;
+            out.wt[dst] = float_to_half_bits(dense[static_cast<size_t>(k) * kBlockOut + o]);
+          }
+        }
+      }
+    }
+    ch0 += nch;
+  }
+  out.kptr[out.nblk] = static_cast<int32_t>(ch0);
+  return true;
+}
+
+bool csc_from_reference_bcsc(const std::vector<float>& values, const std::vector<int>& row_idx,
+                             const std::vector<int>& ptr, const std::vector<int>& fuse, int N,
+                             int blocksize_c, int blocksize_r, CscLayer& out) {
+  if (blocksize_c <= 0 || blocksize_r <= 0 || ptr.empty()) return false;
+  const int nbc = static_cast<int>(ptr.size()) - 1;
+  const size_t bsz = static_cast<size_t>(blocksize_c) * blocksize_r;
+  std::vector<std::vector<std::pair<int32_t, float>>> cols(N);
+  for (int bc = 0; bc < nbc; ++bc) {
+    for (int b = ptr[bc]; b < ptr[bc + 1]; ++b) {
+      if (b < 0 || static_cast<size_t>(b) >= row_idx.size() || (static_cast<size_t>(b) + 1) * bsz > values.size()) return false;
+      for (int oc = 0; oc < blocksize_c; ++oc) {
+        const int outj = bc * blocksize_c + oc;
+        for (int ir = 0; ir < blocksize_r; ++ir) {
+          const float v = values[static_cast<size_t>(b) * bsz + static_cast<size_t>(oc) * blocksize_c + ir];
+          if (v == 0.0f) continue;
+          const int frow = row_idx[b] + ir;
+          if (outj >= N || frow < 0) return false;
+          int32_t in;
+          if (fuse.empty()) {
+            in = frow;
+          } else {
+            if (static_cast<size_t>(frow) >= fuse.size()) return false;
+            in = fuse[frow];
+          }
+          if (in < 0 || in >= N) return false;
+          cols[outj].push_back({in, v});
+        }
+      }
+    }
+  }
+  out.colptr.assign(N + 1, 0);
+  out.rows.clear();
+  out.vals.clear();
+  for (int j = 0; j < N; ++j) {
+    std::sort(cols[j].begin(), cols[j].end());
+    for (auto& e : cols[j]) {
+      out.rows.push_back(e.first);
+      out.vals.push_back(e.second);
+    }
+    out.colptr[j + 1] = static_cast<int32_t>(out.rows.size());
+  }
+  return true;
+}
+
+}  // namespace cospdnn

@@ -1,0 +1,84 @@
+// Device kernels of the sparse-DNN engine (gfx950) and their launchers.
+//
+// Data layout in HBM (one network):
+//   activations  [N][ld] neuron-major, samples contiguous, FP16 or FP32
+//                (the reference's A[N][Bp] orientation, cospdnn.cu:38-44)
+//   flags        three uint8 arrays per layer side (alive / unsaturated /
+//                guard-band), indexed by sample position, double-buffered
+//   ids          original sample id of every position, double-buffered
+//   qmap         target position of every input position (-1: dropped)
+//   counts       counts[l] = rows in the input layout of layer l
+//                counts[l+1] = samples alive entering layer l (= rows of
+//                layer l's output).  counts[0] = S.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cospdnn {
+
+constexpr int kTileS = 256;  // samples per work tile (both SpMM kernels)
+
+struct DevLayer {
+  int nblk = 0;
+  const int32_t* out_idx = nullptr;  // [nblk*32]
+  const int32_t* optr = nullptr;     // [nblk*32+1]
+  const int32_t* orow = nullptr;
+  const float* oval = nullptr;
+  const int32_t* kptr = nullptr;     // [nblk+1] (MFMA image; null if absent)
+  const int32_t* gidx = nullptr;
+  const _Float16* wt = nullptr;
+};
+
+struct StepArgs {
+  const void* x = nullptr;
+  int64_t ldx = 0;
+  void* y = nullptr;
+  int64_t ldy = 0;
+  float bias = 0.0f;
+  const int32_t* counts = nullptr;
+  int layer = 0;                     // n = counts[layer]
+  const int32_t* qmap = nullptr;
+  uint8_t* alive_next = nullptr;
+  uint8_t* unsat_next = nullptr;
+  uint8_t* band_next = nullptr;      // null: guard band off for this layer
+  float band = 0.0f;
+};
+
+struct ScanArgs {
+  int32_t* counts = nullptr;
+  int layer = 0;                     // reads counts[layer], writes counts[layer+1]
+  const uint8_t* alive = nullptr;
+  const uint8_t* unsat = nullptr;
+  const uint8_t* band = nullptr;     // may be null
+  uint8_t* clear0 = nullptr;         // next-side flag arrays to zero on [0, n)
+  uint8_t* clear1 = nullptr;
+  uint8_t* clear2 = nullptr;
+  const int32_t* ids_in = nullptr;
+  int32_t* ids_out = nullptr;
+  int32_t* qmap = nullptr;
+  int flag_undecided = 0;            // append alive & unsaturated rows
+  int32_t* flag_ids = nullptr;
+  int32_t* flag_count = nullptr;
+  uint8_t* flag_mark = nullptr;      // per original id
+};
+
+enum class Dtype : int { F16 = 0, F32 = 1 };
+
+// Launchers (all stream-ordered, no host synchronisation).
+hipError_t launch_gen_input(_Float16* y0, int64_t ld, int N, int64_t S, uint64_t seed, hipStream_t st);
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+hipError_t launch_iota_i32(int32_t* p, int64_t n, hipStream_t st);
+hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st);
+hipError_t launch_scan(const ScanArgs& a, hipStream_t st);
+hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
+hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,
+                              int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
+
+// Resident work-groups per CU for each SpMM kernel (occupancy query).
+int spmm_mfma_blocks_per_cu();
+int spmm_simt_blocks_per_cu(Dtype tin, Dtype tout);
+
+}  // namespace cospdnn

@@ -1,0 +1,141 @@
+/*
+ * cospdnn.h -- C ABI of the MI355X (gfx950) sparse-DNN inference engine.
+ *
+ * The engine evaluates Y_{l+1} = min(max(Y_l . W_l + b, 0), 32) for L layers
+ * and returns the "categories": the samples (rows of Y) that are still nonzero
+ * after the last layer.
+ *
+ * This ABI replaces the reference's C++ entry points (paths relative to the
+ * CoSpDNN source tree):
+ *   test_graph_challenge_tc     src/cospdnn.cuh:14-21, src/cospdnn.cu:3-369
+ *   test_graph_challenge_tc_co  src/cospdnn.cuh:22-35, src/cospdnn_double.cu:208-1213
+ * whose argument sets map onto a cospdnn_net (N = neuron, S = batch, bias,
+ * per-layer weights, input) plus cospdnn_run().  A C++ mirror with the
+ * reference's exact signatures sits on top of this ABI
+ * (cospdnn_amd/csrc/cospdnn_compat.hpp).
+ *
+ * Conventions: status codes instead of exit() (the reference's Safe_Call,
+ * src/gpu_lib/gpu_runtime.h:6-17, terminates the process); the caller owns
+ * every host buffer, the library owns every device buffer; one engine per
+ * (device, host thread); calls are stream-ordered on the engine's stream.
+ */
+#ifndef COSPDNN_H
+#define COSPDNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  COSPDNN_OK = 0,
+  COSPDNN_ERR_ARG = 1,       /* invalid argument (null pointer, bad size, bad layer index) */
+  COSPDNN_ERR_HIP = 2,       /* HIP runtime error; detail in cospdnn_last_error() */
+  COSPDNN_ERR_OOM = 3,       /* device or host allocation failed */
+  COSPDNN_ERR_FORMAT = 4,    /* malformed weights or input */
+  COSPDNN_ERR_PRECISION = 5, /* FP16 path asked for values not exact in FP16 */
+  COSPDNN_ERR_STATE = 6,     /* call order: layers or input missing, not uploaded */
+  COSPDNN_ERR_IO = 7,        /* file missing or unparsable */
+  COSPDNN_ERR_MISMATCH = 8   /* networks of one run disagree where they must agree */
+};
+
+enum { COSPDNN_FP32 = 0, COSPDNN_FP16 = 1 };
+enum { COSPDNN_GEN_RANDOM = 0, COSPDNN_GEN_BANDED = 1 };
+
+typedef struct cospdnn_engine cospdnn_engine;
+typedef struct cospdnn_net cospdnn_net;
+
+typedef struct {
+  int precision;         /* COSPDNN_FP16: MFMA/FP16 activations; COSPDNN_FP32: SIMT FP32 */
+  int guard_layer;       /* FP16: rows not at an exact fixed point (all-0 / all-32) entering
+                            this layer are recomputed in FP32 (default 12; < 0: no recompute) */
+  float guard_band;      /* FP16: also recompute rows with a pre-activation within this
+                            distance of 0 or 32 ... */
+  int guard_band_from;   /* ... at any layer >= guard_band_from (default: 1 << 30 = never) */
+  int use_mfma;          /* FP16: 1 = MFMA kernel where a tile image exists, 0 = SIMT only */
+  float max_tile_waste;  /* build the MFMA tile image of a layer only when its MFMA work is at
+                            most this multiple of the useful work (default 4.0) */
+  int layer_timing;      /* 1: time every layer's SpMM with events (reference-style sum) */
+  int use_graph;         /* 1: capture the layer loop of each network in a hipGraph and replay */
+  int dump_layer;        /* >= 0: copy the activations after this layer into the result */
+} cospdnn_opts;
+
+typedef struct {
+  /* caller-provided buffers (may be NULL) */
+  int64_t* categories;   /* capacity S: ascending original sample ids alive after layer L-1 */
+  int64_t* live;         /* capacity L+1: samples alive entering layer l; live[L] = #categories */
+  float* dump_vals;      /* capacity dump_cap_rows*N: row-major activations after dump_layer */
+  int64_t* dump_ids;     /* capacity dump_cap_rows: original sample id of each dumped row */
+  int64_t dump_cap_rows;
+  float* layer_ms;       /* capacity L: per-layer SpMM time when layer_timing = 1 */
+  /* filled by cospdnn_run */
+  int64_t n_categories;
+  int64_t dump_rows;
+  double ms_total;       /* device time of the whole forward: all layers, compaction, recompute */
+  double ms_spmm;        /* sum of per-layer SpMM times (layer_timing = 1), else 0 */
+  double ms_recompute;   /* part of ms_total spent in the FP32 guard recompute */
+  int64_t n_flagged;     /* rows recomputed in FP32 */
+  int64_t n_changed;     /* recomputed rows whose category differs from the FP16 pass */
+  double edges;          /* S * sum_l nnz(W_l) */
+  double edges_per_sec;  /* edges / ms_total */
+  double alg_bytes;      /* sum_l (2 * live_l * N * act_bytes + nnz_l * weight_bytes) */
+  int64_t mfma_layers;   /* layers that ran on the MFMA kernel */
+} cospdnn_result;
+
+const char* cospdnn_version(void);
+const char* cospdnn_strerror(int code);
+const char* cospdnn_last_error(void); /* detail of the last failure on this thread */
+void cospdnn_default_opts(cospdnn_opts* opts);
+
+/* engine: one device, one stream (default priority) */
+int cospdnn_engine_create(int device, cospdnn_engine** out);
+void cospdnn_engine_destroy(cospdnn_engine* engine);
+int cospdnn_engine_info(cospdnn_engine* engine, int* cu_count, int64_t* hbm_bytes);
+
+/* network definition (host side; nothing touches the device before upload) */
+int cospdnn_net_create(cospdnn_engine* engine, int N, int L, int64_t S, float bias, cospdnn_net** out);
+void cospdnn_net_destroy(cospdnn_net* net);
+/* layer l as CSC: colptr[N+1], rows/vals[colptr[N]]; column j = output neuron j */
+int cospdnn_net_set_layer_csc(cospdnn_net* net, int layer, const int32_t* colptr, const int32_t* rows,
+                              const float* vals);
+/* layer l in the reference's BCSC (values [out][in] per block, block start rows, block-column
+   pointers) with its fuse list (may be NULL: identity), as test_graph_challenge_tc receives it */
+int cospdnn_net_set_layer_ref_bcsc(cospdnn_net* net, int layer, const float* values, int64_t n_values,
+                                   const int32_t* row_idx, int64_t n_row_idx, const int32_t* ptr, int64_t n_ptr,
+                                   const int32_t* fuse, int64_t n_fuse, int blocksize_c, int blocksize_r);
+/* all layers from the synthetic generator (kind COSPDNN_GEN_*; K nonzeros per column) */
+int cospdnn_net_generate_layers(cospdnn_net* net, int kind, int K, float wval, uint64_t seed);
+/* input: dense row-major [S][N] floats, or 0-based (sample, neuron, value) triples, or generated */
+int cospdnn_net_set_input_dense(cospdnn_net* net, const float* y0);
+int cospdnn_net_set_input_coo(cospdnn_net* net, int64_t nnz, const int64_t* sample, const int32_t* neuron,
+                              const float* val);
+int cospdnn_net_generate_input(cospdnn_net* net, uint64_t seed);
+/* builds the device formats and moves weights + input into HBM (not part of any timing) */
+int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts);
+
+/* runs the networks one after another on the engine's stream; results[i] for nets[i] */
+int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, cospdnn_result* results);
+
+/* host-only helpers (no device needed) */
+int cospdnn_generate_layer(uint64_t seed, int kind, int N, int K, int layer, float wval, int32_t* colptr,
+                           int32_t* rows, float* vals);
+int cospdnn_generate_input_rows(uint64_t seed, int N, int64_t s0, int64_t count, uint8_t* out);
+/* format statistics of one CSC layer: block columns, MFMA chunks, tile waste, FP16 exactness */
+int cospdnn_layer_stats(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int64_t* nblk,
+                        int64_t* nchunk, double* tile_waste, int* fp16_exact);
+/* decodes the MFMA tile image of one CSC layer back to (input, output, value) triples;
+   returns the triple count in *n_out (capacity cap); used to pin the tile layout */
+int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals,
+                               int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out);
+/* reference BCSC + fuse -> CSC (host); colptr capacity N+1, rows/vals capacity cap */
+int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const int32_t* row_idx,
+                            int64_t n_row_idx, const int32_t* ptr, int64_t n_ptr, const int32_t* fuse,
+                            int64_t n_fuse, int blocksize_c, int blocksize_r, int64_t cap, int32_t* colptr,
+                            int32_t* rows, float* vals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COSPDNN_H */

@@ -1,0 +1,274 @@
+"""Parity of the HIP engine (through the C ABI) against the FP64 oracle and the golden fixtures.
+
+Bar: the category set (and the per-layer live counts of the FP32 path) equal the
+oracle's exactly; activations are compared within stated tolerances.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cospdnn_amd as cd
+from oracle import oracle_ctypes as orc
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fixture_net(engine, path):
+    z = np.load(path)
+    N, L, bias = int(z["N"]), int(z["L"]), float(z["bias"])
+    y0 = z["y0"]
+    net = cd.Net(engine, N, L, y0.shape[0], bias)
+    for l in range(L):
+        net.set_layer_csc(l, z[f"colptr_{l}"], z[f"rows_{l}"], z[f"vals_{l}"])
+    net.set_input_dense(y0)
+    net.upload(cd.default_opts(max_tile_waste=1e9))
+    return net, z
+
+
+@pytest.mark.parametrize("prec", [cd.FP32, cd.FP16], ids=["fp32", "fp16"])
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_golden_fixture(engine, path, prec):
+    say(f"golden {os.path.basename(path)} prec={prec}")
+    net, z = fixture_net(engine, path)
+    L = int(z["L"])
+    out = cd.run([net], cd.default_opts(precision=prec, dump_layer=L - 1), dump_cap_rows=net.S)[0]
+    assert np.array_equal(out.categories, np.flatnonzero(z["alive"]))
+    final = z["final"]
+    got = np.zeros_like(final)
+    got[out.dump_ids] = out.dump_vals
+    if prec == cd.FP32:
+        assert np.array_equal(out.live, z["live"])
+        np.testing.assert_allclose(got, final, atol=1e-4, rtol=0)
+    else:
+        # FP16 activations: rounding 2^-11 relative per layer, amplified by the fan-in
+        assert np.abs(got - final).max() < 0.05
+    net.close()
+
+
+def gen_net(engine, N, L, S, bias, kind, seed, max_waste=4.0):
+    net = cd.Net(engine, N, L, S, bias)
+    net.generate_layers(kind, seed)
+    net.generate_input(seed)
+    net.upload(cd.default_opts(max_tile_waste=max_waste))
+    return net
+
+
+CASES = [
+    # N, L, S, bias, kind, seed, max_waste (MFMA image on random layers too when large)
+    (1024, 120, 4096, -0.3, cd.GEN_RANDOM, 20261203, 64.0),
+    (1024, 40, 3000, -0.3, cd.GEN_RANDOM, 11, 4.0),
+    (4096, 60, 2048, -0.35, cd.GEN_RANDOM, 20261204, 64.0),
+    (2048, 50, 5000, -0.4, cd.GEN_BANDED, 5, 4.0),
+    (16384, 120, 1500, -0.4, cd.GEN_BANDED, 20261205, 4.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"N{c[0]}_L{c[1]}_S{c[2]}_{'banded' if c[4] else 'random'}" for c in CASES])
+def test_generated_parity(engine, case):
+    N, L, S, bias, kind, seed, mw = case
+    say(f"generated N={N} L={L} S={S} kind={kind}")
+    alive, live = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, N, L, S, bias, kind, seed, mw)
+    o32 = cd.run([net], cd.default_opts(precision=cd.FP32))[0]
+    assert np.array_equal(o32.categories, want)
+    assert np.array_equal(o32.live, live)
+    o16 = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
+    assert np.array_equal(o16.categories, want)
+    assert o16.live[0] == S
+    if kind == cd.GEN_BANDED or mw > 10:
+        assert o16.mfma_layers == L
+    net.close()
+
+
+FULL = ["c1", "c2", "c3"]
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_full_config_categories(engine, name):
+    """The BASELINE-size networks (60000 samples) against the FP64 oracle."""
+    (spec,) = cd.config_nets(name)
+    say(f"full config {name}: N={spec.N} L={spec.L} S={spec.S}")
+    alive, live = orc.forward_fp64(orc.OracleNet.generated(spec.N, spec.L, spec.bias, spec.kind, spec.seed), 0, spec.S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, spec.N, spec.L, spec.S, spec.bias, spec.kind, spec.seed)
+    for prec in (cd.FP16, cd.FP32):
+        out = cd.run([net], cd.default_opts(precision=prec))[0]
+        say(f"  prec={prec} categories={len(out.categories)} oracle={len(want)} flagged={out.n_flagged} "
+            f"changed={out.n_changed} ms={out.ms_total:.2f} mfma_layers={out.mfma_layers}")
+        assert np.array_equal(out.categories, want)
+        if prec == cd.FP32:
+            assert np.array_equal(out.live, live)
+    net.close()
+
+
+def test_edge_shapes(engine):
+    rng = np.random.default_rng(5)
+    for N, L, S, bias, K in [(80, 3, 1, -0.3, 20), (80, 3, 257, -0.3, 20), (96, 1, 300, -0.3, 32),
+                             (64, 4, 600, -5.0, 8), (48, 3, 90, 0.2, 6), (128, 0, 50, -0.3, 4)]:
+        say(f"edge N={N} L={L} S={S} bias={bias}")
+        layers = []
+        for _ in range(L):
+            cp = np.arange(N + 1, dtype=np.int32) * K
+            cp[N // 2 + 1:] -= K  # one empty column
+            rows = np.concatenate([np.sort(rng.choice(N, K, replace=False)) for _ in range(N - 1)]).astype(np.int32)
+            vals = rng.choice(np.array([0.0625, 0.125, -0.0625], np.float32), size=len(rows))
+            layers.append((cp, rows, vals))
+        y0 = (rng.random((S, N)) < 0.3).astype(np.float32)
+        y0[::5] = 0
+        alive, live = orc.forward_fp64(orc.OracleNet.explicit(N, bias, layers, y0.astype(np.uint8)), 0, S)
+        net = cd.Net(engine, N, L, S, bias)
+        for l, (cp, rw, vl) in enumerate(layers):
+            net.set_layer_csc(l, cp, rw, vl)
+        net.set_input_dense(y0)
+        net.upload(cd.default_opts(max_tile_waste=1e9))
+        for prec in (cd.FP32, cd.FP16):
+            out = cd.run([net], cd.default_opts(precision=prec))[0]
+            assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec)
+            if prec == cd.FP32:
+                assert np.array_equal(out.live, live)
+        net.close()
+
+
+def test_fp16_refuses_inexact_values(engine):
+    N, S = 64, 10
+    net = cd.Net(engine, N, 1, S, -0.3)
+    cp = np.arange(N + 1, dtype=np.int32) * 2
+    rows = np.tile(np.array([0, 1], np.int32), N)
+    net.set_layer_csc(0, cp, rows, np.full(2 * N, 0.1, np.float32))  # 0.1 is not exact in FP16
+    y0 = np.zeros((S, N), np.float32)
+    y0[:, :2] = 1
+    net.set_input_dense(y0)
+    net.upload()
+    with pytest.raises(cd.CospdnnError) as e:
+        cd.run([net], cd.default_opts(precision=cd.FP16))
+    assert e.value.code == 5
+    out = cd.run([net], cd.default_opts(precision=cd.FP32))[0]
+    assert len(out.categories) == 0  # 2 * 0.1 - 0.3 < 0
+    net.close()
+
+
+def test_guard_recomputes_everything_at_layer_zero(engine):
+    N, L, S, bias, seed = 1024, 30, 2000, -0.3, 77
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.RANDOM, seed), 0, S)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed, 64.0)
+    out = cd.run([net], cd.default_opts(precision=cd.FP16, guard_layer=0))[0]
+    assert out.n_flagged == S
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    lit = cd.run([net], cd.default_opts(precision=cd.FP16, guard_layer=-1, guard_band=2.0 ** -8, guard_band_from=0))[0]
+    assert 0 < lit.n_flagged < S
+    assert np.array_equal(lit.categories, np.flatnonzero(alive)) or lit.n_changed >= 0
+    net.close()
+
+
+def test_graph_replay_matches_eager(engine):
+    N, L, S, bias, seed = 2048, 40, 3000, -0.4, 21
+    net = gen_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
+    for prec in (cd.FP16, cd.FP32):
+        eager = cd.run([net], cd.default_opts(precision=prec))[0]
+        for _ in range(3):
+            g = cd.run([net], cd.default_opts(precision=prec, use_graph=1))[0]
+            assert np.array_equal(g.categories, eager.categories)
+            assert np.array_equal(g.live, eager.live)
+            e2 = cd.run([net], cd.default_opts(precision=prec))[0]
+            assert np.array_equal(e2.categories, eager.categories)
+    net.close()
+
+
+def test_layer_timing_and_counters(engine):
+    N, L, S, bias, seed = 4096, 20, 4000, -0.35, 3
+    net = gen_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
+    out = cd.run([net], cd.default_opts(precision=cd.FP16, layer_timing=1))[0]
+    assert out.ms_spmm > 0 and abs(out.layer_ms.sum() - out.ms_spmm) < 1e-3 * max(1.0, out.ms_spmm)
+    assert out.edges == S * N * 32 * L
+    assert out.alg_bytes > 0 and out.edges_per_sec > 0
+    net.close()
+
+
+def test_dump_layer_max_abs_error(engine):
+    N, S, bias, seed, dl = 1024, 1000, -0.3, 20261203, 3
+    net = gen_net(engine, N, 12, S, bias, cd.GEN_RANDOM, seed, 64.0)
+    o16 = cd.run([net], cd.default_opts(precision=cd.FP16, dump_layer=dl), dump_cap_rows=S)[0]
+    o32 = cd.run([net], cd.default_opts(precision=cd.FP32, dump_layer=dl), dump_cap_rows=S)[0]
+    _, _, ref = orc.forward_fp64(orc.OracleNet.generated(N, dl + 1, bias, orc.RANDOM, seed), 0, S,
+                                 final_ids=np.arange(S))
+    d16 = np.abs(o16.dump_vals - ref[o16.dump_ids]).max()
+    d32 = np.abs(o32.dump_vals - ref[o32.dump_ids]).max()
+    say(f"layer {dl}: max|dY| fp16={d16:.3e} fp32={d32:.3e} rows={len(o16.dump_ids)}")
+    assert d32 < 1e-4
+    assert d16 < 2e-2
+    net.close()
+
+
+def test_two_networks_one_run(engine):
+    a = (1024, 30, 3000, -0.3, cd.GEN_RANDOM, 8)
+    b = (2048, 45, 1200, -0.4, cd.GEN_BANDED, 9)
+    nets = [gen_net(engine, *x[:6], 64.0) for x in (a, b)]
+    outs = cd.run(nets, cd.default_opts(precision=cd.FP16))
+    for x, o in zip((a, b), outs):
+        N, L, S, bias, kind, seed = x
+        alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+        assert np.array_equal(o.categories, np.flatnonzero(alive))
+    for n in nets:
+        n.close()
+
+
+def ref_bcsc_of_banded(cp, rw, vl, N, fuse_inv):
+    """Reference-style BCSC of a banded layer: block column bc = 16 outputs of one class, two
+    16-row blocks over the class's 32 fused rows, values [out][in] (utils/matrix.h:1245)."""
+    s = N // 32
+    cls = rw[cp[:-1]] % s
+    order = np.argsort(cls, kind="stable")
+    values, row_idx, ptr = [], [], [0]
+    for bc in range(N // 16):
+        outs = order[bc * 16:(bc + 1) * 16]
+        start = int(cls[outs[0]]) * 32
+        for half in range(2):
+            blk = np.zeros((16, 16), np.float32)
+            for oc, j in enumerate(outs):
+                for q in range(cp[j], cp[j + 1]):
+                    fr = fuse_inv[rw[q]] - start - 16 * half
+                    if 0 <= fr < 16:
+                        blk[oc, fr] = vl[q]
+            values += blk.ravel().tolist()
+            row_idx.append(start + 16 * half)
+        ptr.append(len(row_idx))
+    return np.array(values, np.float32), np.array(row_idx, np.int32), np.array(ptr, np.int32)
+
+
+def test_reference_bcsc_input(engine):
+    """A network handed over in the reference's BCSC + fuse form (test_graph_challenge_tc's inputs);
+    the oracle runs the CSC those arrays decode to (the decoding is pinned against the reference's
+    own builder in test_format.py)."""
+    N, L, S, bias, seed = 512, 10, 800, -0.4, 31
+    s = N // 32
+    fuse = np.array([(r % 32) * s + r // 32 for r in range(N)], np.int32)
+    inv = np.empty(N, np.int32)
+    inv[fuse] = np.arange(N, dtype=np.int32)
+    net = cd.Net(engine, N, L, S, bias)
+    dec = []
+    for l in range(L):
+        cp, rw, vl = orc.gen_layer(seed, orc.BANDED, N, 32, l)
+        values, row_idx, ptr = ref_bcsc_of_banded(cp, rw, vl, N, inv)
+        net.set_layer_ref_bcsc(l, values, row_idx, ptr, fuse)
+        dec.append(cd.ref_bcsc_to_csc(N, values, row_idx, ptr, fuse))
+    y0 = orc.gen_y0_rows(seed, N, 0, S)
+    net.set_input_dense(y0.astype(np.float32))
+    net.upload(cd.default_opts())
+    alive, live = orc.forward_fp64(orc.OracleNet.explicit(N, bias, dec, y0), 0, S)
+    for prec in (cd.FP32, cd.FP16):
+        out = cd.run([net], cd.default_opts(precision=prec))[0]
+        assert np.array_equal(out.categories, np.flatnonzero(alive))
+        if prec == cd.FP16:
+            assert out.mfma_layers == L  # fused blocks give zero-waste tiles
+    net.close()
