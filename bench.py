@@ -108,8 +108,9 @@ def main():
     achieved = alg_bytes / (spmm_ms * 1e-3) / 1e9
     cats16 = outs16[0].categories
     log(f"[bench] fp16: {ms_step:.2f} ms/step (device {dev_ms:.2f} ms, spmm sum {spmm_ms:.2f} ms), "
-        f"{value:.3e} edges/s, categories {len(cats16)}, flagged {outs16[0].n_flagged}, "
-        f"recompute {outs16[0].ms_recompute:.2f} ms, mfma layers {outs16[0].mfma_layers}")
+        f"{value:.3e} edges/s, categories {len(cats16)}, recomputed {outs16[0].n_flagged}, "
+        f"replayed {outs16[0].n_replayed}, guard tiers {outs16[0].ms_recompute:.2f} ms, "
+        f"mfma layers {outs16[0].mfma_layers}")
 
     # graph-replayed FP16 forward (no per-layer events), same work
     graph = None
@@ -178,7 +179,9 @@ def main():
         "fp16_graph": graph,
         "device_ms_per_step": dev_ms,
         "categories": int(len(cats16)),
-        "flagged_rows": int(outs16[0].n_flagged),
+        "recomputed_rows_fp32": int(outs16[0].n_flagged),
+        "replayed_rows_fp64": int(outs16[0].n_replayed),
+        "guard_tiers_ms": float(outs16[0].ms_recompute),
         "live_per_layer_head": [int(x) for x in outs16[0].live[:24]],
     }
     print(json.dumps(line), flush=True)

@@ -9,6 +9,7 @@ from ._lib import (  # noqa: F401
     EXPORTS,
     FP16,
     FP32,
+    FP64,
     GEN_BANDED,
     GEN_RANDOM,
     LIB_PATH,

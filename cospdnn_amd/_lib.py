@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libcospdnn.so")
 
 FP32 = 0
 FP16 = 1
+FP64 = 2
 GEN_RANDOM = 0
 GEN_BANDED = 1
 
@@ -28,6 +29,8 @@ class Opts(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int),
         ("guard_layer", ctypes.c_int),
+        ("guard_layer32", ctypes.c_int),
+        ("fp32_mfma", ctypes.c_int),
         ("guard_band", ctypes.c_float),
         ("guard_band_from", ctypes.c_int),
         ("use_mfma", ctypes.c_int),
@@ -52,6 +55,7 @@ class Result(ctypes.Structure):
         ("ms_spmm", ctypes.c_double),
         ("ms_recompute", ctypes.c_double),
         ("n_flagged", ctypes.c_int64),
+        ("n_replayed", ctypes.c_int64),
         ("n_changed", ctypes.c_int64),
         ("edges", ctypes.c_double),
         ("edges_per_sec", ctypes.c_double),
@@ -273,7 +277,7 @@ def run(nets, opts=None, dump_cap_rows=0):
         out.categories = cats[: r.n_categories].copy()
         out.live = live.copy()
         out.layer_ms = lms[: nets[i].L].copy()
-        for f in ("ms_total", "ms_spmm", "ms_recompute", "n_flagged", "n_changed", "edges", "edges_per_sec",
+        for f in ("ms_total", "ms_spmm", "ms_recompute", "n_flagged", "n_replayed", "n_changed", "edges", "edges_per_sec",
                   "alg_bytes", "mfma_layers", "dump_rows"):
             setattr(out, f, getattr(r, f))
         if dv is not None:

@@ -86,7 +86,7 @@ int dupload(DevBuf& b, const std::vector<T>& v) {
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct LayerStore {
-  DevBuf out_idx, optr, orow, oval, kptr, gidx, wt;
+  DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32;
   DevLayer view;
   bool has_tiles = false;
   double waste = 0.0;
@@ -103,6 +103,7 @@ struct RunBufs {
   DevBuf qmap;   // ld int32
   DevBuf counts; // L+2 int32
   DevBuf flag_ids, flag_count, flag_mark;
+  DevBuf work;   // MFMA work-queue counter
   DevBuf in16;   // recompute input
   uint8_t* f(int side, int kind) const { return flags.as<uint8_t>() + static_cast<int64_t>(side * 3 + kind) * ld; }
   int32_t* id(int side) const { return ids.as<int32_t>() + static_cast<int64_t>(side) * ld; }
@@ -121,6 +122,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     CALL_TRY(dalloc(rb.flag_ids, sizeof(int32_t) * rb.ld));
     CALL_TRY(dalloc(rb.flag_count, sizeof(int32_t)));
     CALL_TRY(dalloc(rb.flag_mark, static_cast<size_t>(rb.ld)));
+    CALL_TRY(dalloc(rb.work, sizeof(int32_t)));
     HIP_TRY(hipMemsetAsync(rb.flags.p, 0, rb.flags.bytes, st));
   }
   const size_t elems = static_cast<size_t>(N) * rb.ld;
@@ -152,12 +154,12 @@ struct cospdnn_engine {
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int64_t hbm_bytes = 0;
-  int bpc_mfma = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1;
+  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1;
 };
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0;
   float band = 0.0f;
   ~GraphCache() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -189,6 +191,8 @@ struct cospdnn_net {
   int64_t nnz_total = 0;
   bool weights_fp16_exact = true;
   GraphCache graph;
+  DevBuf dev_layers;                 // DevLayer[L] for the FP64 replay
+  DevBuf rp_scratch, rp_ids, rp_alive;
   std::vector<hipEvent_t> layer_ev;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   ~cospdnn_net() {
@@ -222,6 +226,8 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
     CALL_TRY(dupload(ls->kptr, bl.kptr));
     CALL_TRY(dupload(ls->gidx, bl.gidx));
     CALL_TRY(dupload(ls->wt, bl.wt));
+    CALL_TRY(dupload(ls->wt32, bl.wt32));
+    ls->view.wt32 = ls->wt32.as<float>();
     ls->view.kptr = ls->kptr.as<int32_t>();
     ls->view.gidx = ls->gidx.as<int32_t>();
     ls->view.wt = ls->wt.as<_Float16>();
@@ -232,12 +238,15 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
   return COSPDNN_OK;
 }
 
+constexpr int kMfmaGroupsPerCu = 4;  // 36 KB of LDS each: four fit in the 160 KB of a CU
+
 struct RunCfg {
-  int precision;
-  int guard_layer;  // checkpoint layer (-1: none)
+  int precision;    // COSPDNN_FP16 or COSPDNN_FP32 for a layer-major pass
+  int guard_layer;  // checkpoint layer: rows not all-0/all-32 entering it are flagged (-1: none)
   float band;
   int band_from;
-  int use_mfma;
+  int use_mfma;     // FP16 pass: MFMA kernel for tiled layers
+  int mfma32;       // FP32 pass: MFMA FP32 kernel for tiled layers
 };
 
 int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
@@ -246,7 +255,8 @@ int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   return static_cast<int>(g);
 }
 
-// Enqueues one forward pass (no synchronisation). Returns the side holding the categories.
+// Enqueues one layer-major pass (no synchronisation). *cat_side receives the ids side
+// holding the categories after the final scan.
 int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_t ld_in, int64_t S_run,
                     const RunCfg& cfg, bool timing, int* cat_side) {
   cospdnn_engine* e = net->eng;
@@ -255,11 +265,14 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   HIP_TRY(launch_fill_u8(rb.f(0, 0), 1, S_run, st));
   HIP_TRY(launch_fill_u8(rb.f(0, 1), 1, S_run, st));
   HIP_TRY(launch_fill_u8(rb.f(0, 2), 0, S_run, st));
+  if (L == 0) HIP_TRY(launch_input_alive(input, ld_in, net->N, S_run, rb.f(0, 0), st));
   HIP_TRY(launch_iota_i32(rb.id(0), S_run, st));
+  HIP_TRY(launch_fill_i32(rb.counts.as<int32_t>() + 1, 0, L + 1, st));  // targets of the scans' atomics
   HIP_TRY(launch_set_i32(rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), 0, st));
   HIP_TRY(launch_fill_u8(rb.flag_mark.as<uint8_t>(), 0, S_run, st));
   const int D = cfg.guard_layer >= 0 ? std::min(cfg.guard_layer, L) : -1;
+  const bool fp16 = cfg.precision == COSPDNN_FP16;
   for (int l = 0; l <= L; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     ScanArgs sa;
@@ -278,12 +291,12 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.flag_ids = rb.flag_ids.as<int32_t>();
     sa.flag_count = rb.flag_count.as<int32_t>();
     sa.flag_mark = rb.flag_mark.as<uint8_t>();
-    HIP_TRY(launch_scan(sa, st));
+    sa.work = rb.work.as<int32_t>();
+    HIP_TRY(launch_scan(sa, S_run, st));
     if (l == L) break;
 
     const LayerStore& ls = *net->dl[l];
     StepArgs a;
-    const bool fp16 = cfg.precision == COSPDNN_FP16;
     a.x = l == 0 ? static_cast<const void*>(input) : (fp16 ? rb.act16[cur].p : rb.act32[cur].p);
     a.ldx = l == 0 ? ld_in : rb.ld;
     a.y = fp16 ? rb.act16[nxt].p : rb.act32[nxt].p;
@@ -297,15 +310,21 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     const bool band_on = cfg.band > 0.0f && l >= cfg.band_from;
     a.band_next = band_on ? rb.f(nxt, 2) : nullptr;
     a.band = band_on ? cfg.band : 0.0f;
+    a.work = rb.work.as<int32_t>();
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
+    const int nblk = ls.view.nblk;
     if (fp16 && ls.has_tiles && cfg.use_mfma) {
-      HIP_TRY(launch_spmm_mfma(a, ls.view, grid_for(e, e->bpc_mfma, S_run, ls.view.nblk), st));
+      // items come from a counter, so groups that cannot be resident at once just find less work
+      HIP_TRY(launch_spmm_mfma(a, ls.view, grid_for(e, kMfmaGroupsPerCu, S_run, nblk), st));
     } else if (fp16) {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, ls.view.nblk), st));
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, nblk), st));
+    } else if (ls.has_tiles && cfg.mfma32) {
+      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
+      HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk), st));
     } else if (l == 0) {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, grid_for(e, e->bpc_simt1632, S_run, ls.view.nblk), st));
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, grid_for(e, e->bpc_simt1632, S_run, nblk), st));
     } else {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, grid_for(e, e->bpc_simt32, S_run, ls.view.nblk), st));
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, grid_for(e, e->bpc_simt32, S_run, nblk), st));
     }
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
   }
@@ -313,145 +332,220 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   return COSPDNN_OK;
 }
 
+// Host view of a finished pass (call after the stream is synchronised).
+struct PassOut {
+  std::vector<int32_t> counts;  // L+2
+  std::vector<int32_t> cats;    // ids (pass-local) alive after the last layer
+  std::vector<int32_t> flags;   // ids (pass-local) flagged at the checkpoint
+};
+
+int read_pass(const cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out) {
+  const int L = net->L;
+  out.counts.assign(L + 2, 0);
+  HIP_TRY(hipMemcpy(out.counts.data(), rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost));
+  out.cats.assign(out.counts[L + 1], 0);
+  if (!out.cats.empty())
+    HIP_TRY(hipMemcpy(out.cats.data(), rb.id(cat_side), sizeof(int32_t) * out.cats.size(), hipMemcpyDeviceToHost));
+  std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
+  int32_t nf = 0;
+  HIP_TRY(hipMemcpy(&nf, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  out.flags.assign(nf, 0);
+  if (nf > 0) HIP_TRY(hipMemcpy(out.flags.data(), rb.flag_ids.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost));
+  std::sort(out.flags.begin(), out.flags.end());
+  return COSPDNN_OK;
+}
+
+// FP64 replay of the given original sample ids; alive[i] for ids[i].  Batched so the
+// scratch ([rows][2][N] doubles) stays within 2 GiB.
+int replay_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uint8_t>& alive) {
+  hipStream_t st = net->eng->stream;
+  alive.assign(ids.size(), 0);
+  if (ids.empty()) return COSPDNN_OK;
+  const int64_t per_row = 2 * static_cast<int64_t>(net->N) * static_cast<int64_t>(sizeof(double));
+  const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ids.size()), (int64_t{1} << 31) / per_row));
+  if (net->rp_scratch.bytes < static_cast<size_t>(batch * per_row)) CALL_TRY(dalloc(net->rp_scratch, batch * per_row));
+  if (net->rp_ids.bytes < sizeof(int32_t) * ids.size()) CALL_TRY(dalloc(net->rp_ids, sizeof(int32_t) * ids.size()));
+  if (net->rp_alive.bytes < ids.size()) CALL_TRY(dalloc(net->rp_alive, ids.size()));
+  HIP_TRY(hipMemcpyAsync(net->rp_ids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, st));
+  for (int64_t r0 = 0; r0 < static_cast<int64_t>(ids.size()); r0 += batch) {
+    ReplayArgs ra;
+    ra.layers = net->dev_layers.as<DevLayer>();
+    ra.L = net->L;
+    ra.N = net->N;
+    ra.bias = net->bias;
+    ra.store = net->store.as<_Float16>();
+    ra.ld = net->ld_store;
+    ra.rows = static_cast<int>(std::min<int64_t>(batch, static_cast<int64_t>(ids.size()) - r0));
+    ra.row_ids = net->rp_ids.as<int32_t>() + r0;
+    ra.map = nullptr;
+    ra.scratch = net->rp_scratch.as<double>();
+    ra.alive = net->rp_alive.as<uint8_t>() + r0;
+    HIP_TRY(launch_replay_fp64(ra, st));
+  }
+  HIP_TRY(hipMemcpyAsync(alive.data(), net->rp_alive.p, ids.size(), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return COSPDNN_OK;
+}
+
 int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   cospdnn_engine* e = net->eng;
   hipStream_t st = e->stream;
   const int N = net->N, L = net->L;
+  const int64_t S = net->S;
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
-  if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32) return fail(COSPDNN_ERR_ARG, "precision");
+  if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
+    return fail(COSPDNN_ERR_ARG, "precision");
   if (o.precision == COSPDNN_FP16 && (!net->weights_fp16_exact || !net->input_fp16_exact))
     return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
-  CALL_TRY(ensure_bufs(net->main, net->S, L, N, o.precision, false, st));
-  RunCfg cfg;
-  cfg.precision = o.precision;
-  cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : -1;
-  cfg.band = o.precision == COSPDNN_FP16 ? o.guard_band : 0.0f;
-  cfg.band_from = o.guard_band_from;
-  cfg.use_mfma = o.use_mfma;
-  const bool timing = o.layer_timing != 0;
-  if (timing && net->layer_ev.empty()) {
-    net->layer_ev.resize(2 * static_cast<size_t>(L));
+  if (o.precision != COSPDNN_FP16 && !net->input_fp16_exact)
+    return fail(COSPDNN_ERR_PRECISION, "inputs must be exactly representable in FP16 (binary inputs are)");
+  const bool timing = o.layer_timing != 0 && o.precision != COSPDNN_FP64;
+  if (timing && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
+    for (hipEvent_t ev : net->layer_ev) (void)hipEventDestroy(ev);
+    net->layer_ev.assign(2 * static_cast<size_t>(L), nullptr);
     for (auto& ev : net->layer_ev) HIP_TRY(hipEventCreate(&ev));
   }
   for (auto& ev : net->ev)
     if (!ev) HIP_TRY(hipEventCreate(&ev));
 
-  int cat_side = 0;
-  HIP_TRY(hipEventRecord(net->ev[0], st));
-  const bool graph = o.use_graph != 0 && !timing;
-  if (graph) {
-    GraphCache& gc = net->graph;
-    const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
-                     gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma;
-    if (!hit) {
-      if (gc.exec) {
-        (void)hipGraphExecDestroy(gc.exec);
-        gc.exec = nullptr;
-      }
-      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, false, &cat_side);
-      hipGraph_t g = nullptr;
-      hipError_t ce = hipStreamEndCapture(st, &g);
-      if (rc != COSPDNN_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      HIP_TRY(ce);
-      hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      HIP_TRY(ie);
-      gc.precision = cfg.precision;
-      gc.guard_layer = cfg.guard_layer;
-      gc.band = cfg.band;
-      gc.band_from = cfg.band_from;
-      gc.use_mfma = cfg.use_mfma;
-    }
-    cat_side = (L & 1) ^ 1;
-    HIP_TRY(hipGraphLaunch(gc.exec, st));
-  } else {
-    CALL_TRY(enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, &cat_side));
-  }
-  HIP_TRY(hipEventRecord(net->ev[1], st));
+  std::vector<char> alive(S, 0), first(S, 0);
+  std::vector<int64_t> live(L + 1, -1);
+  int64_t n_flagged = 0, n_replayed = 0;
+  RunCfg cfg;
+  cfg.precision = o.precision == COSPDNN_FP16 ? COSPDNN_FP16 : COSPDNN_FP32;
+  cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : o.guard_layer32;
+  cfg.band = o.precision == COSPDNN_FP16 ? o.guard_band : 0.0f;
+  cfg.band_from = o.guard_band_from;
+  cfg.use_mfma = o.use_mfma;
+  cfg.mfma32 = o.fp32_mfma;
 
-  // guard recompute of the undecided rows in FP32
-  int32_t nflag = 0;
-  std::vector<int32_t> flag_ids;
-  std::vector<int32_t> rec_cats;
-  if (cfg.precision == COSPDNN_FP16 && (cfg.guard_layer >= 0 || cfg.band > 0.0f)) {
-    HIP_TRY(hipMemcpyAsync(&nflag, net->main.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipEventRecord(net->ev[0], st));
+  if (o.precision == COSPDNN_FP64) {
+    std::vector<int32_t> ids(S);
+    for (int64_t s = 0; s < S; ++s) ids[s] = static_cast<int32_t>(s);
+    std::vector<uint8_t> a64;
+    CALL_TRY(replay_rows(net, ids, a64));
+    for (int64_t s = 0; s < S; ++s) alive[s] = static_cast<char>(a64[s]);
+    first = alive;
+    live[0] = S;
+    n_replayed = S;
+    HIP_TRY(hipEventRecord(net->ev[1], st));
+  } else {
+    CALL_TRY(ensure_bufs(net->main, S, L, N, cfg.precision, false, st));
+    int cat_side = (L & 1) ^ 1;
+    const bool graph = o.use_graph != 0 && !timing;
+    if (graph) {
+      GraphCache& gc = net->graph;
+      const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
+                       gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma &&
+                       gc.mfma32 == cfg.mfma32;
+      if (!hit) {
+        if (gc.exec) {
+          (void)hipGraphExecDestroy(gc.exec);
+          gc.exec = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int side = 0;
+        int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, cfg, false, &side);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        if (rc != COSPDNN_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        HIP_TRY(ce);
+        hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(ie);
+        gc.precision = cfg.precision;
+        gc.guard_layer = cfg.guard_layer;
+        gc.band = cfg.band;
+        gc.band_from = cfg.band_from;
+        gc.use_mfma = cfg.use_mfma;
+        gc.mfma32 = cfg.mfma32;
+      }
+      HIP_TRY(hipGraphLaunch(gc.exec, st));
+    } else {
+      CALL_TRY(enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, cfg, timing, &cat_side));
+    }
+    HIP_TRY(hipEventRecord(net->ev[1], st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (nflag > 0) {
+    PassOut p1;
+    CALL_TRY(read_pass(net, net->main, cat_side, p1));
+    for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
+    for (int32_t id : p1.cats) alive[id] = 1;
+    first = alive;
+
+    std::vector<int32_t> replay_ids;
+    if (cfg.precision == COSPDNN_FP16 && !p1.flags.empty()) {
+      // tier 2: the flagged rows again, layer-major in FP32 (MFMA FP32 on tiled layers)
+      const int32_t nflag = static_cast<int32_t>(p1.flags.size());
+      n_flagged = nflag;
       CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
+      HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
       HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, net->main.flag_ids.as<int32_t>(),
                                  net->main.flag_count.as<int32_t>(), nflag, net->rec.in16.as<_Float16>(), net->rec.ld,
                                  N, st));
-      RunCfg rcfg = cfg;
+      RunCfg rcfg;
       rcfg.precision = COSPDNN_FP32;
-      rcfg.guard_layer = -1;
+      rcfg.guard_layer = o.guard_layer32;
       rcfg.band = 0.0f;
+      rcfg.band_from = 1 << 30;
+      rcfg.use_mfma = 0;
+      rcfg.mfma32 = 1;
       int rside = 0;
       CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, &rside));
-      int32_t nrc = 0;
-      HIP_TRY(hipMemcpyAsync(&nrc, net->rec.counts.as<int32_t>() + L + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipEventRecord(net->ev[2], st));
       HIP_TRY(hipStreamSynchronize(st));
-      rec_cats.resize(nrc);
-      flag_ids.resize(nflag);
-      if (nrc > 0)
-        HIP_TRY(hipMemcpy(rec_cats.data(), net->rec.id(rside), sizeof(int32_t) * nrc, hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(flag_ids.data(), net->main.flag_ids.p, sizeof(int32_t) * nflag, hipMemcpyDeviceToHost));
+      PassOut p2;
+      CALL_TRY(read_pass(net, net->rec, rside, p2));
+      for (int32_t id : p1.flags) alive[id] = 0;
+      for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
+      for (int32_t local : p2.flags) replay_ids.push_back(p1.flags[local]);
+    } else if (cfg.precision == COSPDNN_FP32) {
+      replay_ids = p1.flags;
+    }
+    if (!replay_ids.empty()) {
+      // tier 3: FP64 replay, bit-identical to the FP64 oracle
+      std::sort(replay_ids.begin(), replay_ids.end());
+      std::vector<uint8_t> a64;
+      CALL_TRY(replay_rows(net, replay_ids, a64));
+      for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
+      n_replayed = static_cast<int64_t>(replay_ids.size());
     }
   }
-  if (nflag == 0) HIP_TRY(hipEventRecord(net->ev[2], st));
+  HIP_TRY(hipEventRecord(net->ev[2], st));
   HIP_TRY(hipStreamSynchronize(st));
-
-  float ms_main = 0.0f, ms_all = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms_main, net->ev[0], net->ev[1]));
+  float ms_first = 0.0f, ms_all = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms_first, net->ev[0], net->ev[1]));
   HIP_TRY(hipEventElapsedTime(&ms_all, net->ev[0], net->ev[2]));
 
-  std::vector<int32_t> counts(L + 2);
-  HIP_TRY(hipMemcpy(counts.data(), net->main.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost));
-  const int64_t ncat16 = counts[L + 1];
-  std::vector<int32_t> cats(ncat16);
-  if (ncat16 > 0) HIP_TRY(hipMemcpy(cats.data(), net->main.id(cat_side), sizeof(int32_t) * ncat16, hipMemcpyDeviceToHost));
-
-  // merge: categories of flagged rows come from the FP32 recompute
-  int64_t changed = 0;
-  if (nflag > 0) {
-    std::vector<char> is_flag(net->S, 0), alive16(net->S, 0), alive32(net->S, 0);
-    for (int32_t id : flag_ids) is_flag[id] = 1;
-    for (int32_t id : cats) alive16[id] = 1;
-    for (int32_t local : rec_cats) alive32[flag_ids[local]] = 1;
-    std::vector<int32_t> merged;
-    merged.reserve(cats.size() + rec_cats.size());
-    for (int64_t s = 0; s < net->S; ++s) {
-      const bool a = is_flag[s] ? alive32[s] : alive16[s];
-      if (is_flag[s] && alive32[s] != alive16[s]) ++changed;
-      if (a) merged.push_back(static_cast<int32_t>(s));
+  int64_t ncat = 0, changed = 0;
+  for (int64_t s = 0; s < S; ++s) {
+    if (alive[s]) {
+      if (res->categories) res->categories[ncat] = s;
+      ++ncat;
     }
-    cats.swap(merged);
+    changed += alive[s] != first[s];
   }
-
-  res->n_categories = static_cast<int64_t>(cats.size());
-  if (res->categories)
-    for (size_t i = 0; i < cats.size(); ++i) res->categories[i] = cats[i];
-  std::vector<int64_t> live(L + 1);
-  for (int l = 0; l <= L; ++l) live[l] = counts[l + 1];
-  if (nflag > 0) live[L] = static_cast<int64_t>(cats.size());
+  live[L] = ncat;
+  res->n_categories = ncat;
   if (res->live)
     for (int l = 0; l <= L; ++l) res->live[l] = live[l];
   res->ms_total = ms_all;
-  res->ms_recompute = ms_all - ms_main;
-  res->n_flagged = nflag;
+  res->ms_recompute = ms_all - ms_first;
+  res->n_flagged = n_flagged;
+  res->n_replayed = n_replayed;
   res->n_changed = changed;
-  res->edges = static_cast<double>(net->S) * static_cast<double>(net->nnz_total);
+  res->edges = static_cast<double>(S) * static_cast<double>(net->nnz_total);
   res->edges_per_sec = ms_all > 0 ? res->edges / (ms_all * 1e-3) : 0.0;
-  const double ab = cfg.precision == COSPDNN_FP16 ? 2.0 : 4.0;
+  const double ab = o.precision == COSPDNN_FP16 ? 2.0 : (o.precision == COSPDNN_FP32 ? 4.0 : 8.0);
   double bytes = 0.0;
   int64_t mfma_layers = 0;
   for (int l = 0; l < L; ++l) {
-    bytes += 2.0 * static_cast<double>(live[l]) * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
-    if (cfg.precision == COSPDNN_FP16 && net->dl[l]->has_tiles && cfg.use_mfma) ++mfma_layers;
+    const double lv = live[l] >= 0 ? static_cast<double>(live[l]) : static_cast<double>(S);
+    bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
+    if (net->dl[l]->has_tiles && ((o.precision == COSPDNN_FP16 && cfg.use_mfma) || (o.precision == COSPDNN_FP32 && cfg.mfma32)))
+      ++mfma_layers;
   }
   res->alg_bytes = bytes;
   res->mfma_layers = mfma_layers;
@@ -467,7 +561,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
     res->ms_spmm = sum;
   }
   res->dump_rows = 0;
-  if (o.dump_layer >= 0 && o.dump_layer < L && res->dump_vals && res->dump_ids) {
+  if (o.dump_layer >= 0 && o.dump_layer < L && res->dump_vals && res->dump_ids && o.precision != COSPDNN_FP64) {
     // The activations of an inner layer are overwritten by later layers, so
     // the forward is re-run truncated after dump_layer (outside the timing).
     const int dl = o.dump_layer;
@@ -476,7 +570,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
     int side = 0;
     RunCfg dcfg = cfg;
     dcfg.guard_layer = -1;
-    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, dcfg, false, &side);
+    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, dcfg, false, &side);
     net->L = saveL;
     CALL_TRY(rc);
     HIP_TRY(hipStreamSynchronize(st));
@@ -537,6 +631,8 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   if (!o) return;
   o->precision = COSPDNN_FP16;
   o->guard_layer = 12;
+  o->guard_layer32 = 20;
+  o->fp32_mfma = 0;
   o->guard_band = 0.0f;
   o->guard_band_from = 1 << 30;
   o->use_mfma = 1;
@@ -560,10 +656,14 @@ int cospdnn_engine_create(int device, cospdnn_engine** out) {
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b));
   e->hbm_bytes = static_cast<int64_t>(total_b);
+  HIP_TRY(configure_kernels());
   e->bpc_mfma = spmm_mfma_blocks_per_cu();
+  e->bpc_mfma_resident = spmm_mfma_resident_per_cu();
   e->bpc_simt16 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F16);
   e->bpc_simt1632 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F32);
   e->bpc_simt32 = spmm_simt_blocks_per_cu(Dtype::F32, Dtype::F32);
+  e->bpc_m32h = spmm_mfma32_blocks_per_cu(Dtype::F16);
+  e->bpc_m32f = spmm_mfma32_blocks_per_cu(Dtype::F32);
   *out = e.release();
   return COSPDNN_OK;
 }
@@ -735,6 +835,11 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
       CALL_TRY(upload_layer(net, l, net->layers[l], o.max_tile_waste));
       net->layers[l] = CscLayer();
     }
+  }
+  {
+    std::vector<DevLayer> views(std::max(net->L, 1));
+    for (int l = 0; l < net->L; ++l) views[l] = net->dl[l]->view;
+    CALL_TRY(dupload(net->dev_layers, views));
   }
   CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));
   if (net->input_mode == 1) {

@@ -152,6 +152,7 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
   out.kptr.assign(out.nblk + 1, 0);
   out.gidx.assign(static_cast<size_t>(nchunk) * kChunkK, 0);
   out.wt.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0);
+  out.wt32.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0.0f);
   std::vector<float> dense;  // [kpad][32]
   int64_t ch0 = 0;
   for (int c = 0; c < out.nblk; ++c) {
@@ -184,6 +185,17 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
         }
       }
     }
+    // A operand of v_mfma_f32_16x16x4f32, per K step kk of 4 inputs:
+    // lane l holds row i = l%16 and k = 4*kk + l/16.
+    for (int ch = 0; ch < nch; ++ch)
+      for (int m = 0; m < 2; ++m)
+        for (int kk = 0; kk < 8; ++kk)
+          for (int lane = 0; lane < 64; ++lane) {
+            const int o = m * 16 + (lane & 15);
+            const int k = ch * kChunkK + 4 * kk + (lane >> 4);
+            const size_t dst = ((static_cast<size_t>(ch0 + ch) * 2 + m) * 8 + kk) * 64 + lane;
+            out.wt32[dst] = dense[static_cast<size_t>(k) * kBlockOut + o];
+          }
     ch0 += nch;
   }
   out.kptr[out.nblk] = static_cast<int32_t>(ch0);

@@ -39,7 +39,8 @@ struct BlockedLayer {
   bool has_tiles = false;
   std::vector<int32_t> kptr;   // nblk+1, offsets in chunks
   std::vector<int32_t> gidx;   // nchunk*32
-  std::vector<uint16_t> wt;    // nchunk*1024 FP16 bit patterns
+  std::vector<uint16_t> wt;    // nchunk*1024 FP16 bit patterns (v_mfma_f32_16x16x32_f16 A order)
+  std::vector<float> wt32;     // nchunk*1024 FP32 values (v_mfma_f32_16x16x4f32 A order)
   int64_t nchunk = 0;
   double tile_waste = 0.0;     // MFMA MACs per useful MAC (nchunk*32*32 / nnz)
   bool fp16_exact = true;      // every weight survives FP16 round trip

@@ -28,6 +28,7 @@ struct DevLayer {
   const int32_t* kptr = nullptr;     // [nblk+1] (MFMA image; null if absent)
   const int32_t* gidx = nullptr;
   const _Float16* wt = nullptr;
+  const float* wt32 = nullptr;       // FP32 tile image (MFMA FP32 kernel)
 };
 
 struct StepArgs {
@@ -43,6 +44,7 @@ struct StepArgs {
   uint8_t* unsat_next = nullptr;
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
   float band = 0.0f;
+  int32_t* work = nullptr;           // work-item counter of the MFMA kernel (zeroed by the scan)
 };
 
 struct ScanArgs {
@@ -61,9 +63,24 @@ struct ScanArgs {
   int32_t* flag_ids = nullptr;
   int32_t* flag_count = nullptr;
   uint8_t* flag_mark = nullptr;      // per original id
+  int32_t* work = nullptr;           // reset to 0 for the next SpMM's work queue (may be null)
 };
 
 enum class Dtype : int { F16 = 0, F32 = 1 };
+
+// FP64 replay of single rows through every layer (one work-group per row).
+struct ReplayArgs {
+  const DevLayer* layers = nullptr;  // device array of L layer views
+  int L = 0, N = 0;
+  float bias = 0.0f;
+  const _Float16* store = nullptr;   // input [N][ld]
+  int64_t ld = 0;
+  int rows = 0;
+  const int32_t* row_ids = nullptr;  // rows to replay (indices into `map` when map != null)
+  const int32_t* map = nullptr;      // optional: original sample id of row_ids[r]
+  double* scratch = nullptr;         // [rows][2][N]
+  uint8_t* alive = nullptr;          // [rows] result
+};
 
 // Launchers (all stream-ordered, no host synchronisation).
 hipError_t launch_gen_input(_Float16* y0, int64_t ld, int N, int64_t S, uint64_t seed, hipStream_t st);
@@ -71,14 +88,25 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota_i32(int32_t* p, int64_t n, hipStream_t st);
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st);
-hipError_t launch_scan(const ScanArgs& a, hipStream_t st);
+// max_rows: upper bound of counts[layer] (the grid covers it; counts[layer+1] must be 0 beforehand)
+hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st);
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
+hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
+hipError_t launch_replay_fp64(const ReplayArgs& a, hipStream_t st);
+hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,
                               int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
 
+// Kernel attributes set once per process (dynamic LDS limits).
+hipError_t configure_kernels();
+
 // Resident work-groups per CU for each SpMM kernel (occupancy query).
 int spmm_mfma_blocks_per_cu();
+// Work-groups of the MFMA kernel that fit one CU by its VGPR and LDS use (gfx950: 512 VGPRs per
+// lane and SIMD, 160 KB LDS per CU).
+int spmm_mfma_resident_per_cu();
+int spmm_mfma32_blocks_per_cu(Dtype tin);
 int spmm_simt_blocks_per_cu(Dtype tin, Dtype tout);
 
 }  // namespace cospdnn

@@ -40,16 +40,22 @@ enum {
   COSPDNN_ERR_MISMATCH = 8   /* networks of one run disagree where they must agree */
 };
 
-enum { COSPDNN_FP32 = 0, COSPDNN_FP16 = 1 };
+enum { COSPDNN_FP32 = 0, COSPDNN_FP16 = 1, COSPDNN_FP64 = 2 };
 enum { COSPDNN_GEN_RANDOM = 0, COSPDNN_GEN_BANDED = 1 };
 
 typedef struct cospdnn_engine cospdnn_engine;
 typedef struct cospdnn_net cospdnn_net;
 
 typedef struct {
-  int precision;         /* COSPDNN_FP16: MFMA/FP16 activations; COSPDNN_FP32: SIMT FP32 */
+  int precision;         /* COSPDNN_FP16: MFMA with FP16 activations; COSPDNN_FP32: SIMT FP32;
+                            COSPDNN_FP64: every row replayed in FP64 (bit-identical to the oracle) */
   int guard_layer;       /* FP16: rows not at an exact fixed point (all-0 / all-32) entering
                             this layer are recomputed in FP32 (default 12; < 0: no recompute) */
+  int guard_layer32;     /* FP32 passes (the FP32 path and the FP16 path's recompute): rows not at
+                            an exact fixed point entering this layer are replayed in FP64
+                            (default 20; < 0: no replay) */
+  int fp32_mfma;         /* FP32 path: 1 = MFMA FP32 kernel where a tile image exists, 0 = plain
+                            SIMT gather (default 0; the FP16 path's recompute always uses MFMA) */
   float guard_band;      /* FP16: also recompute rows with a pre-activation within this
                             distance of 0 or 32 ... */
   int guard_band_from;   /* ... at any layer >= guard_band_from (default: 1 << 30 = never) */
@@ -75,8 +81,9 @@ typedef struct {
   double ms_total;       /* device time of the whole forward: all layers, compaction, recompute */
   double ms_spmm;        /* sum of per-layer SpMM times (layer_timing = 1), else 0 */
   double ms_recompute;   /* part of ms_total spent in the FP32 guard recompute */
-  int64_t n_flagged;     /* rows recomputed in FP32 */
-  int64_t n_changed;     /* recomputed rows whose category differs from the FP16 pass */
+  int64_t n_flagged;     /* rows recomputed in FP32 (FP16 path) */
+  int64_t n_replayed;    /* rows replayed in FP64 */
+  int64_t n_changed;     /* recomputed or replayed rows whose category differs from the first pass */
   double edges;          /* S * sum_l nnz(W_l) */
   double edges_per_sec;  /* edges / ms_total */
   double alg_bytes;      /* sum_l (2 * live_l * N * act_bytes + nnz_l * weight_bytes) */

@@ -55,6 +55,13 @@ def test_golden_fixture(engine, path, prec):
     net.close()
 
 
+def assert_live_close(got, want, out):
+    """Live counts come from the first pass; rows it decides differently from FP64 (all of them
+    recomputed or replayed, so the categories stay exact) can shift a layer's count by a few."""
+    assert got[0] == want[0] and got[-1] == want[-1]
+    assert np.abs(got - want).max() <= out.n_flagged + out.n_replayed + max(3, int(got[0]) // 1000)
+
+
 def gen_net(engine, N, L, S, bias, kind, seed, max_waste=4.0):
     net = cd.Net(engine, N, L, S, bias)
     net.generate_layers(kind, seed)
@@ -82,10 +89,11 @@ def test_generated_parity(engine, case):
     net = gen_net(engine, N, L, S, bias, kind, seed, mw)
     o32 = cd.run([net], cd.default_opts(precision=cd.FP32))[0]
     assert np.array_equal(o32.categories, want)
-    assert np.array_equal(o32.live, live)
+    assert_live_close(o32.live, live, o32)
     o16 = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
     assert np.array_equal(o16.categories, want)
     assert o16.live[0] == S
+    assert_live_close(o16.live, live, o16)
     if kind == cd.GEN_BANDED or mw > 10:
         assert o16.mfma_layers == L
     net.close()
@@ -105,10 +113,9 @@ def test_full_config_categories(engine, name):
     for prec in (cd.FP16, cd.FP32):
         out = cd.run([net], cd.default_opts(precision=prec))[0]
         say(f"  prec={prec} categories={len(out.categories)} oracle={len(want)} flagged={out.n_flagged} "
-            f"changed={out.n_changed} ms={out.ms_total:.2f} mfma_layers={out.mfma_layers}")
+            f"replayed={out.n_replayed} changed={out.n_changed} ms={out.ms_total:.2f} mfma_layers={out.mfma_layers}")
         assert np.array_equal(out.categories, want)
-        if prec == cd.FP32:
-            assert np.array_equal(out.live, live)
+        assert_live_close(out.live, live, out)
     net.close()
 
 
@@ -132,11 +139,11 @@ def test_edge_shapes(engine):
             net.set_layer_csc(l, cp, rw, vl)
         net.set_input_dense(y0)
         net.upload(cd.default_opts(max_tile_waste=1e9))
-        for prec in (cd.FP32, cd.FP16):
+        for prec in (cd.FP32, cd.FP16, cd.FP64):
             out = cd.run([net], cd.default_opts(precision=prec))[0]
             assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec)
-            if prec == cd.FP32:
-                assert np.array_equal(out.live, live)
+            if prec == cd.FP32 and L > 0:
+                assert_live_close(out.live, live, out)
         net.close()
 
 
@@ -271,4 +278,29 @@ def test_reference_bcsc_input(engine):
         assert np.array_equal(out.categories, np.flatnonzero(alive))
         if prec == cd.FP16:
             assert out.mfma_layers == L  # fused blocks give zero-waste tiles
+    net.close()
+
+
+def test_fp64_replay_matches_oracle(engine):
+    """Every row replayed in FP64 (the exact tier) equals the FP64 oracle, banded and random."""
+    for N, L, S, bias, kind, seed in [(1024, 60, 700, -0.3, cd.GEN_RANDOM, 12), (2048, 40, 500, -0.4, cd.GEN_BANDED, 13)]:
+        alive, live = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+        net = gen_net(engine, N, L, S, bias, kind, seed)
+        out = cd.run([net], cd.default_opts(precision=cd.FP64))[0]
+        assert out.n_replayed == S
+        assert np.array_equal(out.categories, np.flatnonzero(alive))
+        net.close()
+
+
+def test_fp32_pass_replays_undecided_rows(engine):
+    """guard_layer32=0 sends every row of the FP32 path through the FP64 replay."""
+    N, L, S, bias, seed = 2048, 30, 900, -0.4, 14
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.BANDED, seed), 0, S)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
+    out = cd.run([net], cd.default_opts(precision=cd.FP32, guard_layer32=0))[0]
+    assert out.n_replayed == S
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    m32 = cd.run([net], cd.default_opts(precision=cd.FP32, fp32_mfma=1))[0]
+    assert m32.mfma_layers == L
+    assert np.array_equal(m32.categories, np.flatnonzero(alive))
     net.close()
