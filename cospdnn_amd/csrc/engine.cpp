@@ -97,7 +97,7 @@ struct LayerStore {
 struct RunBufs {
   int64_t cap = 0, ld = 0;
   int L = 0;
-  DevBuf act16[2], act32[2];
+  DevBuf act16[2], act32[2], act64[2];
   DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
   DevBuf ids;    // 2 x ld int32
   DevBuf qmap;   // ld int32
@@ -111,7 +111,12 @@ struct RunBufs {
 
 int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, hipStream_t st) {
   if (rb.cap < cap || rb.L != L) {
-    rb.act16[0].reset(); rb.act16[1].reset(); rb.act32[0].reset(); rb.act32[1].reset(); rb.in16.reset();
+    for (int s = 0; s < 2; ++s) {
+      rb.act16[s].reset();
+      rb.act32[s].reset();
+      rb.act64[s].reset();
+    }
+    rb.in16.reset();
     rb.cap = cap;
     rb.L = L;
     rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS);
@@ -132,11 +137,17 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
         CALL_TRY(dalloc(rb.act16[s], elems * 2));
         HIP_TRY(hipMemsetAsync(rb.act16[s].p, 0, rb.act16[s].bytes, st));
       }
-  } else {
+  } else if (precision == COSPDNN_FP32) {
     for (int s = 0; s < 2; ++s)
       if (!rb.act32[s].p) {
         CALL_TRY(dalloc(rb.act32[s], elems * 4));
         HIP_TRY(hipMemsetAsync(rb.act32[s].p, 0, rb.act32[s].bytes, st));
+      }
+  } else {
+    for (int s = 0; s < 2; ++s)
+      if (!rb.act64[s].p) {
+        CALL_TRY(dalloc(rb.act64[s], elems * 8));
+        HIP_TRY(hipMemsetAsync(rb.act64[s].p, 0, rb.act64[s].bytes, st));
       }
   }
   if (need_in16 && !rb.in16.p) {
@@ -154,7 +165,8 @@ struct cospdnn_engine {
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int64_t hbm_bytes = 0;
-  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1;
+  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
+      bpc_simt1664 = 1, bpc_simt64 = 1;
 };
 
 struct GraphCache {
@@ -191,8 +203,7 @@ struct cospdnn_net {
   int64_t nnz_total = 0;
   bool weights_fp16_exact = true;
   GraphCache graph;
-  DevBuf dev_layers;                 // DevLayer[L] for the FP64 replay
-  DevBuf rp_scratch, rp_ids, rp_alive;
+  RunBufs rep;                       // FP64 pass over the rows the FP32 pass leaves undecided
   std::vector<hipEvent_t> layer_ev;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   ~cospdnn_net() {
@@ -231,6 +242,7 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
     ls->view.kptr = ls->kptr.as<int32_t>();
     ls->view.gidx = ls->gidx.as<int32_t>();
     ls->view.wt = ls->wt.as<_Float16>();
+    ls->view.one_chunk = bl.nchunk == bl.nblk ? 1 : 0;
   }
   net->nnz_total += bl.nnz;
   net->weights_fp16_exact = net->weights_fp16_exact && bl.fp16_exact;
@@ -238,10 +250,10 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
   return COSPDNN_OK;
 }
 
-constexpr int kMfmaGroupsPerCu = 4;  // 36 KB of LDS each: four fit in the 160 KB of a CU
+constexpr int kMfmaGroupsPerCu = 4;  // __launch_bounds__(256, 4): four groups of four waves fit a CU
 
 struct RunCfg {
-  int precision;    // COSPDNN_FP16 or COSPDNN_FP32 for a layer-major pass
+  int precision;    // COSPDNN_FP16, COSPDNN_FP32 or COSPDNN_FP64 for a layer-major pass
   int guard_layer;  // checkpoint layer: rows not all-0/all-32 entering it are flagged (-1: none)
   float band;
   int band_from;
@@ -297,9 +309,11 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
 
     const LayerStore& ls = *net->dl[l];
     StepArgs a;
-    a.x = l == 0 ? static_cast<const void*>(input) : (fp16 ? rb.act16[cur].p : rb.act32[cur].p);
+    const bool fp64 = cfg.precision == COSPDNN_FP64;
+    auto side_ptr = [&](int sd) -> void* { return fp16 ? rb.act16[sd].p : (fp64 ? rb.act64[sd].p : rb.act32[sd].p); };
+    a.x = l == 0 ? static_cast<const void*>(input) : side_ptr(cur);
     a.ldx = l == 0 ? ld_in : rb.ld;
-    a.y = fp16 ? rb.act16[nxt].p : rb.act32[nxt].p;
+    a.y = side_ptr(nxt);
     a.ldy = rb.ld;
     a.bias = net->bias;
     a.counts = rb.counts.as<int32_t>();
@@ -314,10 +328,15 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
     const int nblk = ls.view.nblk;
     if (fp16 && ls.has_tiles && cfg.use_mfma) {
-      // items come from a counter, so groups that cannot be resident at once just find less work
-      HIP_TRY(launch_spmm_mfma(a, ls.view, grid_for(e, kMfmaGroupsPerCu, S_run, nblk), st));
+      // items are strided over the waves of a grid that is resident at once
+      const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
+      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{kMfmaGroupsPerCu} * e->cu_count)));
+      HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
     } else if (fp16) {
       HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, nblk), st));
+    } else if (fp64) {
+      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F64;
+      HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk), st));
     } else if (ls.has_tiles && cfg.mfma32) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
       HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk), st));
@@ -355,35 +374,32 @@ int read_pass(const cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& 
   return COSPDNN_OK;
 }
 
-// FP64 replay of the given original sample ids; alive[i] for ids[i].  Batched so the
-// scratch ([rows][2][N] doubles) stays within 2 GiB.
-int replay_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uint8_t>& alive) {
+// Layer-major FP64 pass over the given original sample ids (bit-identical to the FP64
+// oracle); alive[i] for ids[i].
+int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uint8_t>& alive) {
   hipStream_t st = net->eng->stream;
   alive.assign(ids.size(), 0);
   if (ids.empty()) return COSPDNN_OK;
-  const int64_t per_row = 2 * static_cast<int64_t>(net->N) * static_cast<int64_t>(sizeof(double));
-  const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(ids.size()), (int64_t{1} << 31) / per_row));
-  if (net->rp_scratch.bytes < static_cast<size_t>(batch * per_row)) CALL_TRY(dalloc(net->rp_scratch, batch * per_row));
-  if (net->rp_ids.bytes < sizeof(int32_t) * ids.size()) CALL_TRY(dalloc(net->rp_ids, sizeof(int32_t) * ids.size()));
-  if (net->rp_alive.bytes < ids.size()) CALL_TRY(dalloc(net->rp_alive, ids.size()));
-  HIP_TRY(hipMemcpyAsync(net->rp_ids.p, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, st));
-  for (int64_t r0 = 0; r0 < static_cast<int64_t>(ids.size()); r0 += batch) {
-    ReplayArgs ra;
-    ra.layers = net->dev_layers.as<DevLayer>();
-    ra.L = net->L;
-    ra.N = net->N;
-    ra.bias = net->bias;
-    ra.store = net->store.as<_Float16>();
-    ra.ld = net->ld_store;
-    ra.rows = static_cast<int>(std::min<int64_t>(batch, static_cast<int64_t>(ids.size()) - r0));
-    ra.row_ids = net->rp_ids.as<int32_t>() + r0;
-    ra.map = nullptr;
-    ra.scratch = net->rp_scratch.as<double>();
-    ra.alive = net->rp_alive.as<uint8_t>() + r0;
-    HIP_TRY(launch_replay_fp64(ra, st));
-  }
-  HIP_TRY(hipMemcpyAsync(alive.data(), net->rp_alive.p, ids.size(), hipMemcpyDeviceToHost, st));
+  const int32_t nrows = static_cast<int32_t>(ids.size());
+  RunBufs& rb = net->rep;
+  CALL_TRY(ensure_bufs(rb, nrows, net->L, net->N, COSPDNN_FP64, true, st));
+  HIP_TRY(hipMemcpyAsync(rb.flag_ids.p, ids.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), nrows, st));
+  HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, rb.flag_ids.as<int32_t>(),
+                             rb.flag_count.as<int32_t>(), nrows, rb.in16.as<_Float16>(), rb.ld, net->N, st));
+  RunCfg cfg;
+  cfg.precision = COSPDNN_FP64;
+  cfg.guard_layer = -1;
+  cfg.band = 0.0f;
+  cfg.band_from = 1 << 30;
+  cfg.use_mfma = 0;
+  cfg.mfma32 = 0;
+  int side = 0;
+  CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, &side));
   HIP_TRY(hipStreamSynchronize(st));
+  PassOut p;
+  CALL_TRY(read_pass(net, rb, side, p));
+  for (int32_t local : p.cats) alive[local] = 1;
   return COSPDNN_OK;
 }
 
@@ -399,7 +415,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
     return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
   if (o.precision != COSPDNN_FP16 && !net->input_fp16_exact)
     return fail(COSPDNN_ERR_PRECISION, "inputs must be exactly representable in FP16 (binary inputs are)");
-  const bool timing = o.layer_timing != 0 && o.precision != COSPDNN_FP64;
+  const bool timing = o.layer_timing != 0;
   if (timing && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
     for (hipEvent_t ev : net->layer_ev) (void)hipEventDestroy(ev);
     net->layer_ev.assign(2 * static_cast<size_t>(L), nullptr);
@@ -412,25 +428,15 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   std::vector<int64_t> live(L + 1, -1);
   int64_t n_flagged = 0, n_replayed = 0;
   RunCfg cfg;
-  cfg.precision = o.precision == COSPDNN_FP16 ? COSPDNN_FP16 : COSPDNN_FP32;
-  cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : o.guard_layer32;
+  cfg.precision = o.precision;
+  cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : (o.precision == COSPDNN_FP32 ? o.guard_layer32 : -1);
   cfg.band = o.precision == COSPDNN_FP16 ? o.guard_band : 0.0f;
   cfg.band_from = o.guard_band_from;
   cfg.use_mfma = o.use_mfma;
   cfg.mfma32 = o.fp32_mfma;
 
   HIP_TRY(hipEventRecord(net->ev[0], st));
-  if (o.precision == COSPDNN_FP64) {
-    std::vector<int32_t> ids(S);
-    for (int64_t s = 0; s < S; ++s) ids[s] = static_cast<int32_t>(s);
-    std::vector<uint8_t> a64;
-    CALL_TRY(replay_rows(net, ids, a64));
-    for (int64_t s = 0; s < S; ++s) alive[s] = static_cast<char>(a64[s]);
-    first = alive;
-    live[0] = S;
-    n_replayed = S;
-    HIP_TRY(hipEventRecord(net->ev[1], st));
-  } else {
+  {
     CALL_TRY(ensure_bufs(net->main, S, L, N, cfg.precision, false, st));
     int cat_side = (L & 1) ^ 1;
     const bool graph = o.use_graph != 0 && !timing;
@@ -508,7 +514,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       // tier 3: FP64 replay, bit-identical to the FP64 oracle
       std::sort(replay_ids.begin(), replay_ids.end());
       std::vector<uint8_t> a64;
-      CALL_TRY(replay_rows(net, replay_ids, a64));
+      CALL_TRY(fp64_rows(net, replay_ids, a64));
       for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
       n_replayed = static_cast<int64_t>(replay_ids.size());
     }
@@ -662,6 +668,8 @@ int cospdnn_engine_create(int device, cospdnn_engine** out) {
   e->bpc_simt16 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F16);
   e->bpc_simt1632 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F32);
   e->bpc_simt32 = spmm_simt_blocks_per_cu(Dtype::F32, Dtype::F32);
+  e->bpc_simt1664 = spmm_simt_blocks_per_cu(Dtype::F16, Dtype::F64);
+  e->bpc_simt64 = spmm_simt_blocks_per_cu(Dtype::F64, Dtype::F64);
   e->bpc_m32h = spmm_mfma32_blocks_per_cu(Dtype::F16);
   e->bpc_m32f = spmm_mfma32_blocks_per_cu(Dtype::F32);
   *out = e.release();
@@ -835,11 +843,6 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
       CALL_TRY(upload_layer(net, l, net->layers[l], o.max_tile_waste));
       net->layers[l] = CscLayer();
     }
-  }
-  {
-    std::vector<DevLayer> views(std::max(net->L, 1));
-    for (int l = 0; l < net->L; ++l) views[l] = net->dl[l]->view;
-    CALL_TRY(dupload(net->dev_layers, views));
   }
   CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));
   if (net->input_mode == 1) {

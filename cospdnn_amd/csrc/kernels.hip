@@ -15,6 +15,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cospdnn {
 
@@ -23,6 +24,9 @@ typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef __fp16 fp16x4_t __attribute__((__vector_size__(4 * sizeof(__fp16))));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
+// Weight-side arrays are never written while a kernel runs: reading them through the
+// constant address space lets wave-uniform indices become scalar loads.
+typedef const __attribute__((address_space(4))) int32_t* cint_ptr_t;
 
 constexpr int kChunk = 32;  // K per MFMA chunk = gathered input rows per LDS tile
 
@@ -156,222 +160,207 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 }
 
 // --------------------------------------------------------------------------
-// MFMA SpMM.  Work item = (sample tile t of 256, block column c of 32 outputs),
-// handed out by an atomic counter; a work-group walks the K chunks of its
-// items as one stream of steps.  Four waves; wave w owns samples
-// [64w, 64w+64) of the tile as four 16-sample N-tiles and both 16-output
-// M-tiles: D[out][sample] = W^T[out][k] * X[k][sample].
-// Per step the 32 gathered input rows x 256 samples (16 KB) arrive in LDS by
-// global_load_lds_dwordx4 (lane i writes bytes 16i..16i+15 of the wave's LDS
-// window).  The LDS tile is double-buffered: the next step's copy is issued
-// before the current step's MFMAs, after the loads whose values this step
-// still needs (A fragments, qmap, output ids), so the compiler's vmcnt waits
-// for those never wait on the prefetch.  Each lane then gathers its B fragment
-// X[8g..8g+7][sample] with eight 16-bit LDS reads.
+// MFMA SpMM.  Every wave works alone (no work-group barriers): a work item is
+// (64 samples, block column c of 32 outputs), items are strided over all waves
+// of the grid, and the K chunks of an item are consecutive steps.
+// Per step the wave's 32 gathered input rows x 64 samples (4 KB) arrive in
+// registers, prefetched one step ahead (lane L loads row L/2, samples
+// 32(L%2)..+31 with four 16-byte loads; the row ids come two steps ahead), are
+// written to the wave's own LDS slice, and each lane reads its B fragment
+// X[8g..8g+7][sample] with two ds_read_b64_tr_b16.  D[out][sample] =
+// W^T[out][k] * X[k][sample] on v_mfma_f32_16x16x32_f16 (2 M-tiles x 4 N-tiles).
+// Epilogue: the pre-activations are staged in LDS as FP32 [32][64]; lane L then
+// owns sample L: it clamps, stores its 32 outputs (each store instruction covers
+// 64 consecutive target positions of one output row) and sets its flags.
 // --------------------------------------------------------------------------
-// Weight-side arrays are never written while a kernel runs: reading them through the
-// constant address space lets wave-uniform indices become scalar loads.
-typedef const __attribute__((address_space(4))) int32_t* cint_ptr_t;
+constexpr int kWaveSamples = 64;
+constexpr int kWavesPerGroup = 4;
 
-__device__ __forceinline__ void mfma_issue_tile(const _Float16* __restrict__ X, int64_t ldx, cint_ptr_t gidx,
-                                                int ch, int s0, int wave, int lane, _Float16* buf) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = wave * 8 + i * 2;  // wave-uniform first row of the pair
-    const int r0 = gidx[ch * kChunk + r];
-    const int r1 = gidx[ch * kChunk + r + 1];
-    const int row = lane < 32 ? r0 : r1;
-    const _Float16* src = X + static_cast<int64_t>(row) * ldx + s0 + (lane & 31) * 8;
-    __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(buf + r * kTileS), 16, 0, 0);
+struct StepPos {
+  int item, t, c, ch, ch_end;
+  bool valid;
+};
+
+__device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int nblk, cint_ptr_t kptr, bool k1) {
+  s.valid = item < nitems;
+  s.item = item;
+  s.t = item / nblk;
+  s.c = item - s.t * nblk;
+  if (!s.valid) {
+    s.ch = 0;
+    s.ch_end = 1;
+  } else if (k1) {
+    s.ch = s.c;
+    s.ch_end = s.c + 1;
+  } else {
+    s.ch = kptr[s.c];
+    s.ch_end = kptr[s.c + 1];
   }
 }
 
-__device__ __forceinline__ void mfma_issue_weights(const _Float16* __restrict__ wt, int ch, int wave, int lane,
-                                                   _Float16* buf) {
-  // the chunk's 2 KB A-operand image: waves 0 and 1 copy 1 KB each
-  if (wave < 2) {
-    const _Float16* src = wt + static_cast<int64_t>(ch) * 1024 + wave * 512 + lane * 8;
-    __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(buf + wave * 512), 16, 0, 0);
+__device__ __forceinline__ void step_next(StepPos& s, int stride, int nitems, int nblk, cint_ptr_t kptr, bool k1) {
+  if (!s.valid) return;
+  if (s.ch + 1 < s.ch_end) {
+    ++s.ch;
+    return;
   }
+  step_first(s, s.item + stride, nitems, nblk, kptr, k1);
 }
 
-__global__ __launch_bounds__(256) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
-  __shared__ __attribute__((aligned(16))) _Float16 xs[2][kChunk * kTileS];  // activation tiles
-  __shared__ __attribute__((aligned(16))) _Float16 ws[2][2 * 64 * 8];       // weight tiles
-  __shared__ int queue_slot[2];
+
+__global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
+  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kWaveSamples];  // 8 KB per wave
   const int n = a.counts[a.layer];
   const int nblk = w.nblk;
-  const int nitems = (n + kTileS - 1) / kTileS * nblk;
-  // Items come from a counter, two ahead: the current item and the one after it are known,
-  // and the atomic that fetches the third lands before the step that needs it.
-  if (threadIdx.x == 0) {
-    queue_slot[0] = atomicAdd(a.work, 1);
-    queue_slot[1] = atomicAdd(a.work, 1);
-  }
-  __syncthreads();
-  int item = __builtin_amdgcn_readfirstlane(queue_slot[0]);
-  int item_after = __builtin_amdgcn_readfirstlane(queue_slot[1]);
-  if (item >= nitems) return;  // uniform
-  int fetched = 0;             // thread 0: result of the atomic issued in the previous step
-  bool fetch_pending = false;
-  int qsel = 0;                // the two queue slots alternate, so a slot is rewritten only
-                               // after a barrier that follows every wave's read of it
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: keeps gidx loads scalar
-  const int j = lane & 15, g = lane >> 4;
+  const int nitems = (n + kWaveSamples - 1) / kWaveSamples * nblk;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int stride = gridDim.x * kWavesPerGroup;
+  const bool k1 = w.one_chunk != 0;
+  const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
+  const cint_ptr_t out_idx = (cint_ptr_t)w.out_idx;
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
-  const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
-  const cint_ptr_t gidx = (cint_ptr_t)w.gidx;
 
-  // epilogue indices of an item: target positions of its samples, ids of its outputs
-  auto load_item_ids = [&](int t_, int c_, int (&q_)[4], int (&o_)[8]) {
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const int p = t_ * kTileS + wave * 64 + nt * 16 + j;
-      const int qv = a.qmap[min(p, n - 1)];
-      q_[nt] = p < n ? qv : -1;
-    }
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) o_[m * 4 + r] = w.out_idx[c_ * 32 + m * 16 + g * 4 + r];
-  };
+  StepPos cur, nx1, nx2;
+  step_first(cur, blockIdx.x * kWavesPerGroup + wave, nitems, nblk, kptr, k1);
+  if (!cur.valid) return;  // wave-uniform; nothing below synchronises across waves
+  nx1 = cur;
+  step_next(nx1, stride, nitems, nblk, kptr, k1);
+  nx2 = nx1;
+  step_next(nx2, stride, nitems, nblk, kptr, k1);
 
-  int t = item / nblk, c = item - t * nblk;
-  int ch = kptr[c], ch_end = kptr[c + 1];
-  mfma_issue_tile(X, a.ldx, gidx, ch, t * kTileS, wave, lane, xs[0]);
-  mfma_issue_weights(w.wt, ch, wave, lane, ws[0]);
-  int q[4], outs[8];
-  load_item_ids(t, c, q, outs);
-  int buf = 0;
+  _Float16* tile = reinterpret_cast<_Float16*>(wlds[wave]);  // [32][64] FP16 input tile (first 4 KB)
+  float* stage = wlds[wave];                                 // [32][64] FP32 pre-activations
+  const int lrow = lane >> 1, lcol = (lane & 1) * 32;
+  const int j = lane & 15, g = lane >> 4;
+  const int32_t* __restrict__ gidx = w.gidx;
+
+  // Everything a step consumes is loaded one step ahead: its tile rows (d0..d3), its A
+  // fragments, the target position of this lane's sample; the row ids two steps ahead.
+  // After the final step the prefetch re-reads valid (unused) addresses.
+  int row_nx1 = gidx[(nx1.valid ? nx1.ch : cur.ch) * kChunk + lrow];
+  uint4 d0, d1, d2, d3;
+  half8_t a0, a1;
+  int qv;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(
+        X + static_cast<int64_t>(gidx[cur.ch * kChunk + lrow]) * a.ldx + cur.t * kWaveSamples + lcol);
+    d0 = src[0];
+    d1 = src[1];
+    d2 = src[2];
+    d3 = src[3];
+    a0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
+    a1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
+    qv = a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
+  }
   float4_t acc[2][4];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) acc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-  // Software pipeline, one step per K chunk.  At the top of a step every copy and load
-  // issued by the previous step has landed (one s_waitcnt the compiler can see); all LDS
-  // reads of the step come before the next step's copies into the other buffers, and no
-  // register loaded after those copies is used in this step.
   while (true) {
-    __builtin_amdgcn_s_waitcnt(0);
-    if (fetch_pending && tid == 0) queue_slot[qsel] = fetched;
-    __syncthreads();
-    if (fetch_pending) {
-      item_after = __builtin_amdgcn_readfirstlane(queue_slot[qsel]);
-      fetch_pending = false;
-      qsel ^= 1;
+    const bool last = cur.ch + 1 == cur.ch_end;
+    const int p_own = cur.t * kWaveSamples + lane;
+    const int q_own = p_own < n ? qv : -1;
+    const half8_t ca0 = a0, ca1 = a1;
+    // this step's tile into the wave's LDS slice
+    {
+      uint4* dst = reinterpret_cast<uint4*>(tile + lrow * kWaveSamples + lcol);
+      dst[0] = d0;
+      dst[1] = d1;
+      dst[2] = d2;
+      dst[3] = d3;
     }
-    const _Float16* xb = xs[buf];
-    const _Float16* wb = ws[buf];
-    const half8_t a0 = *reinterpret_cast<const half8_t*>(wb + (0 * 64 + lane) * 8);
-    const half8_t a1 = *reinterpret_cast<const half8_t*>(wb + (1 * 64 + lane) * 8);
-    // B fragment of N-tile nt: lane (j, g) needs X[8g+e][16nt+j], e < 8.  ds_read_b64_tr_b16
-    // reads 4 halves per lane and, inside each group of 16 lanes, hands lane i element i%4 of
-    // lanes 4k+i/4 (k < 4).  So lane m of the group reads row 8g+4h+m/4 at samples
-    // 16nt+4(m%4)..+3, and two reads (h = 0, 1) give the eight K values of its own sample.
+    // next step's loads
+    {
+      const int pt = nx1.valid ? nx1.t : cur.t;
+      const int pch = nx1.valid ? nx1.ch : cur.ch;
+      const uint4* src = reinterpret_cast<const uint4*>(X + static_cast<int64_t>(row_nx1) * a.ldx + pt * kWaveSamples + lcol);
+      d0 = src[0];
+      d1 = src[1];
+      d2 = src[2];
+      d3 = src[3];
+      a0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
+      a1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
+      qv = a.qmap[min(pt * kWaveSamples + lane, n - 1)];
+      row_nx1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + lrow];
+    }
+    // B fragments: ds_read_b64_tr_b16 gives lane i of each 16-lane group element i%4 of the
+    // 8 bytes read by lanes 4k+i/4, so lane m reads row 8g+4h+m/4 at samples 16nt+4(m%4)..+3
     half8_t bfrag[4];
-    const int tr_row = g * 8 + (j >> 2);
-    const int tr_col = wave * 64 + 4 * (j & 3);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      const _Float16* p0 = xb + tr_row * kTileS + tr_col + nt * 16;
+      const _Float16* p0 = tile + (g * 8 + (j >> 2)) * kWaveSamples + nt * 16 + 4 * (j & 3);
       const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)p0);
       const fp16x4_t hi =
-          __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)(p0 + 4 * kTileS));
-      const half4_t l4 = __builtin_bit_cast(half4_t, lo);
-      const half4_t h4 = __builtin_bit_cast(half4_t, hi);
-      bfrag[nt] = __builtin_shufflevector(l4, h4, 0, 1, 2, 3, 4, 5, 6, 7);
+          __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)(p0 + 4 * kWaveSamples));
+      bfrag[nt] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
+                                          4, 5, 6, 7);
     }
-    // next step: the following chunk of this item, or the first chunk of this group's next item
-    const bool last = ch + 1 == ch_end;
-    int n_item = item, n_t = t, n_c = c, n_ch = ch + 1, n_end = ch_end;
-    bool more = true;
-    if (last) {
-      n_item = item_after;
-      more = n_item < nitems;
-      if (more) {
-        n_t = n_item / nblk;
-        n_c = n_item - n_t * nblk;
-        n_ch = kptr[n_c];
-        n_end = kptr[n_c + 1];
-      }
-    }
-    // always issued (after the final step it re-reads this step's chunk, which is in bounds)
-    const int p_t = more ? n_t : t, p_c = more ? n_c : c, p_ch = more ? n_ch : ch;
-    mfma_issue_tile(X, a.ldx, gidx, p_ch, p_t * kTileS, wave, lane, xs[buf ^ 1]);
-    mfma_issue_weights(w.wt, p_ch, wave, lane, ws[buf ^ 1]);
-    int q_n[4], outs_n[8];
-    load_item_ids(p_t, p_c, q_n, outs_n);
-    if (last && more) {
-      // the item after the next one; its index is read at the top of the next step
-      if (tid == 0) fetched = atomicAdd(a.work, 1);
-      fetch_pending = true;
-    }
-
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, bfrag[nt], acc[0][nt], 0, 0, 0);
-      acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, bfrag[nt], acc[1][nt], 0, 0, 0);
+      acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ca0, bfrag[nt], acc[0][nt], 0, 0, 0);
+      acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ca1, bfrag[nt], acc[1][nt], 0, 0, 0);
     }
-
     if (last) {
-      // Epilogue. D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4:
-      // this lane holds outputs 16m + 4g + r of sample 16nt + j.
+      // D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4: this lane holds
+      // outputs 16m + 4g + r of sample 16nt + j
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        int fl = 0;
+      for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
+        for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float v = acc[m][nt][r] + a.bias;
-            const int out = outs[m * 4 + r];
-            if (out >= 0) {
-              fl |= (v > 0.0f) ? 1 : 0;
-              fl |= (v < 32.0f) ? 2 : 0;
-              fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4 : 0;
-              const float y = fminf(fmaxf(v, 0.0f), 32.0f);
-              if (q[nt] >= 0) Y[static_cast<int64_t>(out) * a.ldy + q[nt]] = static_cast<_Float16>(y);
-            }
+          for (int r = 0; r < 4; ++r) stage[(m * 16 + g * 4 + r) * kWaveSamples + nt * 16 + j] = acc[m][nt][r] + a.bias;
+      int fl = 0;
+      if (q_own >= 0) {
+        const int32_t cbase = cur.c * 32;
+#pragma unroll 8
+        for (int o = 0; o < 32; ++o) {
+          const int out = out_idx[cbase + o];
+          const float v = stage[o * kWaveSamples + lane];
+          if (out >= 0) {
+            fl |= (v > 0.0f) ? 1 : 0;
+            fl |= (v < 32.0f) ? 2 : 0;
+            fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4 : 0;
+            Y[static_cast<int64_t>(out) * a.ldy + q_own] = static_cast<_Float16>(fminf(fmaxf(v, 0.0f), 32.0f));
           }
         }
-        fl |= __shfl_xor(fl, 16, 64);
-        fl |= __shfl_xor(fl, 32, 64);
-        if (g == 0 && q[nt] >= 0) {
-          if (fl & 1) a.alive_next[q[nt]] = 1;
-          if (fl & 2) a.unsat_next[q[nt]] = 1;
-          if ((fl & 4) && a.band_next != nullptr) a.band_next[q[nt]] = 1;
-        }
+        if (fl & 1) a.alive_next[q_own] = 1;
+        if (fl & 2) a.unsat_next[q_own] = 1;
+        if ((fl & 4) && a.band_next != nullptr) a.band_next[q_own] = 1;
       }
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
     }
-    if (!more) break;
-    item = n_item;
-    t = n_t;
-    c = n_c;
-    ch = n_ch;
-    ch_end = n_end;
-    buf ^= 1;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) q[i] = q_n[i];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) outs[i] = outs_n[i];
+    if (!nx1.valid) break;
+    cur = nx1;
+    nx1 = nx2;
+    step_next(nx2, stride, nitems, nblk, kptr, k1);
   }
-  __builtin_amdgcn_s_waitcnt(0);  // the final re-read copy lands before the group exits
 }
 
 // --------------------------------------------------------------------------
 // SIMT gather SpMM: thread = sample, work item = (tile of 256 samples, block of
 // 32 outputs); each output sums its CSC inputs read straight from global memory.
 // --------------------------------------------------------------------------
+// Accumulation of one product: FP32 lets the compiler fuse; FP64 keeps the oracle's
+// separate multiply and add so FP64 passes are bit-identical to it.
+template <typename TAcc>
+__device__ __forceinline__ TAcc madd(TAcc acc, TAcc x, float w);
+template <>
+__device__ __forceinline__ float madd<float>(float acc, float x, float w) { return acc + x * w; }
+template <>
+__device__ __forceinline__ double madd<double>(double acc, double x, float w) {
+  return __dadd_rn(acc, __dmul_rn(x, static_cast<double>(w)));
+}
+
 template <typename TIn, typename TOut>
 __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) {
+  typedef typename std::conditional<std::is_same<TOut, double>::value, double, float>::type TAcc;
   const int n = a.counts[a.layer];
   const int ntiles = (n + kTileS - 1) / kTileS;
   const int64_t nitems = static_cast<int64_t>(ntiles) * w.nblk;
@@ -389,7 +378,7 @@ __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) 
       const int out = w.out_idx[c * 32 + o];
       if (out < 0) continue;
       const int k0 = w.optr[c * 32 + o], k1 = w.optr[c * 32 + o + 1];
-      float acc = 0.0f;
+      TAcc acc = 0;
       int k = k0;
       // eight independent gathers in flight per thread, summed in CSC order
       for (; k + 8 <= k1; k += 8) {
@@ -397,14 +386,15 @@ __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) 
 #pragma unroll
         for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * a.ldx];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += static_cast<float>(xv[u]) * w.oval[k + u];
+        for (int u = 0; u < 8; ++u) acc = madd<TAcc>(acc, static_cast<TAcc>(xv[u]), w.oval[k + u]);
       }
-      for (; k < k1; ++k) acc += static_cast<float>(xcol[static_cast<int64_t>(w.orow[k]) * a.ldx]) * w.oval[k];
-      const float v = acc + a.bias;
-      fl |= (v > 0.0f) ? 1 : 0;
-      fl |= (v < 32.0f) ? 2 : 0;
-      fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4 : 0;
-      const float y = fminf(fmaxf(v, 0.0f), 32.0f);
+      for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * a.ldx]), w.oval[k]);
+      const TAcc v = std::is_same<TAcc, double>::value ? static_cast<TAcc>(__dadd_rn(acc, static_cast<double>(a.bias)))
+                                                       : acc + static_cast<TAcc>(a.bias);
+      fl |= (v > 0) ? 1 : 0;
+      fl |= (v < 32) ? 2 : 0;
+      fl |= (fabs(static_cast<double>(v)) < a.band || fabs(static_cast<double>(v) - 32.0) < a.band) ? 4 : 0;
+      const TAcc y = v < 0 ? TAcc(0) : (v > 32 ? TAcc(32) : v);
       Y[static_cast<int64_t>(out) * a.ldy + q] = static_cast<TOut>(y);
     }
     if (fl & 1) a.alive_next[q] = 1;
@@ -511,75 +501,6 @@ __global__ __launch_bounds__(256) void spmm_mfma32_f32in_kernel(StepArgs a, DevL
   spmm_mfma32_body<float>(a, w, xs);
 }
 
-// --------------------------------------------------------------------------
-// FP64 replay: one work-group carries one sample through all layers in FP64,
-// each output summing its inputs in CSC order with separate multiply and add
-// (no contraction), i.e. the operation sequence of the FP64 oracle, so the
-// activations and the category of a replayed row are bit-identical to it.
-// The layer's input vector sits in LDS when it fits (N <= 16384); the gathers
-// of an output are issued in groups of eight ahead of the ordered additions.
-// --------------------------------------------------------------------------
-constexpr int kReplayThreads = 1024;
-constexpr int kReplayLdsDoubles = 16384;
-
-template <bool kInLds>
-__device__ __forceinline__ void replay_layer(const DevLayer& w, const double* __restrict__ yin, double* __restrict__ yout,
-                                             double b) {
-  const int slots = w.nblk * 32;
-  for (int slot = threadIdx.x; slot < slots; slot += kReplayThreads) {
-    const int out = w.out_idx[slot];
-    if (out < 0) continue;
-    const int k1 = w.optr[slot + 1];
-    double acc = 0.0;
-    int k = w.optr[slot];
-    for (; k + 8 <= k1; k += 8) {
-      double x[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) x[u] = yin[w.orow[k + u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc = __dadd_rn(acc, __dmul_rn(x[u], static_cast<double>(w.oval[k + u])));
-    }
-    for (; k < k1; ++k) acc = __dadd_rn(acc, __dmul_rn(yin[w.orow[k]], static_cast<double>(w.oval[k])));
-    const double z = __dadd_rn(acc, b);
-    yout[out] = z < 0.0 ? 0.0 : (z > 32.0 ? 32.0 : z);
-  }
-}
-
-__global__ __launch_bounds__(kReplayThreads) void replay_fp64_kernel(ReplayArgs a) {
-  extern __shared__ double ylds[];
-  const int r = blockIdx.x;
-  if (r >= a.rows) return;
-  const int N = a.N;
-  const int32_t local = a.row_ids[r];
-  const int64_t id = a.map != nullptr ? a.map[local] : local;
-  double* ya = a.scratch + static_cast<int64_t>(r) * 2 * N;
-  double* yb = ya + N;
-  const bool in_lds = N <= kReplayLdsDoubles;
-  double* yin = in_lds ? ylds : ya;
-  for (int nn = threadIdx.x; nn < N; nn += kReplayThreads) yin[nn] = static_cast<double>(a.store[static_cast<int64_t>(nn) * a.ld + id]);
-  __syncthreads();
-  const double b = static_cast<double>(a.bias);
-  for (int l = 0; l < a.L; ++l) {
-    const DevLayer w = a.layers[l];
-    if (in_lds) {
-      replay_layer<true>(w, yin, yb, b);
-      __syncthreads();
-      for (int nn = threadIdx.x; nn < N; nn += kReplayThreads) yin[nn] = yb[nn];
-      __syncthreads();
-    } else {
-      replay_layer<false>(w, yin, yb, b);
-      __syncthreads();
-      double* tmp = yin;
-      yin = yb;
-      yb = tmp;
-    }
-  }
-  int any = 0;
-  for (int nn = threadIdx.x; nn < N; nn += kReplayThreads) any |= yin[nn] > 0.0 ? 1 : 0;
-  any = __syncthreads_or(any);
-  if (threadIdx.x == 0) a.alive[r] = static_cast<uint8_t>(any);
-}
-
 // alive flags straight from the input (only for networks without layers)
 __global__ void input_alive_kernel(const _Float16* __restrict__ store, int64_t ld, int N, int64_t S,
                                    uint8_t* __restrict__ alive) {
@@ -651,6 +572,10 @@ hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dty
     hipLaunchKernelGGL((spmm_simt_kernel<_Float16, float>), dim3(grid), dim3(256), 0, st, a, w);
   else if (tin == Dtype::F32 && tout == Dtype::F32)
     hipLaunchKernelGGL((spmm_simt_kernel<float, float>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F16 && tout == Dtype::F64)
+    hipLaunchKernelGGL((spmm_simt_kernel<_Float16, double>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F64 && tout == Dtype::F64)
+    hipLaunchKernelGGL((spmm_simt_kernel<double, double>), dim3(grid), dim3(256), 0, st, a, w);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -670,12 +595,6 @@ hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, i
     hipLaunchKernelGGL(spmm_mfma32_f32in_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
 }
-hipError_t launch_replay_fp64(const ReplayArgs& a, hipStream_t st) {
-  if (a.rows <= 0) return hipSuccess;
-  const size_t lds = a.N <= kReplayLdsDoubles ? sizeof(double) * static_cast<size_t>(a.N) : 0;
-  hipLaunchKernelGGL(replay_fp64_kernel, dim3(a.rows), dim3(kReplayThreads), lds, st, a);
-  return hipGetLastError();
-}
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st) {
   if (S <= 0) return hipSuccess;
   hipLaunchKernelGGL(input_alive_kernel, dim3(static_cast<unsigned>((S + 255) / 256)), dim3(256), 0, st, store, ld, N, S,
@@ -691,12 +610,7 @@ int spmm_mfma32_blocks_per_cu(Dtype tin) {
   return nb;
 }
 
-hipError_t configure_kernels() {
-  // the FP64 replay keeps up to 128 KB of its row in LDS
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(replay_fp64_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             static_cast<int>(sizeof(double) * kReplayLdsDoubles));
-}
+hipError_t configure_kernels() { return hipSuccess; }
 
 int spmm_mfma_resident_per_cu() {
   hipFuncAttributes attr;
@@ -722,8 +636,12 @@ int spmm_simt_blocks_per_cu(Dtype tin, Dtype tout) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<_Float16, _Float16>, 256, 0);
   else if (tin == Dtype::F16 && tout == Dtype::F32)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<_Float16, float>, 256, 0);
-  else
+  else if (tin == Dtype::F32 && tout == Dtype::F32)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<float, float>, 256, 0);
+  else if (tin == Dtype::F16)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<_Float16, double>, 256, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<double, double>, 256, 0);
   if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }

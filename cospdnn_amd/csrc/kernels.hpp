@@ -29,6 +29,7 @@ struct DevLayer {
   const int32_t* gidx = nullptr;
   const _Float16* wt = nullptr;
   const float* wt32 = nullptr;       // FP32 tile image (MFMA FP32 kernel)
+  int one_chunk = 0;                 // every block column has exactly one K chunk (kptr[c] = c)
 };
 
 struct StepArgs {
@@ -66,21 +67,7 @@ struct ScanArgs {
   int32_t* work = nullptr;           // reset to 0 for the next SpMM's work queue (may be null)
 };
 
-enum class Dtype : int { F16 = 0, F32 = 1 };
-
-// FP64 replay of single rows through every layer (one work-group per row).
-struct ReplayArgs {
-  const DevLayer* layers = nullptr;  // device array of L layer views
-  int L = 0, N = 0;
-  float bias = 0.0f;
-  const _Float16* store = nullptr;   // input [N][ld]
-  int64_t ld = 0;
-  int rows = 0;
-  const int32_t* row_ids = nullptr;  // rows to replay (indices into `map` when map != null)
-  const int32_t* map = nullptr;      // optional: original sample id of row_ids[r]
-  double* scratch = nullptr;         // [rows][2][N]
-  uint8_t* alive = nullptr;          // [rows] result
-};
+enum class Dtype : int { F16 = 0, F32 = 1, F64 = 2 };
 
 // Launchers (all stream-ordered, no host synchronisation).
 hipError_t launch_gen_input(_Float16* y0, int64_t ld, int N, int64_t S, uint64_t seed, hipStream_t st);
@@ -93,7 +80,6 @@ hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st);
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
-hipError_t launch_replay_fp64(const ReplayArgs& a, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,
                               int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);

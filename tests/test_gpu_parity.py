@@ -281,13 +281,14 @@ def test_reference_bcsc_input(engine):
     net.close()
 
 
-def test_fp64_replay_matches_oracle(engine):
-    """Every row replayed in FP64 (the exact tier) equals the FP64 oracle, banded and random."""
+def test_fp64_precision_matches_oracle(engine):
+    """The FP64 precision (layer-major double SpMM, the exact tier) equals the FP64 oracle, banded and random."""
     for N, L, S, bias, kind, seed in [(1024, 60, 700, -0.3, cd.GEN_RANDOM, 12), (2048, 40, 500, -0.4, cd.GEN_BANDED, 13)]:
         alive, live = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
         net = gen_net(engine, N, L, S, bias, kind, seed)
         out = cd.run([net], cd.default_opts(precision=cd.FP64))[0]
-        assert out.n_replayed == S
+        assert out.n_flagged == 0 and out.n_replayed == 0
+        assert np.array_equal(out.live, live)
         assert np.array_equal(out.categories, np.flatnonzero(alive))
         net.close()
 
