@@ -24,6 +24,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (profiles/hbm_traffic_<config>_spmm_mfma.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}_spmm_mfma.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def timed_runs(cd, net, opts, steps, warmup):
     for _ in range(warmup):
         cd.run([net], opts)
@@ -143,6 +157,7 @@ def main():
         cpu = cpu_baseline(spec, min(args.cpu_samples, spec.S), cats16)
         log(f"[bench] cpu: {cpu['value']:.3e} edges/s on {cpu['cores']} threads")
 
+    traffic, traffic_src = profiled_traffic(args.config, "spmm_mfma_kernel")
     line = {
         "metric": METRIC,
         "value": value,
@@ -168,7 +183,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "spmm_mfma_kernel",
             "avg_launch_ms": spmm_ms / launches,
             "alg_bytes_per_launch": alg_bytes / launches,

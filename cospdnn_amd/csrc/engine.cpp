@@ -91,6 +91,7 @@ struct LayerStore {
   bool has_tiles = false;
   double waste = 0.0;
   bool fp16_exact = true;
+  bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
   int64_t nnz = 0;
 };
 
@@ -105,6 +106,7 @@ struct RunBufs {
   DevBuf flag_ids, flag_count, flag_mark;
   DevBuf work;   // MFMA work-queue counter
   DevBuf in16;   // recompute input
+  DevBuf ret_ids, ret_count;  // rows retired as saturated (guard passes)
   uint8_t* f(int side, int kind) const { return flags.as<uint8_t>() + static_cast<int64_t>(side * 3 + kind) * ld; }
   int32_t* id(int side) const { return ids.as<int32_t>() + static_cast<int64_t>(side) * ld; }
 };
@@ -128,6 +130,8 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     CALL_TRY(dalloc(rb.flag_count, sizeof(int32_t)));
     CALL_TRY(dalloc(rb.flag_mark, static_cast<size_t>(rb.ld)));
     CALL_TRY(dalloc(rb.work, sizeof(int32_t)));
+    CALL_TRY(dalloc(rb.ret_ids, sizeof(int32_t) * rb.ld));
+    CALL_TRY(dalloc(rb.ret_count, sizeof(int32_t)));
     HIP_TRY(hipMemsetAsync(rb.flags.p, 0, rb.flags.bytes, st));
   }
   const size_t elems = static_cast<size_t>(N) * rb.ld;
@@ -166,7 +170,7 @@ struct cospdnn_engine {
   int cu_count = 0;
   int64_t hbm_bytes = 0;
   int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
-      bpc_simt1664 = 1, bpc_simt64 = 1;
+      bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
 };
 
 struct GraphCache {
@@ -233,6 +237,7 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
   ls->waste = bl.tile_waste;
   ls->fp16_exact = bl.fp16_exact;
   ls->nnz = bl.nnz;
+  ls->sat_stable = all32_fixed_point(csc, net->N, net->bias);
   if (bl.has_tiles) {
     CALL_TRY(dupload(ls->kptr, bl.kptr));
     CALL_TRY(dupload(ls->gidx, bl.gidx));
@@ -259,7 +264,16 @@ struct RunCfg {
   int band_from;
   int use_mfma;     // FP16 pass: MFMA kernel for tiled layers
   int mfma32;       // FP32 pass: MFMA FP32 kernel for tiled layers
+  int retire_sat = 0;  // guard passes: saturated rows leave the pass once all-32 is a fixed point of the rest
 };
+
+constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
+
+int grid_rows(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
+  const int64_t items = (S_run + kFewRowsTile - 1) / kFewRowsTile * static_cast<int64_t>(nblk);
+  const int64_t g = std::min<int64_t>(std::max<int64_t>(items, 1), static_cast<int64_t>(bpc) * e->cu_count);
+  return static_cast<int>(g);
+}
 
 int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   const int64_t items = (S_run + kTileS - 1) / kTileS * static_cast<int64_t>(nblk);
@@ -283,8 +297,13 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   HIP_TRY(launch_set_i32(rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), 0, st));
   HIP_TRY(launch_fill_u8(rb.flag_mark.as<uint8_t>(), 0, S_run, st));
+  HIP_TRY(launch_set_i32(rb.ret_count.as<int32_t>(), 0, st));
   const int D = cfg.guard_layer >= 0 ? std::min(cfg.guard_layer, L) : -1;
   const bool fp16 = cfg.precision == COSPDNN_FP16;
+  // sat_from[l]: all-32 is a fixed point of layers l..L-1
+  std::vector<char> sat_from(L + 1, 1);
+  for (int l = L - 1; l >= 0; --l) sat_from[l] = sat_from[l + 1] && net->dl[l]->sat_stable;
+  const bool few = S_run <= kFewRows;
   for (int l = 0; l <= L; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     ScanArgs sa;
@@ -304,6 +323,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.flag_count = rb.flag_count.as<int32_t>();
     sa.flag_mark = rb.flag_mark.as<uint8_t>();
     sa.work = rb.work.as<int32_t>();
+    sa.retire_sat = (cfg.retire_sat && l < L && sat_from[l]) ? 1 : 0;
+    sa.ret_ids = rb.ret_ids.as<int32_t>();
+    sa.ret_count = rb.ret_count.as<int32_t>();
     HIP_TRY(launch_scan(sa, S_run, st));
     if (l == L) break;
 
@@ -333,10 +355,19 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{kMfmaGroupsPerCu} * e->cu_count)));
       HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
     } else if (fp16) {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, nblk), st));
+      if (few)
+        HIP_TRY(launch_spmm_simt_rows(a, ls.view, Dtype::F16, Dtype::F16, grid_rows(e, e->bpc_rows, S_run, nblk), st));
+      else
+        HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, nblk), st));
     } else if (fp64) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F64;
-      HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk), st));
+      if (few)
+        HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F64, grid_rows(e, e->bpc_rows, S_run, nblk), st));
+      else
+        HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk), st));
+    } else if (few && !(ls.has_tiles && cfg.mfma32)) {
+      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
+      HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F32, grid_rows(e, e->bpc_rows, S_run, nblk), st));
     } else if (ls.has_tiles && cfg.mfma32) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
       HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk), st));
@@ -362,9 +393,13 @@ int read_pass(const cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& 
   const int L = net->L;
   out.counts.assign(L + 2, 0);
   HIP_TRY(hipMemcpy(out.counts.data(), rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost));
-  out.cats.assign(out.counts[L + 1], 0);
-  if (!out.cats.empty())
-    HIP_TRY(hipMemcpy(out.cats.data(), rb.id(cat_side), sizeof(int32_t) * out.cats.size(), hipMemcpyDeviceToHost));
+  int32_t nret = 0;
+  HIP_TRY(hipMemcpy(&nret, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int32_t nlive = out.counts[L + 1];
+  out.cats.assign(static_cast<size_t>(nlive) + nret, 0);
+  if (nlive > 0) HIP_TRY(hipMemcpy(out.cats.data(), rb.id(cat_side), sizeof(int32_t) * nlive, hipMemcpyDeviceToHost));
+  if (nret > 0)
+    HIP_TRY(hipMemcpy(out.cats.data() + nlive, rb.ret_ids.p, sizeof(int32_t) * nret, hipMemcpyDeviceToHost));
   std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
   int32_t nf = 0;
   HIP_TRY(hipMemcpy(&nf, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -394,6 +429,7 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   cfg.band_from = 1 << 30;
   cfg.use_mfma = 0;
   cfg.mfma32 = 0;
+  cfg.retire_sat = 1;
   int side = 0;
   CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, &side));
   HIP_TRY(hipStreamSynchronize(st));
@@ -499,6 +535,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       rcfg.band_from = 1 << 30;
       rcfg.use_mfma = 0;
       rcfg.mfma32 = 1;
+      rcfg.retire_sat = 1;
       int rside = 0;
       CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, &rside));
       HIP_TRY(hipStreamSynchronize(st));
@@ -672,6 +709,7 @@ int cospdnn_engine_create(int device, cospdnn_engine** out) {
   e->bpc_simt64 = spmm_simt_blocks_per_cu(Dtype::F64, Dtype::F64);
   e->bpc_m32h = spmm_mfma32_blocks_per_cu(Dtype::F16);
   e->bpc_m32f = spmm_mfma32_blocks_per_cu(Dtype::F32);
+  e->bpc_rows = spmm_simt_rows_blocks_per_cu(Dtype::F64, Dtype::F64);
   *out = e.release();
   return COSPDNN_OK;
 }

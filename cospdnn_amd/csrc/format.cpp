@@ -66,6 +66,22 @@ uint64_t set_signature(const int32_t* r, int n) {
 
 }  // namespace
 
+bool all32_fixed_point(const CscLayer& w, int N, float bias) {
+  if (N <= 0 || static_cast<int>(w.colptr.size()) != N + 1) return false;
+  const double u = std::ldexp(1.0, -24);
+  for (int j = 0; j < N; ++j) {
+    double cs = 0.0;
+    for (int32_t q = w.colptr[j]; q < w.colptr[j + 1]; ++q) {
+      if (!(w.vals[q] >= 0.0f)) return false;
+      cs += w.vals[q];
+    }
+    const double kj = static_cast<double>(w.colptr[j + 1] - w.colptr[j]);
+    const double lo = (32.0 * cs * (1.0 - 2.0 * (kj + 1.0) * u) + bias) * (1.0 - 2.0 * u);
+    if (!(lo >= 32.0)) return false;
+  }
+  return true;
+}
+
 bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) {
   if (N <= 0 || static_cast<int>(w.colptr.size()) != N + 1 || w.colptr[0] != 0) return false;
   const int64_t nnz = w.colptr[N];

@@ -52,6 +52,12 @@ struct BlockedLayer {
 // Returns false (and leaves `out` unspecified) if the CSC is malformed.
 bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out);
 
+// True when an all-32 input row gives an all-32 output row whatever the summation order
+// in FP32 or FP64: every weight is nonnegative and, for every column j (empty ones
+// included), 32 * colsum_j + bias >= 32 after the worst rounding loss of a K_j-term FP32
+// sum (products by 32 are exact), with a factor 2 of slack.
+bool all32_fixed_point(const CscLayer& w, int N, float bias);
+
 // Reference BCSC (values [out][in] per 16x16 block, utils/matrix.h:1245;
 // block start rows; block-column pointers) + fuse list (row map applied to
 // block rows, kernels/kernel.cu:310) -> CSC.  Zero entries are dropped.

@@ -4,10 +4,12 @@
 //                      fused bias + clamp(0,32) + liveness / guard flags + compacted
 //                      store epilogue (replaces cospdnn_single, kernels/kernel.cu:240-379,
 //                      and check_active_f_col / matrix_reshape, kernel.cu:129-172).
-//   spmm_simt_kernel   plain SIMT per-output gather SpMM (FP32 baseline path and the
-//                      FP16 path for layers without a compact tile image).
-//   scan_kernel        one work-group: live flags -> prefix sum -> position map, id
-//                      compaction, guard bookkeeping (replaces the host scan of
+//   spmm_mfma32_*      FP32 MFMA SpMM (v_mfma_f32_16x16x4_f32) for the FP32 guard pass.
+//   spmm_simt_kernel   plain SIMT per-output gather SpMM (FP32 baseline path, the FP64
+//                      pass, and the FP16 path for layers without a compact tile image).
+//   spmm_simt_rows_kernel  the same arithmetic spread over outputs for passes over few rows.
+//   scan_kernel        live flags -> prefix sum -> position map, id compaction, guard
+//                      bookkeeping, saturated-row retirement (replaces the host scan of
 //                      cospdnn.cu:249-265).
 #include <hip/hip_runtime.h>
 
@@ -99,12 +101,18 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   const int pc = blockIdx.x * kScanTile + tid * kScanPer;  // multiple of 16: aligned, inside the ld-sized arrays
   if (blockIdx.x * kScanTile >= n) return;                  // uniform per group
   const uint4 zero = {0u, 0u, 0u, 0u};
-  uint4 va = zero, vu = zero, vb = zero;
+  uint4 va = zero, vu = zero, vb = zero, vr = zero;
   int cnt = 0;
   if (pc < n) {
     va = *reinterpret_cast<const uint4*>(a.alive + pc);
     vu = *reinterpret_cast<const uint4*>(a.unsat + pc);
     if (a.band != nullptr) vb = *reinterpret_cast<const uint4*>(a.band + pc);
+    if (a.retire_sat) {
+      // flags are 0/1 bytes: retired = alive & !unsat, kept = alive & unsat
+      vr = make_uint4(va.x & ~vu.x & 0x01010101u, va.y & ~vu.y & 0x01010101u, va.z & ~vu.z & 0x01010101u,
+                      va.w & ~vu.w & 0x01010101u);
+      va = make_uint4(va.x & vu.x, va.y & vu.y, va.z & vu.z, va.w & vu.w);
+    }
     if (pc + kScanPer <= n) {
       cnt = count_flags16(va);
     } else {
@@ -142,6 +150,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
       a.ids_out[q] = a.ids_in[p];
       ++q;
     }
+    if (p < n && flag_at(vr, i)) a.ret_ids[atomicAdd(a.ret_count, 1)] = a.ids_in[p];
     const bool fl = p < n && (flag_at(vb, i) || (a.flag_undecided && al && flag_at(vu, i)));
     if (fl) {
       const int32_t id = a.ids_in[p];
@@ -403,6 +412,53 @@ __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) 
   }
 }
 
+// Few-row variant for the guard passes: a work item is (8 samples, block column c of 32
+// outputs) and thread t computes output t/8 of sample t%8, so a pass over a handful of
+// rows still spreads each layer's gathers over the grid (the per-sample kernel above
+// would leave 248 of 256 threads idle and run 32 outputs x K gathers in series).
+// Same arithmetic and CSC summation order as spmm_simt_kernel.  Several threads set the
+// same flag byte of a sample; they all write 1.
+template <typename TIn, typename TOut>
+__global__ __launch_bounds__(256) void spmm_simt_rows_kernel(StepArgs a, DevLayer w) {
+  typedef typename std::conditional<std::is_same<TOut, double>::value, double, float>::type TAcc;
+  const int n = a.counts[a.layer];
+  const int ntiles = (n + kFewRowsTile - 1) / kFewRowsTile;
+  const int64_t nitems = static_cast<int64_t>(ntiles) * w.nblk;
+  const TIn* __restrict__ X = static_cast<const TIn*>(a.x);
+  TOut* __restrict__ Y = static_cast<TOut*>(a.y);
+  const int o = static_cast<int>(threadIdx.x) / kFewRowsTile;
+  const int sp = static_cast<int>(threadIdx.x) % kFewRowsTile;
+  for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int t = static_cast<int>(item / w.nblk);
+    const int c = static_cast<int>(item - static_cast<int64_t>(t) * w.nblk);
+    const int p = t * kFewRowsTile + sp;
+    const int q = p < n ? a.qmap[p] : -1;
+    if (q < 0) continue;
+    const int out = w.out_idx[c * 32 + o];
+    if (out < 0) continue;
+    const TIn* __restrict__ xcol = X + p;
+    const int k0 = w.optr[c * 32 + o], k1 = w.optr[c * 32 + o + 1];
+    TAcc acc = 0;
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+      TIn xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * a.ldx];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = madd<TAcc>(acc, static_cast<TAcc>(xv[u]), w.oval[k + u]);
+    }
+    for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * a.ldx]), w.oval[k]);
+    const TAcc v = std::is_same<TAcc, double>::value ? static_cast<TAcc>(__dadd_rn(acc, static_cast<double>(a.bias)))
+                                                     : acc + static_cast<TAcc>(a.bias);
+    const TAcc y = v < 0 ? TAcc(0) : (v > 32 ? TAcc(32) : v);
+    Y[static_cast<int64_t>(out) * a.ldy + q] = static_cast<TOut>(y);
+    if (v > 0) a.alive_next[q] = 1;
+    if (v < 32) a.unsat_next[q] = 1;
+    if (a.band_next != nullptr && (fabs(static_cast<double>(v)) < a.band || fabs(static_cast<double>(v) - 32.0) < a.band))
+      a.band_next[q] = 1;
+  }
+}
+
 // --------------------------------------------------------------------------
 // FP32 MFMA SpMM (v_mfma_f32_16x16x4f32), same work decomposition as the FP16
 // kernel; used for the FP32 guard recompute.  TIn = _Float16 reads the input
@@ -565,6 +621,21 @@ hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipS
   hipLaunchKernelGGL(spmm_mfma_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
 }
+hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st) {
+  if (tin == Dtype::F16 && tout == Dtype::F16)
+    hipLaunchKernelGGL((spmm_simt_rows_kernel<_Float16, _Float16>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F16 && tout == Dtype::F32)
+    hipLaunchKernelGGL((spmm_simt_rows_kernel<_Float16, float>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F32 && tout == Dtype::F32)
+    hipLaunchKernelGGL((spmm_simt_rows_kernel<float, float>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F16 && tout == Dtype::F64)
+    hipLaunchKernelGGL((spmm_simt_rows_kernel<_Float16, double>), dim3(grid), dim3(256), 0, st, a, w);
+  else if (tin == Dtype::F64 && tout == Dtype::F64)
+    hipLaunchKernelGGL((spmm_simt_rows_kernel<double, double>), dim3(grid), dim3(256), 0, st, a, w);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st) {
   if (tin == Dtype::F16 && tout == Dtype::F16)
     hipLaunchKernelGGL((spmm_simt_kernel<_Float16, _Float16>), dim3(grid), dim3(256), 0, st, a, w);
@@ -642,6 +713,23 @@ int spmm_simt_blocks_per_cu(Dtype tin, Dtype tout) {
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<_Float16, double>, 256, 0);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_kernel<double, double>, 256, 0);
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+int spmm_simt_rows_blocks_per_cu(Dtype tin, Dtype tout) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (tin == Dtype::F16 && tout == Dtype::F16)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_rows_kernel<_Float16, _Float16>, 256, 0);
+  else if (tin == Dtype::F16 && tout == Dtype::F32)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_rows_kernel<_Float16, float>, 256, 0);
+  else if (tin == Dtype::F32 && tout == Dtype::F32)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_rows_kernel<float, float>, 256, 0);
+  else if (tin == Dtype::F16)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_rows_kernel<_Float16, double>, 256, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, spmm_simt_rows_kernel<double, double>, 256, 0);
   if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }

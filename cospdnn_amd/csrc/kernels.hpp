@@ -65,6 +65,12 @@ struct ScanArgs {
   int32_t* flag_count = nullptr;
   uint8_t* flag_mark = nullptr;      // per original id
   int32_t* work = nullptr;           // reset to 0 for the next SpMM's work queue (may be null)
+  // Retirement of saturated rows (guard passes only): a live row whose input to this layer
+  // is all 32 leaves the pass as a category (its id appended to ret_ids) when all-32 is a
+  // fixed point of every remaining layer.
+  int retire_sat = 0;
+  int32_t* ret_ids = nullptr;
+  int32_t* ret_count = nullptr;
 };
 
 enum class Dtype : int { F16 = 0, F32 = 1, F64 = 2 };
@@ -79,6 +85,9 @@ hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st);
 hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st);
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
+// Few-row variant (work item = 8 samples x 32 outputs, one thread per output and sample).
+constexpr int kFewRowsTile = 8;
+hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,
@@ -94,5 +103,6 @@ int spmm_mfma_blocks_per_cu();
 int spmm_mfma_resident_per_cu();
 int spmm_mfma32_blocks_per_cu(Dtype tin);
 int spmm_simt_blocks_per_cu(Dtype tin, Dtype tout);
+int spmm_simt_rows_blocks_per_cu(Dtype tin, Dtype tout);
 
 }  // namespace cospdnn

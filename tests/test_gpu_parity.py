@@ -305,3 +305,37 @@ def test_fp32_pass_replays_undecided_rows(engine):
     assert m32.mfma_layers == L
     assert np.array_equal(m32.categories, np.flatnonzero(alive))
     net.close()
+
+
+@pytest.mark.gpu
+def test_guard_retires_saturated_rows_only_where_all32_is_fixed(engine):
+    """The guard passes drop all-32 rows only while every remaining layer keeps all-32 fixed.
+    With bias -20, layer 7 (16 inputs per column, column sum 1) maps all-32 to 12 and the
+    rows die two layers later: retiring them before layer 7 would report them as categories."""
+    N, S, bias = 256, 400, -20.0
+    rng = np.random.default_rng(21)
+
+    def layer(K, w):
+        cp = np.arange(N + 1, dtype=np.int32) * K
+        rows = np.concatenate([np.sort(rng.choice(N, K, replace=False)) for _ in range(N)]).astype(np.int32)
+        return cp, rows, np.full(N * K, w, np.float32)
+
+    layers = [layer(32, 4.0)] + [layer(32, 0.0625) for _ in range(6)] + [layer(16, 0.0625)]
+    layers += [layer(32, 0.0625) for _ in range(4)]
+    y0 = (rng.random((S, N)) < rng.uniform(0.2, 0.8, (S, 1))).astype(np.float32)
+    for L in (7, 12):
+        say(f"retire N={N} L={L} S={S}")
+        alive, _ = orc.forward_fp64(orc.OracleNet.explicit(N, bias, layers[:L], y0.astype(np.uint8)), 0, S)
+        want = np.flatnonzero(alive)
+        assert (len(want) > 0) if L == 7 else (len(want) == 0)
+        net = cd.Net(engine, N, L, S, bias)
+        for l, (cp, rw, vl) in enumerate(layers[:L]):
+            net.set_layer_csc(l, cp, rw, vl)
+        net.set_input_dense(y0)
+        net.upload(cd.default_opts(max_tile_waste=1e9))
+        for opts in (cd.default_opts(precision=cd.FP32, guard_layer32=0),
+                     cd.default_opts(precision=cd.FP16, guard_layer=0, guard_layer32=0),
+                     cd.default_opts(precision=cd.FP16)):
+            out = cd.run([net], opts)[0]
+            assert np.array_equal(out.categories, want), (L, opts.precision, opts.guard_layer)
+        net.close()
