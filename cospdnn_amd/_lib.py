@@ -38,6 +38,8 @@ class Opts(ctypes.Structure):
         ("layer_timing", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("dump_layer", ctypes.c_int),
+        ("spmm_order", ctypes.c_int),
+        ("mfma_groups_per_cu", ctypes.c_int),
     ]
 
 

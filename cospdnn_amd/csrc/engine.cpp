@@ -175,7 +175,7 @@ struct cospdnn_engine {
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
   float band = 0.0f;
   ~GraphCache() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -248,6 +248,7 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
     ls->view.gidx = ls->gidx.as<int32_t>();
     ls->view.wt = ls->wt.as<_Float16>();
     ls->view.one_chunk = bl.nchunk == bl.nblk ? 1 : 0;
+    ls->view.full_blocks = std::all_of(bl.out_idx.begin(), bl.out_idx.end(), [](int32_t o) { return o >= 0; }) ? 1 : 0;
   }
   net->nnz_total += bl.nnz;
   net->weights_fp16_exact = net->weights_fp16_exact && bl.fp16_exact;
@@ -264,6 +265,8 @@ struct RunCfg {
   int band_from;
   int use_mfma;     // FP16 pass: MFMA kernel for tiled layers
   int mfma32;       // FP32 pass: MFMA FP32 kernel for tiled layers
+  int order = 1;    // MFMA SpMM item order
+  int groups_per_cu = kMfmaGroupsPerCu;
   int retire_sat = 0;  // guard passes: saturated rows leave the pass once all-32 is a fixed point of the rest
 };
 
@@ -352,7 +355,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     if (fp16 && ls.has_tiles && cfg.use_mfma) {
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
-      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{kMfmaGroupsPerCu} * e->cu_count)));
+      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{cfg.groups_per_cu} * e->cu_count)));
+      a.order = cfg.order;
+      // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
+      if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
       HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
     } else if (fp16) {
       if (few)
@@ -447,6 +453,9 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
+  if (o.spmm_order != 0 && o.spmm_order != 1) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0 or 1");
+  if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
+    return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
   if (o.precision == COSPDNN_FP16 && (!net->weights_fp16_exact || !net->input_fp16_exact))
     return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
   if (o.precision != COSPDNN_FP16 && !net->input_fp16_exact)
@@ -470,6 +479,8 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   cfg.band_from = o.guard_band_from;
   cfg.use_mfma = o.use_mfma;
   cfg.mfma32 = o.fp32_mfma;
+  cfg.order = o.spmm_order;
+  cfg.groups_per_cu = o.mfma_groups_per_cu;
 
   HIP_TRY(hipEventRecord(net->ev[0], st));
   {
@@ -480,7 +491,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       GraphCache& gc = net->graph;
       const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
                        gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma &&
-                       gc.mfma32 == cfg.mfma32;
+                       gc.mfma32 == cfg.mfma32 && gc.order == cfg.order && gc.groups == cfg.groups_per_cu;
       if (!hit) {
         if (gc.exec) {
           (void)hipGraphExecDestroy(gc.exec);
@@ -505,6 +516,8 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
         gc.band_from = cfg.band_from;
         gc.use_mfma = cfg.use_mfma;
         gc.mfma32 = cfg.mfma32;
+        gc.order = cfg.order;
+        gc.groups = cfg.groups_per_cu;
       }
       HIP_TRY(hipGraphLaunch(gc.exec, st));
     } else {
@@ -683,6 +696,8 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->layer_timing = 0;
   o->use_graph = 0;
   o->dump_layer = -1;
+  o->spmm_order = 1;
+  o->mfma_groups_per_cu = 4;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {

@@ -73,11 +73,12 @@ __global__ void set_i32_kernel(int32_t* p, int32_t v) { *p = v; }
 // scan: work-group g owns positions [4096g, 4096g+4096); thread t of its 256
 // owns 16 consecutive positions and moves them with 16-byte loads and stores.
 // The flags are 0/1 bytes, so a popcount of (word & 0x01010101) counts four
-// of them.  A group reserves its slice of the compacted layout with one
-// atomicAdd on counts[layer+1] (zeroed when the pass starts), so the compacted
-// order follows the order in which groups finish, not the sample order; the
-// category ids are sorted on the host.  Thread offsets inside the group come
-// from a wave64 shuffle scan and a 4-entry LDS scan over the waves.
+// of them.  The compacted order is the sample order: group g starts at the
+// number of kept rows in positions [0, 4096g), which it counts itself from the
+// flag bytes (4096g bytes, L2-resident), so no group waits for another and the
+// layout is the same on every run.  When no row dies the layout is the identity,
+// which keeps the SpMM's 64-sample tiles contiguous and aligned.  Thread offsets
+// inside the group come from a wave64 shuffle scan and a 4-entry LDS scan.
 // --------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
 constexpr int kScanPer = 16;
@@ -92,14 +93,31 @@ __device__ __forceinline__ bool flag_at(const uint4& v, int i) {
   return ((w >> (8 * (i & 3))) & 0xFFu) != 0;
 }
 
+__device__ __forceinline__ uint4 kept_flags(const ScanArgs& a, int pos) {
+  uint4 v = *reinterpret_cast<const uint4*>(a.alive + pos);
+  if (a.retire_sat) {
+    const uint4 u = *reinterpret_cast<const uint4*>(a.unsat + pos);
+    v = make_uint4(v.x & u.x, v.y & u.y, v.z & u.z, v.w & u.w);
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   __shared__ int wsum[kScanThreads / 64];
-  __shared__ int base;
+  __shared__ int wpre[kScanThreads / 64];
   const int n = a.counts[a.layer];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (blockIdx.x == 0 && tid == 0 && a.work != nullptr) *a.work = 0;
-  const int pc = blockIdx.x * kScanTile + tid * kScanPer;  // multiple of 16: aligned, inside the ld-sized arrays
-  if (blockIdx.x * kScanTile >= n) return;                  // uniform per group
+  const int g0 = blockIdx.x * kScanTile;
+  const int pc = g0 + tid * kScanPer;  // multiple of 16: aligned, inside the ld-sized arrays
+  if (g0 >= n) return;                 // uniform per group
+  // kept rows before this group: positions [0, g0) are all < n
+  int before = 0;
+  for (int pos = tid * kScanPer; pos < g0; pos += kScanTile) before += count_flags16(kept_flags(a, pos));
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
+  if (lane == 0) wpre[wv] = before;
+
   const uint4 zero = {0u, 0u, 0u, 0u};
   uint4 va = zero, vu = zero, vb = zero, vr = zero;
   int cnt = 0;
@@ -127,18 +145,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   }
   if (lane == 63) wsum[wv] = x;
   __syncthreads();
-  if (tid == 0) {
-    int sum = 0;
-    for (int i = 0; i < kScanThreads / 64; ++i) {
-      const int t = wsum[i];
-      wsum[i] = sum;
-      sum += t;
-    }
-    base = atomicAdd(a.counts + a.layer + 1, sum);
+  int base = 0, wbase = 0, total = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / 64; ++i) {
+    base += wpre[i];
+    wbase += i < wv ? wsum[i] : 0;
+    total += wsum[i];
   }
-  __syncthreads();
+  // the last group writes the kept-row total
+  if (tid == 0 && g0 + kScanTile >= n) a.counts[a.layer + 1] = base + total;
   if (pc >= n) return;
-  int q = base + wsum[wv] + x - cnt;
+  int q = base + wbase + x - cnt;
   int qm[kScanPer];
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
@@ -174,27 +191,45 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 // of the grid, and the K chunks of an item are consecutive steps.
 // Per step the wave's 32 gathered input rows x 64 samples (4 KB) arrive in
 // registers, prefetched one step ahead (lane L loads row L/2, samples
-// 32(L%2)..+31 with four 16-byte loads; the row ids come two steps ahead), are
-// written to the wave's own LDS slice, and each lane reads its B fragment
-// X[8g..8g+7][sample] with two ds_read_b64_tr_b16.  D[out][sample] =
-// W^T[out][k] * X[k][sample] on v_mfma_f32_16x16x32_f16 (2 M-tiles x 4 N-tiles).
-// Epilogue: the pre-activations are staged in LDS as FP32 [32][64]; lane L then
-// owns sample L: it clamps, stores its 32 outputs (each store instruction covers
-// 64 consecutive target positions of one output row) and sets its flags.
+// 32(L%2)..+31 with four 16-byte loads; the row ids come two steps ahead), and
+// are written to the wave's own LDS slice.  The product is formed transposed,
+// D'[sample][out] = X^T[sample][k] * W[k][out], on v_mfma_f32_16x16x32_f16
+// (4 sample tiles x 2 output tiles): the X fragment of sample tile t holds the
+// samples 16(i/4) + 4t + i%4 (i < 16), read with two ds_read_b64_tr_b16 per tile,
+// so that the accumulators of lane j + 16g end up holding output j (and 16+j)
+// for the 16 consecutive samples 16g..16g+15.
+// Epilogue, when the 64 samples keep 64 consecutive positions starting at a
+// multiple of 8 (every tile of a layer in which no row died): bias, clamp and
+// FP16 conversion in registers, two 16-byte stores per output row and lane, and
+// the liveness bits OR-reduced over the 16 lanes of a row group.  Otherwise the
+// pre-activations are staged in LDS as FP32 [32][68] and lane L, owning sample L,
+// stores its 32 outputs one by one.
 // --------------------------------------------------------------------------
 constexpr int kWaveSamples = 64;
 constexpr int kWavesPerGroup = 4;
+constexpr int kStageLd = kWaveSamples + 4;  // FP32 staging row stride (floats): 16 rows cover all 64 banks
+constexpr int kOutLd = kWaveSamples + 8;    // FP16 output staging row stride (halves, 144 B)
 
 struct StepPos {
   int item, t, c, ch, ch_end;
   bool valid;
 };
 
-__device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int nblk, cint_ptr_t kptr, bool k1) {
+// Item order: block-column-major (cm) puts consecutive items on consecutive 64-sample
+// tiles of one block column, so the waves of a group and neighbouring groups read the
+// same gathered rows at adjacent addresses and write adjacent runs of the same output
+// rows; sample-tile-major puts them on different block columns of one tile.
+__device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int nblk, int ntiles, bool cm,
+                                           cint_ptr_t kptr, bool k1) {
   s.valid = item < nitems;
   s.item = item;
-  s.t = item / nblk;
-  s.c = item - s.t * nblk;
+  if (cm) {
+    s.c = item / ntiles;
+    s.t = item - s.c * ntiles;
+  } else {
+    s.t = item / nblk;
+    s.c = item - s.t * nblk;
+  }
   if (!s.valid) {
     s.ch = 0;
     s.ch_end = 1;
@@ -207,21 +242,70 @@ __device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int
   }
 }
 
-__device__ __forceinline__ void step_next(StepPos& s, int stride, int nitems, int nblk, cint_ptr_t kptr, bool k1) {
+__device__ __forceinline__ void step_next(StepPos& s, int stride, int nitems, int nblk, int ntiles, bool cm,
+                                          cint_ptr_t kptr, bool k1) {
   if (!s.valid) return;
   if (s.ch + 1 < s.ch_end) {
     ++s.ch;
     return;
   }
-  step_first(s, s.item + stride, nitems, nblk, kptr, k1);
+  step_first(s, s.item + stride, nitems, nblk, ntiles, cm, kptr, k1);
+}
+
+// OR over the 16 lanes of each DPP row (row_ror by 1, 2, 4, 8).
+__device__ __forceinline__ uint32_t row_or16(uint32_t x) {
+  x |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x121, 0xF, 0xF, false));
+  x |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x122, 0xF, 0xF, false));
+  x |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x124, 0xF, 0xF, false));
+  x |= static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x128, 0xF, 0xF, false));
+  return x;
+}
+
+// Per-sample flag bits of the epilogue's fast path.  acc[t][m][r] is output 16m + j of
+// sample 16g + 4t + r; bit 4t+r of the result (after the OR over the 16 lanes of the
+// row group) says whether any of the block's outputs of that sample is > 0 (kind 0),
+// < 32 (kind 1) or within `band` of 0 or 32 (kind 2).  ok0/ok1: the lane's outputs exist.
+template <int kKind, bool kFull>
+__device__ __forceinline__ uint32_t flag_bits(const float4_t (&acc)[4][2], bool ok0, bool ok1, float band) {
+  uint32_t bits = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float v0 = acc[t][0][r], v1 = acc[t][1][r];
+      bool hit;
+      if (kKind == 0) {
+        hit = kFull ? fmaxf(v0, v1) > 0.0f : ((ok0 && v0 > 0.0f) || (ok1 && v1 > 0.0f));
+      } else if (kKind == 1) {
+        hit = kFull ? fminf(v0, v1) < 32.0f : ((ok0 && v0 < 32.0f) || (ok1 && v1 < 32.0f));
+      } else {
+        hit = (ok0 && (fabsf(v0) < band || fabsf(v0 - 32.0f) < band)) ||
+              (ok1 && (fabsf(v1) < band || fabsf(v1 - 32.0f) < band));
+      }
+      bits |= hit ? (1u << (4 * t + r)) : 0u;
+    }
+  return row_or16(bits);
 }
 
 
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// Streaming 16-byte global accesses (activations are touched once per layer).
+__device__ __forceinline__ uint4 load_stream(const _Float16* p) {
+  const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_stream(_Float16* p, uint4 v) {
+  __builtin_nontemporal_store(u32x4_t{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4_t*>(p));
+}
+
 __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
-  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kWaveSamples];  // 8 KB per wave
+  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];  // 8.5 KB per wave
   const int n = a.counts[a.layer];
   const int nblk = w.nblk;
-  const int nitems = (n + kWaveSamples - 1) / kWaveSamples * nblk;
+  const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
+  const int nitems = ntiles * nblk;
+  const bool cm = a.order != 0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int stride = gridDim.x * kWavesPerGroup;
@@ -232,123 +316,210 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
 
   StepPos cur, nx1, nx2;
-  step_first(cur, blockIdx.x * kWavesPerGroup + wave, nitems, nblk, kptr, k1);
+  step_first(cur, blockIdx.x * kWavesPerGroup + wave, nitems, nblk, ntiles, cm, kptr, k1);
   if (!cur.valid) return;  // wave-uniform; nothing below synchronises across waves
   nx1 = cur;
-  step_next(nx1, stride, nitems, nblk, kptr, k1);
+  step_next(nx1, stride, nitems, nblk, ntiles, cm, kptr, k1);
   nx2 = nx1;
-  step_next(nx2, stride, nitems, nblk, kptr, k1);
+  step_next(nx2, stride, nitems, nblk, ntiles, cm, kptr, k1);
 
-  _Float16* tile = reinterpret_cast<_Float16*>(wlds[wave]);  // [32][64] FP16 input tile (first 4 KB)
-  float* stage = wlds[wave];                                 // [32][64] FP32 pre-activations
-  const int lrow = lane >> 1, lcol = (lane & 1) * 32;
+  _Float16* tile = reinterpret_cast<_Float16*>(wlds[wave]);  // [32][64] FP16 input tile (bytes 0..4095)
+  _Float16* ostage = tile + 32 * kWaveSamples;               // [32][kOutLd] FP16 outputs (bytes 4096..8703)
+  float* stage = wlds[wave];                                 // [32][kStageLd] FP32 pre-activations (slow path)
+  // global <-> registers: access k (0..3) of a lane covers row 8k + r8, samples c8..c8+7
+  // (16 B), so each load / store instruction moves 8 whole 128-byte row segments
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
   const int j = lane & 15, g = lane >> 4;
   const int32_t* __restrict__ gidx = w.gidx;
+  const int32_t* __restrict__ oidx = w.out_idx;
 
-  // Everything a step consumes is loaded one step ahead: its tile rows (d0..d3), its A
-  // fragments, the target position of this lane's sample; the row ids two steps ahead.
-  // After the final step the prefetch re-reads valid (unused) addresses.
-  int row_nx1 = gidx[(nx1.valid ? nx1.ch : cur.ch) * kChunk + lrow];
-  uint4 d0, d1, d2, d3;
-  half8_t a0, a1;
-  int qv;
+  // Everything a step consumes is loaded one step ahead: its tile rows (d0..d3), its W
+  // fragments, the target position of this lane's sample, the output ids; the row ids
+  // two steps ahead.  After the final step the prefetch re-reads valid (unused) addresses.
+  int rn0, rn1, rn2, rn3;  // row ids of step nx1
   {
-    const uint4* src = reinterpret_cast<const uint4*>(
-        X + static_cast<int64_t>(gidx[cur.ch * kChunk + lrow]) * a.ldx + cur.t * kWaveSamples + lcol);
-    d0 = src[0];
-    d1 = src[1];
-    d2 = src[2];
-    d3 = src[3];
-    a0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
-    a1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
-    qv = a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
+    const int ch = (nx1.valid ? nx1.ch : cur.ch) * kChunk + r8;
+    rn0 = gidx[ch];
+    rn1 = gidx[ch + 8];
+    rn2 = gidx[ch + 16];
+    rn3 = gidx[ch + 24];
   }
-  float4_t acc[2][4];
+  uint4 d0, d1, d2, d3;
+  half8_t w0, w1;
+  int qv, ov0, ov1, oa0, oa1, oa2, oa3;
+  {
+    const int ch = cur.ch * kChunk + r8;
+    const _Float16* base = X + cur.t * kWaveSamples + c8;
+    d0 = load_stream(base + static_cast<int64_t>(gidx[ch]) * a.ldx);
+    d1 = load_stream(base + static_cast<int64_t>(gidx[ch + 8]) * a.ldx);
+    d2 = load_stream(base + static_cast<int64_t>(gidx[ch + 16]) * a.ldx);
+    d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * a.ldx);
+    w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
+    w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
+    qv = a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
+    ov0 = oidx[cur.c * 32 + j];
+    ov1 = oidx[cur.c * 32 + 16 + j];
+    oa0 = oidx[cur.c * 32 + r8];
+    oa1 = oidx[cur.c * 32 + r8 + 8];
+    oa2 = oidx[cur.c * 32 + r8 + 16];
+    oa3 = oidx[cur.c * 32 + r8 + 24];
+  }
+  // accumulators start at the bias: D' + b comes out of the MFMA chain
+  const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
+  const bool full = w.full_blocks != 0;
+  const bool want_unsat = a.unsat_next != nullptr;
+  const bool want_band = a.band_next != nullptr;
+  float4_t acc[4][2];
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) acc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < 2; ++m) acc[t][m] = bias4;
 
   while (true) {
     const bool last = cur.ch + 1 == cur.ch_end;
     const int p_own = cur.t * kWaveSamples + lane;
     const int q_own = p_own < n ? qv : -1;
-    const half8_t ca0 = a0, ca1 = a1;
-    // this step's tile into the wave's LDS slice
+    const half8_t cw0 = w0, cw1 = w1;
+    const int cov0 = ov0, cov1 = ov1;
+    const int coa0 = oa0, coa1 = oa1, coa2 = oa2, coa3 = oa3;
+    const int cur_c = cur.c;
+    // this step's tile into the wave's LDS slice (waits for the loads issued last step)
     {
-      uint4* dst = reinterpret_cast<uint4*>(tile + lrow * kWaveSamples + lcol);
-      dst[0] = d0;
-      dst[1] = d1;
-      dst[2] = d2;
-      dst[3] = d3;
+      uint4* dst = reinterpret_cast<uint4*>(tile + r8 * kWaveSamples + c8);
+      dst[0 * 8 * kWaveSamples / 8] = d0;
+      dst[1 * 8 * kWaveSamples / 8] = d1;
+      dst[2 * 8 * kWaveSamples / 8] = d2;
+      dst[3 * 8 * kWaveSamples / 8] = d3;
     }
     // next step's loads
     {
       const int pt = nx1.valid ? nx1.t : cur.t;
       const int pch = nx1.valid ? nx1.ch : cur.ch;
-      const uint4* src = reinterpret_cast<const uint4*>(X + static_cast<int64_t>(row_nx1) * a.ldx + pt * kWaveSamples + lcol);
-      d0 = src[0];
-      d1 = src[1];
-      d2 = src[2];
-      d3 = src[3];
-      a0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
-      a1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
+      const int pcn = nx1.valid ? nx1.c : cur.c;
+      const _Float16* base = X + pt * kWaveSamples + c8;
+      d0 = load_stream(base + static_cast<int64_t>(rn0) * a.ldx);
+      d1 = load_stream(base + static_cast<int64_t>(rn1) * a.ldx);
+      d2 = load_stream(base + static_cast<int64_t>(rn2) * a.ldx);
+      d3 = load_stream(base + static_cast<int64_t>(rn3) * a.ldx);
+      w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
+      w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
       qv = a.qmap[min(pt * kWaveSamples + lane, n - 1)];
-      row_nx1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + lrow];
+      ov0 = oidx[pcn * 32 + j];
+      ov1 = oidx[pcn * 32 + 16 + j];
+      oa0 = oidx[pcn * 32 + r8];
+      oa1 = oidx[pcn * 32 + r8 + 8];
+      oa2 = oidx[pcn * 32 + r8 + 16];
+      oa3 = oidx[pcn * 32 + r8 + 24];
+      const int ch2 = (nx2.valid ? nx2.ch : pch) * kChunk + r8;
+      rn0 = gidx[ch2];
+      rn1 = gidx[ch2 + 8];
+      rn2 = gidx[ch2 + 16];
+      rn3 = gidx[ch2 + 24];
     }
-    // B fragments: ds_read_b64_tr_b16 gives lane i of each 16-lane group element i%4 of the
-    // 8 bytes read by lanes 4k+i/4, so lane m reads row 8g+4h+m/4 at samples 16nt+4(m%4)..+3
-    half8_t bfrag[4];
+    // X fragments: ds_read_b64_tr_b16 gives lane i of each 16-lane group element i%4 of
+    // the 8 bytes read by lanes 4k+i/4 (k < 4).  Lane m reads row 8g+4h+m/4 at samples
+    // 16(m%4)+4t..+3, so lane i receives rows 8g+4h+k of sample 16(i/4)+4t+i%4.
+    half8_t xfrag[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const _Float16* p0 = tile + (g * 8 + (j >> 2)) * kWaveSamples + nt * 16 + 4 * (j & 3);
+    for (int t = 0; t < 4; ++t) {
+      const _Float16* p0 = tile + (g * 8 + (j >> 2)) * kWaveSamples + 16 * (j & 3) + 4 * t;
       const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)p0);
       const fp16x4_t hi =
           __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)(p0 + 4 * kWaveSamples));
-      bfrag[nt] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
-                                          4, 5, 6, 7);
+      xfrag[t] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
+                                         4, 5, 6, 7);
     }
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ca0, bfrag[nt], acc[0][nt], 0, 0, 0);
-      acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ca1, bfrag[nt], acc[1][nt], 0, 0, 0);
+    for (int t = 0; t < 4; ++t) {
+      acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw0, acc[t][0], 0, 0, 0);
+      acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw1, acc[t][1], 0, 0, 0);
     }
     if (last) {
-      // D[i][j] of a 16x16 tile sits in lane j + 16*(i/4), element i%4: this lane holds
-      // outputs 16m + 4g + r of sample 16nt + j
+      // D'[i][jj] of a 16x16 tile sits in lane jj + 16*(i/4), element i%4: acc[t][m][r] of
+      // lane j + 16g is output 16m + j of sample 16g + 4t + r.
+      const int q0 = __builtin_amdgcn_readfirstlane(q_own);
+      const bool fast =
+          __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
+      if (fast) {
+        // FP16 outputs clamped with med3, transposed through LDS ([32][kOutLd]) so that the
+        // stores run 8 lanes x 16 B along each output row
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+        for (int m = 0; m < 2; ++m) {
+          uint32_t pk[8];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) stage[(m * 16 + g * 4 + r) * kWaveSamples + nt * 16 + j] = acc[m][nt][r] + a.bias;
-      int fl = 0;
-      if (q_own >= 0) {
-        const int32_t cbase = cur.c * 32;
-#pragma unroll 8
-        for (int o = 0; o < 32; ++o) {
-          const int out = out_idx[cbase + o];
-          const float v = stage[o * kWaveSamples + lane];
-          if (out >= 0) {
-            fl |= (v > 0.0f) ? 1 : 0;
-            fl |= (v < 32.0f) ? 2 : 0;
-            fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4 : 0;
-            Y[static_cast<int64_t>(out) * a.ldy + q_own] = static_cast<_Float16>(fminf(fmaxf(v, 0.0f), 32.0f));
+          for (int e = 0; e < 8; ++e) {
+            const int t = e / 2, r = 2 * (e % 2);
+            const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[t][m][r], 0.0f, 32.0f));
+            const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[t][m][r + 1], 0.0f, 32.0f));
+            pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
+                    (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
           }
+          uint4* o = reinterpret_cast<uint4*>(ostage + (16 * m + j) * kOutLd + 16 * g);
+          o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+          o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
         }
-        if (fl & 1) a.alive_next[q_own] = 1;
-        if (fl & 2) a.unsat_next[q_own] = 1;
-        if ((fl & 4) && a.band_next != nullptr) a.band_next[q_own] = 1;
+        {
+          const _Float16* orow = ostage + r8 * kOutLd + c8;
+          const uint4 v0 = *reinterpret_cast<const uint4*>(orow);
+          const uint4 v1 = *reinterpret_cast<const uint4*>(orow + 8 * kOutLd);
+          const uint4 v2 = *reinterpret_cast<const uint4*>(orow + 16 * kOutLd);
+          const uint4 v3 = *reinterpret_cast<const uint4*>(orow + 24 * kOutLd);
+          _Float16* ybase = Y + q0 + c8;
+          if (full || coa0 >= 0) store_stream(ybase + static_cast<int64_t>(coa0) * a.ldy, v0);
+          if (full || coa1 >= 0) store_stream(ybase + static_cast<int64_t>(coa1) * a.ldy, v1);
+          if (full || coa2 >= 0) store_stream(ybase + static_cast<int64_t>(coa2) * a.ldy, v2);
+          if (full || coa3 >= 0) store_stream(ybase + static_cast<int64_t>(coa3) * a.ldy, v3);
+        }
+        // liveness bits of the lane's 16 samples (bit 4t+r), OR-reduced over the 16 lanes of
+        // the row group: lane j + 16g owns sample 16g + j
+        const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
+        const uint32_t ba = full ? flag_bits<0, true>(acc, true, true, 0.0f) : flag_bits<0, false>(acc, ok0, ok1, 0.0f);
+        if ((ba >> j) & 1u) a.alive_next[q_own] = 1;
+        if (want_unsat) {
+          const uint32_t bu = flag_bits<1, false>(acc, ok0, ok1, 0.0f);
+          if ((bu >> j) & 1u) a.unsat_next[q_own] = 1;
+        }
+        if (want_band) {
+          const uint32_t bb = flag_bits<2, false>(acc, ok0, ok1, a.band);
+          if ((bb >> j) & 1u) a.band_next[q_own] = 1;
+        }
+      } else {
+        // rows died in this tile: stage the pre-activations, then each lane stores the 32
+        // outputs of its own sample
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            *reinterpret_cast<float4_t*>(stage + (16 * m + j) * kStageLd + 16 * g + 4 * t) = acc[t][m];
+          }
+        if (q_own >= 0) {
+          uint32_t fl = 0;
+          const int32_t cbase = cur_c * 32;
+#pragma unroll 8
+          for (int o = 0; o < 32; ++o) {
+            const int out = out_idx[cbase + o];
+            const float v = stage[o * kStageLd + lane];
+            if (out >= 0) {
+              fl |= v > 0.0f ? 1u : 0u;
+              fl |= v < 32.0f ? 2u : 0u;
+              fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4u : 0u;
+              Y[static_cast<int64_t>(out) * a.ldy + q_own] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(v, 0.0f, 32.0f));
+            }
+          }
+          if (fl & 1u) a.alive_next[q_own] = 1;
+          if (want_unsat && (fl & 2u)) a.unsat_next[q_own] = 1;
+          if (want_band && (fl & 4u)) a.band_next[q_own] = 1;
+        }
       }
 #pragma unroll
-      for (int m = 0; m < 2; ++m)
+      for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[m][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < 2; ++m) acc[t][m] = bias4;
     }
     if (!nx1.valid) break;
     cur = nx1;
     nx1 = nx2;
-    step_next(nx2, stride, nitems, nblk, kptr, k1);
+    step_next(nx2, stride, nitems, nblk, ntiles, cm, kptr, k1);
   }
 }
 

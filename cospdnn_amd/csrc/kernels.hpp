@@ -30,6 +30,7 @@ struct DevLayer {
   const _Float16* wt = nullptr;
   const float* wt32 = nullptr;       // FP32 tile image (MFMA FP32 kernel)
   int one_chunk = 0;                 // every block column has exactly one K chunk (kptr[c] = c)
+  int full_blocks = 0;               // every out_idx slot holds an output (no padding slots)
 };
 
 struct StepArgs {
@@ -42,10 +43,11 @@ struct StepArgs {
   int layer = 0;                     // n = counts[layer]
   const int32_t* qmap = nullptr;
   uint8_t* alive_next = nullptr;
-  uint8_t* unsat_next = nullptr;
+  uint8_t* unsat_next = nullptr;     // MFMA FP16 kernel: may be null (flag not needed after this layer)
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
   float band = 0.0f;
   int32_t* work = nullptr;           // work-item counter of the MFMA kernel (zeroed by the scan)
+  int order = 1;                     // MFMA kernel: 0 = items in sample-tile-major order, 1 = block-column-major
 };
 
 struct ScanArgs {
@@ -81,7 +83,8 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota_i32(int32_t* p, int64_t n, hipStream_t st);
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st);
-// max_rows: upper bound of counts[layer] (the grid covers it; counts[layer+1] must be 0 beforehand)
+// max_rows: upper bound of counts[layer] (the grid covers it).  counts[layer+1] must be 0
+// beforehand: no group writes it when counts[layer] is 0.
 hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st);
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
