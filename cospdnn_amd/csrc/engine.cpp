@@ -197,12 +197,15 @@ struct cospdnn_net {
   int input_mode = 0;  // 0 none, 1 host image, 2 generated
   std::vector<_Float16> input_img;  // [N][ld_store]
   bool input_fp16_exact = true;
+  bool input_binary = true;  // every input value is 0 or 1
   uint64_t input_seed = 0;
   // device side
   bool uploaded = false;
   std::vector<std::unique_ptr<LayerStore>> dl;
   DevBuf store;
   int64_t ld_store = 0;
+  DevBuf packed;       // binary inputs: [S][words] bit-packed sample-major copy of the store
+  int words = 0;
   RunBufs main, rec;
   int64_t nnz_total = 0;
   bool weights_fp16_exact = true;
@@ -268,6 +271,10 @@ struct RunCfg {
   int order = 1;    // MFMA SpMM item order
   int groups_per_cu = kMfmaGroupsPerCu;
   int retire_sat = 0;  // guard passes: saturated rows leave the pass once all-32 is a fixed point of the rest
+  int drop_flagged = 0;  // rows flagged at the checkpoint leave the pass (a later pass decides them)
+  // Early stop (passes that are not captured in a graph): after the scan of layer check_from,
+  // and every check_every layers after it, read the kept-row count and stop once it is 0.
+  int check_from = -1, check_every = 16;
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -329,8 +336,15 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.retire_sat = (cfg.retire_sat && l < L && sat_from[l]) ? 1 : 0;
     sa.ret_ids = rb.ret_ids.as<int32_t>();
     sa.ret_count = rb.ret_count.as<int32_t>();
+    sa.drop_flagged = cfg.drop_flagged;
     HIP_TRY(launch_scan(sa, S_run, st));
     if (l == L) break;
+    if (cfg.check_from >= 0 && l >= cfg.check_from && (l - cfg.check_from) % std::max(1, cfg.check_every) == 0) {
+      int32_t kept = 0;
+      HIP_TRY(hipMemcpyAsync(&kept, rb.counts.as<int32_t>() + l + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (kept == 0) break;  // counts[l+2..L+1] stay 0: no live rows, only retired ones
+    }
 
     const LayerStore& ls = *net->dl[l];
     StepArgs a;
@@ -388,6 +402,17 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   return COSPDNN_OK;
 }
 
+// Input rows ids[0..*count) of the network into dst ([N][ldd] FP16), stream-ordered.
+int gather_input_rows(cospdnn_net* net, const int32_t* ids, const int32_t* count, int64_t max_rows, _Float16* dst,
+                      int64_t ldd) {
+  hipStream_t st = net->eng->stream;
+  if (net->packed.p)
+    HIP_TRY(launch_gather_packed(net->packed.as<uint32_t>(), net->words, ids, count, max_rows, dst, ldd, net->N, st));
+  else
+    HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, ids, count, max_rows, dst, ldd, net->N, st));
+  return COSPDNN_OK;
+}
+
 // Host view of a finished pass (call after the stream is synchronised).
 struct PassOut {
   std::vector<int32_t> counts;  // L+2
@@ -426,8 +451,8 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   CALL_TRY(ensure_bufs(rb, nrows, net->L, net->N, COSPDNN_FP64, true, st));
   HIP_TRY(hipMemcpyAsync(rb.flag_ids.p, ids.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice, st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), nrows, st));
-  HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, rb.flag_ids.as<int32_t>(),
-                             rb.flag_count.as<int32_t>(), nrows, rb.in16.as<_Float16>(), rb.ld, net->N, st));
+  CALL_TRY(gather_input_rows(net, rb.flag_ids.as<int32_t>(), rb.flag_count.as<int32_t>(), nrows, rb.in16.as<_Float16>(),
+                             rb.ld));
   RunCfg cfg;
   cfg.precision = COSPDNN_FP64;
   cfg.guard_layer = -1;
@@ -436,6 +461,8 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   cfg.use_mfma = 0;
   cfg.mfma32 = 0;
   cfg.retire_sat = 1;
+  cfg.check_from = 16;
+  cfg.check_every = 16;
   int side = 0;
   CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, &side));
   HIP_TRY(hipStreamSynchronize(st));
@@ -538,9 +565,8 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       n_flagged = nflag;
       CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
       HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
-      HIP_TRY(launch_gather_rows(net->store.as<_Float16>(), net->ld_store, net->main.flag_ids.as<int32_t>(),
-                                 net->main.flag_count.as<int32_t>(), nflag, net->rec.in16.as<_Float16>(), net->rec.ld,
-                                 N, st));
+      CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), net->main.flag_count.as<int32_t>(), nflag,
+                                 net->rec.in16.as<_Float16>(), net->rec.ld));
       RunCfg rcfg;
       rcfg.precision = COSPDNN_FP32;
       rcfg.guard_layer = o.guard_layer32;
@@ -549,6 +575,9 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       rcfg.use_mfma = 0;
       rcfg.mfma32 = 1;
       rcfg.retire_sat = 1;
+      rcfg.drop_flagged = 1;  // the rows handed to the FP64 tier are not carried further here
+      rcfg.check_from = o.guard_layer32 >= 0 ? std::min(o.guard_layer32, L) : 16;
+      rcfg.check_every = 16;
       int rside = 0;
       CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, &rside));
       HIP_TRY(hipStreamSynchronize(st));
@@ -826,12 +855,14 @@ int cospdnn_net_set_input_dense(cospdnn_net* net, const float* y0) {
   const int64_t ld = net->ld_store;
   net->input_img.assign(static_cast<size_t>(N) * ld, static_cast<_Float16>(0.0f));
   net->input_fp16_exact = true;
+  net->input_binary = true;
   for (int64_t s = 0; s < net->S; ++s)
     for (int nn = 0; nn < N; ++nn) {
       const float v = y0[s * N + nn];
       if (!std::isfinite(v)) return fail(COSPDNN_ERR_FORMAT, "input not finite");
       const _Float16 h = static_cast<_Float16>(v);
       if (static_cast<float>(h) != v) net->input_fp16_exact = false;
+      if (v != 0.0f && v != 1.0f) net->input_binary = false;
       net->input_img[static_cast<size_t>(nn) * ld + s] = h;
     }
   net->input_mode = 1;
@@ -846,12 +877,14 @@ int cospdnn_net_set_input_coo(cospdnn_net* net, int64_t nnz, const int64_t* samp
   const int64_t ld = net->ld_store;
   net->input_img.assign(static_cast<size_t>(N) * ld, static_cast<_Float16>(0.0f));
   net->input_fp16_exact = true;
+  net->input_binary = true;
   for (int64_t i = 0; i < nnz; ++i) {
     if (sample[i] < 0 || sample[i] >= net->S || neuron[i] < 0 || neuron[i] >= N)
       return fail(COSPDNN_ERR_FORMAT, "input entry out of range");
     if (!std::isfinite(val[i])) return fail(COSPDNN_ERR_FORMAT, "input not finite");
     const _Float16 h = static_cast<_Float16>(val[i]);
     if (static_cast<float>(h) != val[i]) net->input_fp16_exact = false;
+    if (val[i] != 0.0f && val[i] != 1.0f) net->input_binary = false;
     net->input_img[static_cast<size_t>(neuron[i]) * ld + sample[i]] = h;
   }
   net->input_mode = 1;
@@ -864,6 +897,7 @@ int cospdnn_net_generate_input(cospdnn_net* net, uint64_t seed) {
   net->input_mode = 2;
   net->input_seed = seed;
   net->input_fp16_exact = true;
+  net->input_binary = true;
   net->input_img.clear();
   return COSPDNN_OK;
 }
@@ -905,6 +939,14 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
   } else {
     HIP_TRY(hipMemsetAsync(net->store.p, 0, net->store.bytes, net->eng->stream));
     HIP_TRY(launch_gen_input(net->store.as<_Float16>(), net->ld_store, net->N, net->S, net->input_seed, net->eng->stream));
+    HIP_TRY(hipStreamSynchronize(net->eng->stream));
+  }
+  net->packed.reset();
+  net->words = (net->N + 31) / 32;
+  if (net->input_binary) {
+    CALL_TRY(dalloc(net->packed, sizeof(uint32_t) * static_cast<size_t>(net->words) * static_cast<size_t>(net->S)));
+    HIP_TRY(launch_pack_rows(net->store.as<_Float16>(), net->ld_store, net->N, net->S, net->packed.as<uint32_t>(),
+                             net->words, net->eng->stream));
     HIP_TRY(hipStreamSynchronize(net->eng->stream));
   }
   net->uploaded = true;

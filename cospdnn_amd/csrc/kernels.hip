@@ -93,13 +93,20 @@ __device__ __forceinline__ bool flag_at(const uint4& v, int i) {
   return ((w >> (8 * (i & 3))) & 0xFFu) != 0;
 }
 
+// Rows a scan keeps, four flag bytes at a time: alive, minus retired saturated rows
+// (retire_sat: alive & !unsat), minus flagged undecided rows (drop_flagged at the
+// checkpoint: alive & unsat).
+__device__ __forceinline__ uint32_t kept_word(const ScanArgs& a, uint32_t al, uint32_t un) {
+  uint32_t k = al;
+  if (a.retire_sat) k &= un;
+  if (a.drop_flagged && a.flag_undecided) k &= ~(al & un);
+  return k & 0x01010101u;
+}
 __device__ __forceinline__ uint4 kept_flags(const ScanArgs& a, int pos) {
-  uint4 v = *reinterpret_cast<const uint4*>(a.alive + pos);
-  if (a.retire_sat) {
-    const uint4 u = *reinterpret_cast<const uint4*>(a.unsat + pos);
-    v = make_uint4(v.x & u.x, v.y & u.y, v.z & u.z, v.w & u.w);
-  }
-  return v;
+  const uint4 v = *reinterpret_cast<const uint4*>(a.alive + pos);
+  uint4 u = make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
+  if (a.retire_sat || (a.drop_flagged && a.flag_undecided)) u = *reinterpret_cast<const uint4*>(a.unsat + pos);
+  return make_uint4(kept_word(a, v.x, u.x), kept_word(a, v.y, u.y), kept_word(a, v.z, u.z), kept_word(a, v.w, u.w));
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
@@ -119,18 +126,18 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   if (lane == 0) wpre[wv] = before;
 
   const uint4 zero = {0u, 0u, 0u, 0u};
-  uint4 va = zero, vu = zero, vb = zero, vr = zero;
+  uint4 va = zero, vu = zero, vb = zero, vr = zero, vf = zero;
   int cnt = 0;
   if (pc < n) {
-    va = *reinterpret_cast<const uint4*>(a.alive + pc);
+    const uint4 v0 = *reinterpret_cast<const uint4*>(a.alive + pc);
     vu = *reinterpret_cast<const uint4*>(a.unsat + pc);
     if (a.band != nullptr) vb = *reinterpret_cast<const uint4*>(a.band + pc);
-    if (a.retire_sat) {
-      // flags are 0/1 bytes: retired = alive & !unsat, kept = alive & unsat
-      vr = make_uint4(va.x & ~vu.x & 0x01010101u, va.y & ~vu.y & 0x01010101u, va.z & ~vu.z & 0x01010101u,
-                      va.w & ~vu.w & 0x01010101u);
-      va = make_uint4(va.x & vu.x, va.y & vu.y, va.z & vu.z, va.w & vu.w);
-    }
+    // flags are 0/1 bytes.  retired = alive & !unsat; flagged (checkpoint) = alive & unsat
+    if (a.retire_sat)
+      vr = make_uint4(v0.x & ~vu.x & 0x01010101u, v0.y & ~vu.y & 0x01010101u, v0.z & ~vu.z & 0x01010101u,
+                      v0.w & ~vu.w & 0x01010101u);
+    if (a.flag_undecided) vf = make_uint4(v0.x & vu.x, v0.y & vu.y, v0.z & vu.z, v0.w & vu.w);
+    va = make_uint4(kept_word(a, v0.x, vu.x), kept_word(a, v0.y, vu.y), kept_word(a, v0.z, vu.z), kept_word(a, v0.w, vu.w));
     if (pc + kScanPer <= n) {
       cnt = count_flags16(va);
     } else {
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
       ++q;
     }
     if (p < n && flag_at(vr, i)) a.ret_ids[atomicAdd(a.ret_count, 1)] = a.ids_in[p];
-    const bool fl = p < n && (flag_at(vb, i) || (a.flag_undecided && al && flag_at(vu, i)));
+    const bool fl = p < n && (flag_at(vb, i) || flag_at(vf, i));
     if (fl) {
       const int32_t id = a.ids_in[p];
       if (a.flag_mark[id] == 0) {
@@ -738,6 +745,37 @@ __global__ void input_alive_kernel(const _Float16* __restrict__ store, int64_t l
   alive[s] = static_cast<uint8_t>(any);
 }
 
+__global__ __launch_bounds__(256) void pack_rows_kernel(const _Float16* __restrict__ store, int64_t ld, int N,
+                                                        int64_t S, uint32_t* __restrict__ packed, int words) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  for (int wd = blockIdx.y; wd < words; wd += gridDim.y) {
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; ++b) {
+      const int nn = 32 * wd + b;
+      if (nn < N && static_cast<float>(store[static_cast<int64_t>(nn) * ld + s]) != 0.0f) bits |= 1u << b;
+    }
+    packed[s * words + wd] = bits;
+  }
+}
+
+// thread = selected row (consecutive rows write consecutive positions of each neuron row)
+__global__ __launch_bounds__(256) void gather_packed_kernel(const uint32_t* __restrict__ packed, int words,
+                                                            const int32_t* __restrict__ ids,
+                                                            const int32_t* __restrict__ count,
+                                                            _Float16* __restrict__ dst, int64_t ldd, int N) {
+  const int f = *count;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= f) return;
+  const int64_t id = ids[r];
+  for (int wd = blockIdx.y; wd < words; wd += gridDim.y) {
+    const uint32_t bits = packed[id * words + wd];
+    const int nb = min(32, N - 32 * wd);
+    for (int b = 0; b < nb; ++b)
+      dst[static_cast<int64_t>(32 * wd + b) * ldd + r] = (bits >> b) & 1u ? static_cast<_Float16>(1.0f) : static_cast<_Float16>(0.0f);
+  }
+}
+
 __global__ __launch_bounds__(256) void gather_rows_kernel(const _Float16* __restrict__ src, int64_t lds,
                                                           const int32_t* __restrict__ ids,
                                                           const int32_t* __restrict__ count,
@@ -830,6 +868,20 @@ hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* i
   return hipGetLastError();
 }
 
+hipError_t launch_pack_rows(const _Float16* store, int64_t ld, int N, int64_t S, uint32_t* packed, int words,
+                            hipStream_t st) {
+  if (S <= 0 || words <= 0) return hipSuccess;
+  dim3 grid(static_cast<unsigned>((S + 255) / 256), static_cast<unsigned>(words < 256 ? words : 256));
+  hipLaunchKernelGGL(pack_rows_kernel, grid, dim3(256), 0, st, store, ld, N, S, packed, words);
+  return hipGetLastError();
+}
+hipError_t launch_gather_packed(const uint32_t* packed, int words, const int32_t* ids, const int32_t* count,
+                                int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st) {
+  if (max_rows <= 0 || words <= 0) return hipSuccess;
+  dim3 grid(static_cast<unsigned>((max_rows + 255) / 256), static_cast<unsigned>(words < 512 ? words : 512));
+  hipLaunchKernelGGL(gather_packed_kernel, grid, dim3(256), 0, st, packed, words, ids, count, dst, ldd, N);
+  return hipGetLastError();
+}
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st) {
   if (tin == Dtype::F16)
     hipLaunchKernelGGL(spmm_mfma32_f16in_kernel, dim3(grid), dim3(256), 0, st, a, w);

@@ -73,6 +73,8 @@ struct ScanArgs {
   int retire_sat = 0;
   int32_t* ret_ids = nullptr;
   int32_t* ret_count = nullptr;
+  // With flag_undecided: the flagged rows also leave the pass (a later pass decides them).
+  int drop_flagged = 0;
 };
 
 enum class Dtype : int { F16 = 0, F32 = 1, F64 = 2 };
@@ -95,6 +97,12 @@ hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, i
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,
                               int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
+// Binary inputs: bit-packed sample-major copy [S][words] (words = ceil(N/32)) of the
+// neuron-major FP16 store, and the gather of selected samples from it.
+hipError_t launch_pack_rows(const _Float16* store, int64_t ld, int N, int64_t S, uint32_t* packed, int words,
+                            hipStream_t st);
+hipError_t launch_gather_packed(const uint32_t* packed, int words, const int32_t* ids, const int32_t* count,
+                                int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
 
 // Kernel attributes set once per process (dynamic LDS limits).
 hipError_t configure_kernels();
