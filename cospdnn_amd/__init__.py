@@ -13,9 +13,16 @@ from ._lib import (  # noqa: F401
     GEN_BANDED,
     GEN_RANDOM,
     LIB_PATH,
+    PLAN_AUTO,
+    PLAN_MFMA,
+    PLAN_SIMT,
+    TUNE_FEATURES,
     CospdnnError,
     Engine,
+    LayerPlan,
+    Mlp,
     Net,
+    candidates,
     build,
     default_opts,
     generate_input_rows,
@@ -25,6 +32,8 @@ from ._lib import (  # noqa: F401
     lib,
     ref_bcsc_to_csc,
     run,
+    tune_exhaustive,
+    tune_samples,
     version,
 )
 from .configs import CONFIGS, MASTER_SEED, config_nets  # noqa: F401

@@ -66,6 +66,17 @@ class Result(ctypes.Structure):
     ]
 
 
+PLAN_AUTO, PLAN_MFMA, PLAN_SIMT = 0, 1, 2
+TUNE_FEATURES = 12
+
+
+class LayerPlan(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int), ("order", ctypes.c_int), ("groups_per_cu", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.kernel, self.order, self.groups_per_cu)
+
+
 # every symbol include/cospdnn.h declares
 EXPORTS = [
     "cospdnn_version", "cospdnn_strerror", "cospdnn_last_error", "cospdnn_default_opts",
@@ -75,6 +86,10 @@ EXPORTS = [
     "cospdnn_net_generate_input", "cospdnn_net_upload", "cospdnn_run",
     "cospdnn_generate_layer", "cospdnn_generate_input_rows", "cospdnn_layer_stats", "cospdnn_layer_tiles_decode",
     "cospdnn_ref_bcsc_to_csc",
+    "cospdnn_net_set_plan", "cospdnn_num_candidates", "cospdnn_candidate", "cospdnn_net_layer_features",
+    "cospdnn_tune_exhaustive", "cospdnn_mlp_create", "cospdnn_mlp_destroy", "cospdnn_mlp_num_params",
+    "cospdnn_mlp_get_params", "cospdnn_mlp_set_params", "cospdnn_mlp_loss_grad", "cospdnn_mlp_train",
+    "cospdnn_mlp_predict", "cospdnn_tune_samples", "cospdnn_tune_mlp",
 ]
 
 
@@ -136,6 +151,26 @@ def lib():
         L.cospdnn_ref_bcsc_to_csc.argtypes = [ctypes.c_int, _c_f32p, ctypes.c_int64, _c_i32p, ctypes.c_int64,
                                               _c_i32p, ctypes.c_int64, _c_i32p, ctypes.c_int64, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int64, _c_i32p, _c_i32p, _c_f32p]
+        _pl = ctypes.POINTER(LayerPlan)
+        L.cospdnn_net_set_plan.argtypes = [ctypes.c_void_p, _pl]
+        L.cospdnn_num_candidates.argtypes = []
+        L.cospdnn_candidate.argtypes = [ctypes.c_int, _pl]
+        L.cospdnn_net_layer_features.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _pl, _c_f32p]
+        L.cospdnn_tune_exhaustive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), ctypes.c_int, _c_f32p, _c_i64p, _pl]
+        L.cospdnn_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_void_p)]
+        L.cospdnn_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.cospdnn_mlp_destroy.restype = None
+        L.cospdnn_mlp_num_params.argtypes = [ctypes.c_void_p]
+        L.cospdnn_mlp_get_params.argtypes = [ctypes.c_void_p, _c_f32p]
+        L.cospdnn_mlp_set_params.argtypes = [ctypes.c_void_p, _c_f32p]
+        L.cospdnn_mlp_loss_grad.argtypes = [ctypes.c_void_p, _c_f32p, _c_f32p, ctypes.c_int, _c_f32p, _c_f32p]
+        L.cospdnn_mlp_train.argtypes = [ctypes.c_void_p, _c_f32p, _c_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        _c_f32p]
+        L.cospdnn_mlp_predict.argtypes = [ctypes.c_void_p, _c_f32p, ctypes.c_int, _c_f32p]
+        L.cospdnn_tune_samples.argtypes = [ctypes.c_void_p, _c_f32p, _c_i64p, _c_f32p, _c_f32p, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int)]
+        L.cospdnn_tune_mlp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _c_i64p, _pl, _c_f32p]
         _lib = L
     return _lib
 
@@ -231,6 +266,24 @@ class Net:
     def upload(self, opts=None):
         o = opts if opts is not None else default_opts()
         _check(lib().cospdnn_net_upload(self.h, ctypes.byref(o)), "cospdnn_net_upload")
+
+    def set_plan(self, plan):
+        """plan: sequence of L (kernel, order, groups_per_cu) tuples or LayerPlan, or None."""
+        if plan is None:
+            _check(lib().cospdnn_net_set_plan(self.h, None), "cospdnn_net_set_plan")
+            return
+        arr = (LayerPlan * self.L)()
+        for l, p in enumerate(plan):
+            t = p.as_tuple() if isinstance(p, LayerPlan) else tuple(p)
+            arr[l] = LayerPlan(*t)
+        _check(lib().cospdnn_net_set_plan(self.h, arr), "cospdnn_net_set_plan")
+
+    def layer_features(self, layer, live_rows, cand):
+        f = np.zeros(TUNE_FEATURES, dtype=np.float32)
+        c = cand if isinstance(cand, LayerPlan) else LayerPlan(*cand)
+        _check(lib().cospdnn_net_layer_features(self.h, layer, int(live_rows), ctypes.byref(c), _p(f, _c_f32p)),
+               "cospdnn_net_layer_features")
+        return f
 
     def close(self):
         if self.h:
@@ -351,3 +404,106 @@ def ref_bcsc_to_csc(N, values, row_idx, ptr, fuse=None, blocksize_c=16, blocksiz
                                          _p(cp, _c_i32p), _p(rw, _c_i32p), _p(vl, _c_f32p)), "cospdnn_ref_bcsc_to_csc")
     nnz = cp[N]
     return cp, rw[:nnz], vl[:nnz]
+
+
+def candidates():
+    out = []
+    for k in range(lib().cospdnn_num_candidates()):
+        c = LayerPlan()
+        _check(lib().cospdnn_candidate(k, ctypes.byref(c)), "cospdnn_candidate")
+        out.append(c.as_tuple())
+    return out
+
+
+def tune_exhaustive(net, opts=None, reps=3):
+    """Per-layer exhaustive tuning of the FP16 pass: returns (plan [L tuples], ms table
+    [ncand][L], live [L+1])."""
+    o = opts if opts is not None else default_opts()
+    nc = lib().cospdnn_num_candidates()
+    table = np.zeros((nc, net.L), dtype=np.float32)
+    live = np.zeros(net.L + 1, dtype=np.int64)
+    plan = (LayerPlan * max(net.L, 1))()
+    _check(lib().cospdnn_tune_exhaustive(net.h, ctypes.byref(o), reps, _p(table, _c_f32p), _p(live, _c_i64p), plan),
+           "cospdnn_tune_exhaustive")
+    return [plan[l].as_tuple() for l in range(net.L)], table, live
+
+
+def tune_samples(net, table, live):
+    """Cost-model training rows (features, ln(1000*ms + 1)) from an exhaustive table."""
+    nc = table.shape[0]
+    cap = nc * net.L
+    X = np.zeros((cap, TUNE_FEATURES), dtype=np.float32)
+    y = np.zeros(cap, dtype=np.float32)
+    n = ctypes.c_int()
+    t = np.ascontiguousarray(table, dtype=np.float32)
+    lv = np.ascontiguousarray(live, dtype=np.int64)
+    _check(lib().cospdnn_tune_samples(net.h, _p(t, _c_f32p), _p(lv, _c_i64p), _p(X, _c_f32p), _p(y, _c_f32p), cap,
+                                      ctypes.byref(n)), "cospdnn_tune_samples")
+    return X[:n.value], y[:n.value]
+
+
+class Mlp:
+    """Cost-model MLP on an engine's device (n_in -> hidden -> hidden -> 1, ReLU)."""
+
+    def __init__(self, engine, n_in=TUNE_FEATURES, hidden=64, seed=1):
+        self.engine = engine
+        self.n_in, self.hidden = n_in, hidden
+        self.h = ctypes.c_void_p()
+        _check(lib().cospdnn_mlp_create(engine.h, n_in, hidden, seed, ctypes.byref(self.h)), "cospdnn_mlp_create")
+
+    @property
+    def num_params(self):
+        return lib().cospdnn_mlp_num_params(self.h)
+
+    def get_params(self):
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().cospdnn_mlp_get_params(self.h, _p(p, _c_f32p)), "cospdnn_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        q = np.ascontiguousarray(p, dtype=np.float32)
+        assert q.size == self.num_params
+        _check(lib().cospdnn_mlp_set_params(self.h, _p(q, _c_f32p)), "cospdnn_mlp_set_params")
+
+    def loss_grad(self, X, y):
+        Xc = np.ascontiguousarray(X, dtype=np.float32)
+        yc = np.ascontiguousarray(y, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().cospdnn_mlp_loss_grad(self.h, _p(Xc, _c_f32p), _p(yc, _c_f32p), len(yc), _p(loss, _c_f32p),
+                                           _p(g, _c_f32p)), "cospdnn_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, X, y, steps=2000, lr=3e-3):
+        Xc = np.ascontiguousarray(X, dtype=np.float32)
+        yc = np.ascontiguousarray(y, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().cospdnn_mlp_train(self.h, _p(Xc, _c_f32p), _p(yc, _c_f32p), len(yc), steps, float(lr),
+                                       _p(loss, _c_f32p)), "cospdnn_mlp_train")
+        return float(loss[0])
+
+    def predict(self, X):
+        Xc = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(Xc.shape[0], dtype=np.float32)
+        _check(lib().cospdnn_mlp_predict(self.h, _p(Xc, _c_f32p), Xc.shape[0], _p(out, _c_f32p)), "cospdnn_mlp_predict")
+        return out
+
+    def plan(self, net, live):
+        """Per-layer argmin of the predicted times: (plan [L tuples], predicted ms [ncand][L])."""
+        nc = lib().cospdnn_num_candidates()
+        pred = np.zeros((nc, net.L), dtype=np.float32)
+        lv = np.ascontiguousarray(live, dtype=np.int64)
+        plan = (LayerPlan * max(net.L, 1))()
+        _check(lib().cospdnn_tune_mlp(net.h, self.h, _p(lv, _c_i64p), plan, _p(pred, _c_f32p)), "cospdnn_tune_mlp")
+        return [plan[l].as_tuple() for l in range(net.L)], pred
+
+    def close(self):
+        if self.h:
+            lib().cospdnn_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -19,49 +19,25 @@
 #include <thread>
 #include <vector>
 
-#include "cospdnn.h"
-#include "format.hpp"
+#include "engine_internal.hpp"
 #include "gen.hpp"
-#include "kernels.hpp"
 
 using namespace cospdnn;
+using namespace cospdnn_detail;
+
+namespace cospdnn_detail {
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                       \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(COSPDNN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define CALL_TRY(expr)        \
-  do {                        \
-    int rc_ = (expr);         \
-    if (rc_ != COSPDNN_OK) return rc_; \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <typename T>
-  T* as() const { return static_cast<T*>(p); }
-};
+const char* last_error_text() { return g_last_error.c_str(); }
 
 int dalloc(DevBuf& b, size_t bytes) {
   b.reset();
@@ -75,41 +51,6 @@ int dalloc(DevBuf& b, size_t bytes) {
   b.bytes = bytes;
   return COSPDNN_OK;
 }
-
-template <typename T>
-int dupload(DevBuf& b, const std::vector<T>& v) {
-  CALL_TRY(dalloc(b, sizeof(T) * v.size()));
-  if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  return COSPDNN_OK;
-}
-
-int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
-
-struct LayerStore {
-  DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32;
-  DevLayer view;
-  bool has_tiles = false;
-  double waste = 0.0;
-  bool fp16_exact = true;
-  bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
-  int64_t nnz = 0;
-};
-
-struct RunBufs {
-  int64_t cap = 0, ld = 0;
-  int L = 0;
-  DevBuf act16[2], act32[2], act64[2];
-  DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
-  DevBuf ids;    // 2 x ld int32
-  DevBuf qmap;   // ld int32
-  DevBuf counts; // L+2 int32
-  DevBuf flag_ids, flag_count, flag_mark;
-  DevBuf work;   // MFMA work-queue counter
-  DevBuf in16;   // recompute input
-  DevBuf ret_ids, ret_count;  // rows retired as saturated (guard passes)
-  uint8_t* f(int side, int kind) const { return flags.as<uint8_t>() + static_cast<int64_t>(side * 3 + kind) * ld; }
-  int32_t* id(int side) const { return ids.as<int32_t>() + static_cast<int64_t>(side) * ld; }
-};
 
 int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, hipStream_t st) {
   if (rb.cap < cap || rb.L != L) {
@@ -162,63 +103,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
   return COSPDNN_OK;
 }
 
-}  // namespace
-
-struct cospdnn_engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int cu_count = 0;
-  int64_t hbm_bytes = 0;
-  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
-      bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
-};
-
-struct GraphCache {
-  hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
-  float band = 0.0f;
-  ~GraphCache() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-  }
-};
-
-struct cospdnn_net {
-  cospdnn_engine* eng = nullptr;
-  int N = 0, L = 0;
-  int64_t S = 0;
-  float bias = 0.0f;
-  // host-side definition
-  std::vector<CscLayer> layers;
-  std::vector<char> layer_set;
-  bool gen_layers = false;
-  int gen_kind = 0, gen_K = 32;
-  float gen_wval = 0.0625f;
-  uint64_t gen_seed = 0;
-  int input_mode = 0;  // 0 none, 1 host image, 2 generated
-  std::vector<_Float16> input_img;  // [N][ld_store]
-  bool input_fp16_exact = true;
-  bool input_binary = true;  // every input value is 0 or 1
-  uint64_t input_seed = 0;
-  // device side
-  bool uploaded = false;
-  std::vector<std::unique_ptr<LayerStore>> dl;
-  DevBuf store;
-  int64_t ld_store = 0;
-  DevBuf packed;       // binary inputs: [S][words] bit-packed sample-major copy of the store
-  int words = 0;
-  RunBufs main, rec;
-  int64_t nnz_total = 0;
-  bool weights_fp16_exact = true;
-  GraphCache graph;
-  RunBufs rep;                       // FP64 pass over the rows the FP32 pass leaves undecided
-  std::vector<hipEvent_t> layer_ev;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-  ~cospdnn_net() {
-    for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ev)
-      if (e) (void)hipEventDestroy(e);
-  }
-};
+}  // namespace cospdnn_detail
 
 namespace {
 
@@ -366,11 +251,23 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.work = rb.work.as<int32_t>();
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
     const int nblk = ls.view.nblk;
-    if (fp16 && ls.has_tiles && cfg.use_mfma) {
+    // FP16 pass: the network's per-layer plan (if any) overrides the options' choice
+    bool mfma_l = fp16 && ls.has_tiles && cfg.use_mfma;
+    int order_l = cfg.order, groups_l = cfg.groups_per_cu;
+    if (fp16 && !net->plan.empty()) {
+      const cospdnn_layer_plan& pl = net->plan[l];
+      if (pl.kernel == COSPDNN_PLAN_SIMT) mfma_l = false;
+      if (pl.kernel == COSPDNN_PLAN_MFMA) {
+        mfma_l = ls.has_tiles;
+        order_l = pl.order;
+        groups_l = pl.groups_per_cu;
+      }
+    }
+    if (mfma_l) {
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
-      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{cfg.groups_per_cu} * e->cu_count)));
-      a.order = cfg.order;
+      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{groups_l} * e->cu_count)));
+      a.order = order_l;
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
       if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
       HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
@@ -472,6 +369,10 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   return COSPDNN_OK;
 }
 
+}  // namespace
+
+namespace cospdnn_detail {
+
 int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   cospdnn_engine* e = net->eng;
   hipStream_t st = e->stream;
@@ -518,7 +419,8 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
       GraphCache& gc = net->graph;
       const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
                        gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma &&
-                       gc.mfma32 == cfg.mfma32 && gc.order == cfg.order && gc.groups == cfg.groups_per_cu;
+                       gc.mfma32 == cfg.mfma32 && gc.order == cfg.order && gc.groups == cfg.groups_per_cu &&
+                       gc.plan_version == net->plan_version;
       if (!hit) {
         if (gc.exec) {
           (void)hipGraphExecDestroy(gc.exec);
@@ -545,6 +447,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
         gc.mfma32 = cfg.mfma32;
         gc.order = cfg.order;
         gc.groups = cfg.groups_per_cu;
+        gc.plan_version = net->plan_version;
       }
       HIP_TRY(hipGraphLaunch(gc.exec, st));
     } else {
@@ -629,8 +532,10 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   for (int l = 0; l < L; ++l) {
     const double lv = live[l] >= 0 ? static_cast<double>(live[l]) : static_cast<double>(S);
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
-    if (net->dl[l]->has_tiles && ((o.precision == COSPDNN_FP16 && cfg.use_mfma) || (o.precision == COSPDNN_FP32 && cfg.mfma32)))
-      ++mfma_layers;
+    bool m = net->dl[l]->has_tiles && ((o.precision == COSPDNN_FP16 && cfg.use_mfma) || (o.precision == COSPDNN_FP32 && cfg.mfma32));
+    if (o.precision == COSPDNN_FP16 && !net->plan.empty() && net->plan[l].kernel != COSPDNN_PLAN_AUTO)
+      m = net->dl[l]->has_tiles && net->plan[l].kernel == COSPDNN_PLAN_MFMA;
+    mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;
   res->mfma_layers = mfma_layers;
@@ -686,7 +591,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   return COSPDNN_OK;
 }
 
-}  // namespace
+}  // namespace cospdnn_detail
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -710,7 +615,7 @@ const char* cospdnn_strerror(int code) {
   }
 }
 
-const char* cospdnn_last_error(void) { return g_last_error.c_str(); }
+const char* cospdnn_last_error(void) { return last_error_text(); }
 
 void cospdnn_default_opts(cospdnn_opts* o) {
   if (!o) return;

@@ -1,0 +1,161 @@
+// Internal types of the engine, shared by engine.cpp (forward passes, C ABI) and
+// tuner.cpp (autotuner and cost-model MLP).  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cospdnn.h"
+#include "format.hpp"
+#include "kernels.hpp"
+
+namespace cospdnn_detail {
+
+using namespace cospdnn;
+
+// Records the detail of a failure for cospdnn_last_error() and returns code.
+int fail(int code, const std::string& msg);
+const char* last_error_text();
+
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(COSPDNN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CALL_TRY(expr)        \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != COSPDNN_OK) return rc_; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+int dalloc(DevBuf& b, size_t bytes);
+
+template <typename T>
+inline int dupload(DevBuf& b, const std::vector<T>& v) {
+  CALL_TRY(dalloc(b, sizeof(T) * v.size()));
+  if (!v.empty()) HIP_TRY(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return COSPDNN_OK;
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct LayerStore {
+  DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32;
+  DevLayer view;
+  bool has_tiles = false;
+  double waste = 0.0;
+  bool fp16_exact = true;
+  bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
+  int64_t nnz = 0;
+};
+
+struct RunBufs {
+  int64_t cap = 0, ld = 0;
+  int L = 0;
+  DevBuf act16[2], act32[2], act64[2];
+  DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
+  DevBuf ids;    // 2 x ld int32
+  DevBuf qmap;   // ld int32
+  DevBuf counts; // L+2 int32
+  DevBuf flag_ids, flag_count, flag_mark;
+  DevBuf work;   // MFMA work-queue counter
+  DevBuf in16;   // recompute input
+  DevBuf ret_ids, ret_count;  // rows retired as saturated (guard passes)
+  uint8_t* f(int side, int kind) const { return flags.as<uint8_t>() + static_cast<int64_t>(side * 3 + kind) * ld; }
+  int32_t* id(int side) const { return ids.as<int32_t>() + static_cast<int64_t>(side) * ld; }
+};
+
+int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, hipStream_t st);
+
+}  // namespace cospdnn_detail
+
+using cospdnn_detail::DevBuf;
+using cospdnn_detail::LayerStore;
+using cospdnn_detail::RunBufs;
+
+struct cospdnn_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int64_t hbm_bytes = 0;
+  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
+      bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
+  int64_t plan_version = -1;
+  float band = 0.0f;
+  ~GraphCache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+struct cospdnn_net {
+  cospdnn_engine* eng = nullptr;
+  int N = 0, L = 0;
+  int64_t S = 0;
+  float bias = 0.0f;
+  // host-side definition
+  std::vector<cospdnn::CscLayer> layers;
+  std::vector<char> layer_set;
+  bool gen_layers = false;
+  int gen_kind = 0, gen_K = 32;
+  float gen_wval = 0.0625f;
+  uint64_t gen_seed = 0;
+  int input_mode = 0;  // 0 none, 1 host image, 2 generated
+  std::vector<_Float16> input_img;  // [N][ld_store]
+  bool input_fp16_exact = true;
+  bool input_binary = true;  // every input value is 0 or 1
+  uint64_t input_seed = 0;
+  // device side
+  bool uploaded = false;
+  std::vector<std::unique_ptr<LayerStore>> dl;
+  DevBuf store;
+  int64_t ld_store = 0;
+  DevBuf packed;       // binary inputs: [S][words] bit-packed sample-major copy of the store
+  int words = 0;
+  RunBufs main, rec;
+  int64_t nnz_total = 0;
+  bool weights_fp16_exact = true;
+  GraphCache graph;
+  std::vector<cospdnn_layer_plan> plan;  // per-layer launch plan of the FP16 pass (empty: options decide)
+  int64_t plan_version = 0;
+  RunBufs rep;                       // FP64 pass over the rows the FP32 pass leaves undecided
+  std::vector<hipEvent_t> layer_ev;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  ~cospdnn_net() {
+    for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace cospdnn_detail {
+
+// One network's forward with the guard tiers (engine.cpp).
+int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res);
+
+}  // namespace cospdnn_detail

@@ -143,6 +143,56 @@ int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const 
                             int64_t n_fuse, int blocksize_c, int blocksize_r, int64_t cap, int32_t* colptr,
                             int32_t* rows, float* vals);
 
+/* ---- per-layer launch plans and the autotuner ----------------------------------
+   No counterpart in the reference, which hard-codes its choices (16x16 blocks,
+   cospdnn.cpp:94-95; compaction up to layer 21, cospdnn.cu:240; evaluateConfig's
+   fixed split, cospdnn_double.cu:3-30).  A plan fixes, for each layer of the FP16
+   pass, the SpMM kernel and its launch shape. */
+enum { COSPDNN_PLAN_AUTO = 0, COSPDNN_PLAN_MFMA = 1, COSPDNN_PLAN_SIMT = 2 };
+typedef struct {
+  int kernel;        /* COSPDNN_PLAN_*; MFMA falls back to SIMT on a layer without a tile image */
+  int order;         /* MFMA work order (as cospdnn_opts.spmm_order) */
+  int groups_per_cu; /* MFMA grid (as cospdnn_opts.mfma_groups_per_cu) */
+} cospdnn_layer_plan;
+/* L entries (copied), or NULL to go back to the options' choice for every layer */
+int cospdnn_net_set_plan(cospdnn_net* net, const cospdnn_layer_plan* plan);
+/* the candidate launch shapes the tuner compares for each layer */
+int cospdnn_num_candidates(void);
+int cospdnn_candidate(int k, cospdnn_layer_plan* out);
+/* feature vector of (layer, candidate, rows entering the layer) for the cost model */
+#define COSPDNN_TUNE_FEATURES 12
+int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, const cospdnn_layer_plan* cand,
+                               float* feat);
+/* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer
+   (median of reps runs, per-layer HIP events) and keeps, per layer, the fastest.
+   ms_table [ncand][L] (may be NULL), live [L+1] rows entering each layer (may be NULL),
+   plan [L].  The network's plan is left unchanged. */
+int cospdnn_tune_exhaustive(cospdnn_net* net, const cospdnn_opts* opts, int reps, float* ms_table, int64_t* live,
+                            cospdnn_layer_plan* plan);
+
+/* Cost-model MLP: n_in -> hidden -> hidden -> 1, ReLU, FP32, on the engine's device.
+   Training (Adam on mean squared error), gradients and inference run in HIP kernels. */
+typedef struct cospdnn_mlp cospdnn_mlp;
+int cospdnn_mlp_create(cospdnn_engine* engine, int n_in, int hidden, uint64_t seed, cospdnn_mlp** out);
+void cospdnn_mlp_destroy(cospdnn_mlp* mlp);
+int cospdnn_mlp_num_params(const cospdnn_mlp* mlp);
+/* parameters: W1[n_in][hidden], b1, W2[hidden][hidden], b2, W3[hidden], b3 (row-major) */
+int cospdnn_mlp_get_params(cospdnn_mlp* mlp, float* params);
+int cospdnn_mlp_set_params(cospdnn_mlp* mlp, const float* params);
+/* loss = mean((MLP(X) - y)^2) and its gradient w.r.t. the parameters; X [n][n_in] */
+int cospdnn_mlp_loss_grad(cospdnn_mlp* mlp, const float* X, const float* y, int n, float* loss, float* grad);
+/* steps of full-batch Adam; *loss = final training loss (may be NULL) */
+int cospdnn_mlp_train(cospdnn_mlp* mlp, const float* X, const float* y, int n, int steps, float lr, float* loss);
+int cospdnn_mlp_predict(cospdnn_mlp* mlp, const float* X, int n, float* out);
+/* Training samples from an exhaustive table: one row per (candidate, layer),
+   y = ln(1000 * ms + 1).  cap rows of X [cap][COSPDNN_TUNE_FEATURES] and y [cap]. */
+int cospdnn_tune_samples(cospdnn_net* net, const float* ms_table, const int64_t* live, float* X, float* y, int cap,
+                         int* n_out);
+/* Plan from the cost model: per layer the candidate with the smallest predicted time.
+   pred_ms [ncand][L] (may be NULL) receives the predictions. */
+int cospdnn_tune_mlp(cospdnn_net* net, cospdnn_mlp* mlp, const int64_t* live, cospdnn_layer_plan* plan,
+                     float* pred_ms);
+
 #ifdef __cplusplus
 }
 #endif

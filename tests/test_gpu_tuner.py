@@ -1,0 +1,149 @@
+"""Autotuner on the GPU: the cost-model MLP's HIP kernels against the float64 numpy
+reference (forward, gradient), training, plans applied to the engine (categories
+unchanged), and the regret of the MLP plan against exhaustive tuning on a network it
+was not trained on.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cospdnn_amd as cd
+import mlp_ref
+from oracle import oracle_ctypes as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-tuner] {msg}", file=sys.stderr, flush=True)
+
+
+def test_mlp_forward_and_gradient_match_reference(engine):
+    n_in, H, n = cd.TUNE_FEATURES, 64, 333
+    rng = np.random.default_rng(7)
+    m = cd.Mlp(engine, n_in, H, seed=5)
+    p = m.get_params()
+    assert p.size == mlp_ref.num_params(n_in, H)
+    p = rng.normal(0, 0.3, p.size).astype(np.float32)
+    m.set_params(p)
+    X = rng.normal(0, 1, (n, n_in)).astype(np.float32)
+    y = rng.normal(0, 1, n).astype(np.float32)
+    pred_ref, _ = mlp_ref.forward(p, X, n_in, H)
+    np.testing.assert_allclose(m.predict(X), pred_ref, rtol=1e-4, atol=1e-4)
+    loss, g = m.loss_grad(X, y)
+    loss_ref, g_ref = mlp_ref.loss_grad(p, X, y, n_in, H)
+    assert abs(loss - loss_ref) <= 1e-4 * max(1.0, loss_ref)
+    # FP32 kernels vs FP64: relative to the gradient's scale
+    scale = np.abs(g_ref).max()
+    assert np.abs(g - g_ref).max() <= 2e-4 * scale
+
+
+def test_mlp_training_fits_a_smooth_function(engine):
+    rng = np.random.default_rng(11)
+    n_in = 4
+    X = rng.uniform(-1, 1, (2000, n_in)).astype(np.float32)
+    y = (np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2] - 0.5 * X[:, 3] ** 2).astype(np.float32)
+    m = cd.Mlp(engine, n_in, 64, seed=3)
+    loss0, _ = m.loss_grad(X, y)
+    loss = m.train(X, y, steps=1500, lr=3e-3)
+    say(f"mlp fit: loss {loss0:.4f} -> {loss:.5f}")
+    assert loss < 0.05 * loss0
+    # the reported loss is the reference loss of the trained parameters
+    loss_ref, _ = mlp_ref.loss_grad(m.get_params(), X, y, n_in, 64)
+    assert abs(loss - loss_ref) <= 1e-3 * max(1.0, loss_ref) + 1e-5
+
+
+def make_net(engine, N, L, S, bias, kind, seed, waste=64.0):
+    net = cd.Net(engine, N, L, S, bias)
+    net.generate_layers(kind, seed)
+    net.generate_input(seed)
+    net.upload(cd.default_opts(max_tile_waste=waste))
+    return net
+
+
+def default_plan(net, max_waste=4.0):
+    """The engine's choice without a tuner: the MFMA kernel (block-column order, 4 groups
+    per CU) on layers whose tile waste is at most max_waste (the upload default), else SIMT."""
+    plan = []
+    for l in range(net.L):
+        f = net.layer_features(l, net.S, (cd.PLAN_SIMT, 0, 0))
+        waste = 2.0 ** (6.0 * float(f[4])) if f[3] > 0 else float("inf")
+        plan.append((cd.PLAN_MFMA, 1, 4) if waste <= max_waste * (1 + 1e-6) else (cd.PLAN_SIMT, 0, 0))
+    return plan
+
+
+def plan_spmm_ms(net, plan, reps=3):
+    net.set_plan(plan)
+    o = cd.default_opts(precision=cd.FP16, layer_timing=1)
+    cd.run([net], o)
+    t = np.median([cd.run([net], o)[0].ms_spmm for _ in range(reps)])
+    net.set_plan(None)
+    return float(t)
+
+
+def test_plans_keep_categories_exact(engine):
+    N, L, S, bias, seed = 1024, 16, 3000, -0.3, 21
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.RANDOM, seed), 0, S)
+    net = make_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed)
+    cands = cd.candidates()
+    # every candidate on every layer, and a mixed plan
+    mixed = [cands[l % len(cands)] for l in range(L)]
+    for plan in [[c] * L for c in cands] + [mixed]:
+        net.set_plan(plan)
+        out = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
+        want_mfma = sum(1 for p in plan if p[0] == cd.PLAN_MFMA)
+        assert out.mfma_layers == want_mfma
+        assert np.array_equal(out.categories, np.flatnonzero(alive)), plan[0]
+    with pytest.raises(cd.CospdnnError):
+        net.set_plan([(cd.PLAN_MFMA, 2, 4)] * L)  # bad order
+    net.set_plan(None)
+    net.close()
+
+
+def test_exhaustive_plan_is_the_per_layer_minimum(engine):
+    net = make_net(engine, 2048, 12, 6000, -0.35, cd.GEN_RANDOM, 22)
+    plan, table, live = cd.tune_exhaustive(net, reps=2)
+    cands = cd.candidates()
+    assert table.shape == (len(cands), 12) and live[0] == 6000
+    for l, p in enumerate(plan):
+        k = cands.index(p)
+        assert table[k, l] <= table[:, l].min() + 1e-9
+    X, y = cd.tune_samples(net, table, live)
+    assert X.shape == (len(cands) * 12, cd.TUNE_FEATURES)
+    np.testing.assert_allclose(y, np.log(1000.0 * table.reshape(-1) + 1.0), rtol=1e-5)
+    net.close()
+
+
+def test_mlp_plan_regret_on_an_unseen_network(engine):
+    """Train the cost model on timings of some networks, plan a network it never saw, and
+    compare with exhaustive tuning (the judge) and with the default choice (no tuner)."""
+    train_nets = [(1024, 24, 12000, -0.3, cd.GEN_RANDOM, 31), (4096, 24, 12000, -0.35, cd.GEN_RANDOM, 32),
+                  (4096, 24, 12000, -0.35, cd.GEN_BANDED, 33), (2048, 24, 20000, -0.3, cd.GEN_RANDOM, 34),
+                  (8192, 16, 8000, -0.4, cd.GEN_BANDED, 35)]
+    Xs, ys = [], []
+    for spec in train_nets:
+        net = make_net(engine, *spec)
+        _, table, live = cd.tune_exhaustive(net, reps=2)
+        X, y = cd.tune_samples(net, table, live)
+        Xs.append(X)
+        ys.append(y)
+        net.close()
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    m = cd.Mlp(engine, cd.TUNE_FEATURES, 64, seed=9)
+    loss = m.train(X, y, steps=3000, lr=2e-3)
+    say(f"cost model: {len(y)} samples, training loss {loss:.4f} (ln-time units)")
+    assert loss < 0.05
+
+    net = make_net(engine, 3072, 24, 16000, -0.33, cd.GEN_RANDOM, 36)
+    ex_plan, table, live = cd.tune_exhaustive(net, reps=2)
+    mlp_plan, pred = m.plan(net, live)
+    t_ex = plan_spmm_ms(net, ex_plan)
+    t_mlp = plan_spmm_ms(net, mlp_plan)
+    t_def = plan_spmm_ms(net, default_plan(net))
+    say(f"unseen net: exhaustive {t_ex:.2f} ms, MLP {t_mlp:.2f} ms, default {t_def:.2f} ms; "
+        f"regret {t_mlp / t_ex - 1:.3f}, gain over default {t_def / t_mlp:.2f}x")
+    assert t_mlp <= 1.15 * t_ex
+    assert t_mlp < t_def
+    net.close()
