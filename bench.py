@@ -1,11 +1,13 @@
 """Benchmark of the sparse-DNN inference path on one MI355X.
 
 A step is one full forward of the network (all L layers, compaction and the
-FP32 guard recompute) over the synthetic input already resident in HBM.
+FP32 / FP64 guard recompute) over the synthetic input already resident in HBM.
+Before the timed steps the per-layer launch plan is autotuned (exhaustive search,
+then the cost-model MLP trained on those timings picks the plan that is timed).
 Default workload: N=16384, L=120, 60000 samples, banded connectivity, bias -0.4
 (the configuration the FP16-MFMA target is stated on).  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--cpu-samples 1500]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--cpu-samples 1500] [--tune 1] [--graph 1]
 """
 import argparse
 import json
@@ -82,6 +84,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the FP16 layer loop from a hipGraph")
+    ap.add_argument("--tune", type=int, default=1, help="1: autotune the per-layer launch plan before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -107,6 +110,32 @@ def main():
     net.generate_input(spec.seed)
     net.upload(cd.default_opts())
     log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
+
+    # Autotuning (untimed): exhaustive per-layer search over the launch candidates, then
+    # the cost-model MLP (12-64-64-1, trained by the library's HIP kernels on this run's
+    # per-layer timings) picks the plan the timed runs use.
+    tuning = None
+    if args.tune:
+        t0 = time.perf_counter()
+        ex_plan, table, live = cd.tune_exhaustive(net, reps=2)
+        X, y = cd.tune_samples(net, table, live)
+        mlp = cd.Mlp(eng, cd.TUNE_FEATURES, 64, seed=1)
+        loss = mlp.train(X, y, steps=3000, lr=2e-3)
+        mlp_plan, _ = mlp.plan(net, live)
+        cands = cd.candidates()
+        pick = lambda plan: float(sum(table[cands.index(p), l] for l, p in enumerate(plan)))  # noqa: E731
+        default = [cands.index((cd.PLAN_MFMA, 1, 4))] * spec.L
+        tuning = {
+            "candidates": [list(c) for c in cands],
+            "training_samples": int(len(y)), "training_loss_ln_time": loss,
+            "spmm_ms_exhaustive": pick(ex_plan), "spmm_ms_mlp": pick(mlp_plan),
+            "spmm_ms_default": float(sum(table[k, l] for l, k in enumerate(default))),
+            "tuning_s": time.perf_counter() - t0,
+        }
+        tuning["mlp_regret"] = tuning["spmm_ms_mlp"] / tuning["spmm_ms_exhaustive"] - 1.0
+        net.set_plan(mlp_plan)
+        log(f"[bench] tuning: exhaustive {tuning['spmm_ms_exhaustive']:.2f} ms, MLP plan {tuning['spmm_ms_mlp']:.2f} ms "
+            f"(regret {tuning['mlp_regret']:.3f}), default {tuning['spmm_ms_default']:.2f} ms, {tuning['tuning_s']:.1f} s")
 
     # FP16-MFMA path with per-layer HIP events on the engine's stream: times the SpMM kernel
     # of every layer inside the timed steps (roofline of the dominant kernel).
@@ -191,6 +220,7 @@ def main():
         "fp32_gpu": fp32,
         "fp16_over_fp32": (value / fp32["value"]) if fp32 else None,
         "fp16_launch": "hipGraph replay" if args.graph else "eager",
+        "tuning": tuning,
         "fp16_with_layer_events_ms_per_step": ms_timed,
         "device_ms_per_step": float(np.median([o.ms_total for o in outs_g])),
         "categories": int(len(cats16)),
