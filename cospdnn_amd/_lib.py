@@ -40,6 +40,8 @@ class Opts(ctypes.Structure):
         ("dump_layer", ctypes.c_int),
         ("spmm_order", ctypes.c_int),
         ("mfma_groups_per_cu", ctypes.c_int),
+        ("batch_concurrent", ctypes.c_int),
+        ("guard_tail", ctypes.c_int),
     ]
 
 

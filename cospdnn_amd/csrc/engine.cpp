@@ -107,10 +107,8 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
 
 namespace {
 
-int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) {
-  BlockedLayer bl;
-  if (!build_layer(csc, net->N, max_waste, bl))
-    return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
+// Device images of one layer from its host images (built by build_layer).
+int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stable) {
   auto ls = std::make_unique<LayerStore>();
   CALL_TRY(dupload(ls->out_idx, bl.out_idx));
   CALL_TRY(dupload(ls->optr, bl.optr));
@@ -125,13 +123,16 @@ int upload_layer(cospdnn_net* net, int l, const CscLayer& csc, float max_waste) 
   ls->waste = bl.tile_waste;
   ls->fp16_exact = bl.fp16_exact;
   ls->nnz = bl.nnz;
-  ls->sat_stable = all32_fixed_point(csc, net->N, net->bias);
+  ls->sat_stable = sat_stable;
   if (bl.has_tiles) {
     CALL_TRY(dupload(ls->kptr, bl.kptr));
     CALL_TRY(dupload(ls->gidx, bl.gidx));
     CALL_TRY(dupload(ls->wt, bl.wt));
-    CALL_TRY(dupload(ls->wt32, bl.wt32));
-    ls->view.wt32 = ls->wt32.as<float>();
+    if (bl.has_tiles32) {
+      CALL_TRY(dupload(ls->wt32, bl.wt32));
+      ls->view.wt32 = ls->wt32.as<float>();
+      ls->has_tiles32 = true;
+    }
     ls->view.kptr = ls->kptr.as<int32_t>();
     ls->view.gidx = ls->gidx.as<int32_t>();
     ls->view.wt = ls->wt.as<_Float16>();
@@ -160,6 +161,7 @@ struct RunCfg {
   // Early stop (passes that are not captured in a graph): after the scan of layer check_from,
   // and every check_every layers after it, read the kept-row count and stop once it is 0.
   int check_from = -1, check_every = 16;
+  int tail = 0;  // guarded passes: rows that die in the last `tail` layers are flagged too
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -179,9 +181,8 @@ int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
 // Enqueues one layer-major pass (no synchronisation). *cat_side receives the ids side
 // holding the categories after the final scan.
 int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_t ld_in, int64_t S_run,
-                    const RunCfg& cfg, bool timing, int* cat_side) {
+                    const RunCfg& cfg, bool timing, hipStream_t st, int* cat_side) {
   cospdnn_engine* e = net->eng;
-  hipStream_t st = e->stream;
   const int L = net->L;
   HIP_TRY(launch_fill_u8(rb.f(0, 0), 1, S_run, st));
   HIP_TRY(launch_fill_u8(rb.f(0, 1), 1, S_run, st));
@@ -222,6 +223,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.ret_ids = rb.ret_ids.as<int32_t>();
     sa.ret_count = rb.ret_count.as<int32_t>();
     sa.drop_flagged = cfg.drop_flagged;
+    // at the checkpoint, all-32 decides a row only if it is a fixed point of the rest
+    sa.flag_all_alive = (l == D && !sat_from[l]) ? 1 : 0;
+    // scan l sees the deaths of layer l-1
+    sa.flag_dead = (D >= 0 && cfg.tail > 0 && l >= 1 && l - 1 >= L - cfg.tail) ? 1 : 0;
     HIP_TRY(launch_scan(sa, S_run, st));
     if (l == L) break;
     if (cfg.check_from >= 0 && l >= cfg.check_from && (l - cfg.check_from) % std::max(1, cfg.check_every) == 0) {
@@ -282,10 +287,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F64, grid_rows(e, e->bpc_rows, S_run, nblk), st));
       else
         HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk), st));
-    } else if (few && !(ls.has_tiles && cfg.mfma32)) {
+    } else if (few && !(ls.has_tiles32 && cfg.mfma32)) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
       HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F32, grid_rows(e, e->bpc_rows, S_run, nblk), st));
-    } else if (ls.has_tiles && cfg.mfma32) {
+    } else if (ls.has_tiles32 && cfg.mfma32) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
       HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk), st));
     } else if (l == 0) {
@@ -361,7 +366,7 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   cfg.check_from = 16;
   cfg.check_every = 16;
   int side = 0;
-  CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, &side));
+  CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, st, &side));
   HIP_TRY(hipStreamSynchronize(st));
   PassOut p;
   CALL_TRY(read_pass(net, rb, side, p));
@@ -369,38 +374,42 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   return COSPDNN_OK;
 }
 
-}  // namespace
+constexpr int kMaxSideStreams = 8;
 
-namespace cospdnn_detail {
+int ensure_side_streams(cospdnn_engine* e, int n) {
+  const int want = std::min(std::max(n, 1), kMaxSideStreams);
+  while (static_cast<int>(e->side.size()) < want) {
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+    e->side.push_back(s);
+  }
+  return COSPDNN_OK;
+}
 
-int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
+// Checks the options against the network, creates its events and main-pass buffers, and
+// fills cfg.  Nothing is enqueued.
+int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cospdnn_engine* e = net->eng;
-  hipStream_t st = e->stream;
   const int N = net->N, L = net->L;
-  const int64_t S = net->S;
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
   if (o.spmm_order != 0 && o.spmm_order != 1) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0 or 1");
   if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
     return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
+  if (o.guard_tail < 0) return fail(COSPDNN_ERR_ARG, "guard_tail must be >= 0");
   if (o.precision == COSPDNN_FP16 && (!net->weights_fp16_exact || !net->input_fp16_exact))
     return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
   if (o.precision != COSPDNN_FP16 && !net->input_fp16_exact)
     return fail(COSPDNN_ERR_PRECISION, "inputs must be exactly representable in FP16 (binary inputs are)");
-  const bool timing = o.layer_timing != 0;
-  if (timing && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
+  if (o.layer_timing != 0 && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
     for (hipEvent_t ev : net->layer_ev) (void)hipEventDestroy(ev);
     net->layer_ev.assign(2 * static_cast<size_t>(L), nullptr);
     for (auto& ev : net->layer_ev) HIP_TRY(hipEventCreate(&ev));
   }
   for (auto& ev : net->ev)
     if (!ev) HIP_TRY(hipEventCreate(&ev));
-
-  std::vector<char> alive(S, 0), first(S, 0);
-  std::vector<int64_t> live(L + 1, -1);
-  int64_t n_flagged = 0, n_replayed = 0;
-  RunCfg cfg;
+  cfg = RunCfg();
   cfg.precision = o.precision;
   cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : (o.precision == COSPDNN_FP32 ? o.guard_layer32 : -1);
   cfg.band = o.precision == COSPDNN_FP16 ? o.guard_band : 0.0f;
@@ -409,103 +418,117 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   cfg.mfma32 = o.fp32_mfma;
   cfg.order = o.spmm_order;
   cfg.groups_per_cu = o.mfma_groups_per_cu;
+  cfg.tail = o.guard_tail;
+  CALL_TRY(ensure_bufs(net->main, net->S, L, N, cfg.precision, false, e->stream));
+  return COSPDNN_OK;
+}
 
-  HIP_TRY(hipEventRecord(net->ev[0], st));
-  {
-    CALL_TRY(ensure_bufs(net->main, S, L, N, cfg.precision, false, st));
-    int cat_side = (L & 1) ^ 1;
-    const bool graph = o.use_graph != 0 && !timing;
-    if (graph) {
-      GraphCache& gc = net->graph;
-      const bool hit = gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer &&
-                       gc.band == cfg.band && gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma &&
-                       gc.mfma32 == cfg.mfma32 && gc.order == cfg.order && gc.groups == cfg.groups_per_cu &&
-                       gc.plan_version == net->plan_version;
-      if (!hit) {
-        if (gc.exec) {
-          (void)hipGraphExecDestroy(gc.exec);
-          gc.exec = nullptr;
-        }
-        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int side = 0;
-        int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, cfg, false, &side);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(st, &g);
-        if (rc != COSPDNN_OK) {
-          if (g) (void)hipGraphDestroy(g);
-          return rc;
-        }
-        HIP_TRY(ce);
-        hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_TRY(ie);
-        gc.precision = cfg.precision;
-        gc.guard_layer = cfg.guard_layer;
-        gc.band = cfg.band;
-        gc.band_from = cfg.band_from;
-        gc.use_mfma = cfg.use_mfma;
-        gc.mfma32 = cfg.mfma32;
-        gc.order = cfg.order;
-        gc.groups = cfg.groups_per_cu;
-        gc.plan_version = net->plan_version;
-      }
-      HIP_TRY(hipGraphLaunch(gc.exec, st));
-    } else {
-      CALL_TRY(enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, cfg, timing, &cat_side));
-    }
-    HIP_TRY(hipEventRecord(net->ev[1], st));
-    HIP_TRY(hipStreamSynchronize(st));
-    PassOut p1;
-    CALL_TRY(read_pass(net, net->main, cat_side, p1));
-    for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
-    for (int32_t id : p1.cats) alive[id] = 1;
-    first = alive;
+bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version) {
+  return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
+         gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
+         gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version;
+}
 
-    std::vector<int32_t> replay_ids;
-    if (cfg.precision == COSPDNN_FP16 && !p1.flags.empty()) {
-      // tier 2: the flagged rows again, layer-major in FP32 (MFMA FP32 on tiled layers)
-      const int32_t nflag = static_cast<int32_t>(p1.flags.size());
-      n_flagged = nflag;
-      CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
-      HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
-      CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), net->main.flag_count.as<int32_t>(), nflag,
-                                 net->rec.in16.as<_Float16>(), net->rec.ld));
-      RunCfg rcfg;
-      rcfg.precision = COSPDNN_FP32;
-      rcfg.guard_layer = o.guard_layer32;
-      rcfg.band = 0.0f;
-      rcfg.band_from = 1 << 30;
-      rcfg.use_mfma = 0;
-      rcfg.mfma32 = 1;
-      rcfg.retire_sat = 1;
-      rcfg.drop_flagged = 1;  // the rows handed to the FP64 tier are not carried further here
-      rcfg.check_from = o.guard_layer32 >= 0 ? std::min(o.guard_layer32, L) : 16;
-      rcfg.check_every = 16;
-      int rside = 0;
-      CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, &rside));
-      HIP_TRY(hipStreamSynchronize(st));
-      PassOut p2;
-      CALL_TRY(read_pass(net, net->rec, rside, p2));
-      for (int32_t id : p1.flags) alive[id] = 0;
-      for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
-      for (int32_t local : p2.flags) replay_ids.push_back(p1.flags[local]);
-    } else if (cfg.precision == COSPDNN_FP32) {
-      replay_ids = p1.flags;
+void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version) {
+  gc.precision = cfg.precision;
+  gc.guard_layer = cfg.guard_layer;
+  gc.band = cfg.band;
+  gc.band_from = cfg.band_from;
+  gc.use_mfma = cfg.use_mfma;
+  gc.mfma32 = cfg.mfma32;
+  gc.order = cfg.order;
+  gc.groups = cfg.groups_per_cu;
+  gc.plan_version = plan_version;
+}
+
+// The main pass of one network on stream st: eagerly, or (use_graph, no per-layer events)
+// replayed from the network's cached hipGraph.
+int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side) {
+  const bool timing = o.layer_timing != 0;
+  *cat_side = (net->L & 1) ^ 1;
+  if (o.use_graph == 0 || timing)
+    return enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, st, cat_side);
+  GraphCache& gc = net->graph;
+  if (!graph_matches(gc, cfg, net->plan_version)) {
+    if (gc.exec) {
+      (void)hipGraphExecDestroy(gc.exec);
+      gc.exec = nullptr;
     }
-    if (!replay_ids.empty()) {
-      // tier 3: FP64 replay, bit-identical to the FP64 oracle
-      std::sort(replay_ids.begin(), replay_ids.end());
-      std::vector<uint8_t> a64;
-      CALL_TRY(fp64_rows(net, replay_ids, a64));
-      for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
-      n_replayed = static_cast<int64_t>(replay_ids.size());
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int side = 0;
+    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, false, st, &side);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    if (rc != COSPDNN_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
     }
+    HIP_TRY(ce);
+    hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(ie);
+    graph_remember(gc, cfg, net->plan_version);
   }
-  HIP_TRY(hipEventRecord(net->ev[2], st));
-  HIP_TRY(hipStreamSynchronize(st));
-  float ms_first = 0.0f, ms_all = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms_first, net->ev[0], net->ev[1]));
-  HIP_TRY(hipEventElapsedTime(&ms_all, net->ev[0], net->ev[2]));
+  HIP_TRY(hipGraphLaunch(gc.exec, st));
+  return COSPDNN_OK;
+}
+
+// After the main pass has finished (stream synchronised): categories, the FP32 and FP64
+// guard tiers on the engine's stream, and the result fields except the times.
+int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int cat_side, cospdnn_result* res) {
+  cospdnn_engine* e = net->eng;
+  hipStream_t st = e->stream;
+  const int N = net->N, L = net->L;
+  const int64_t S = net->S;
+  std::vector<char> alive(S, 0), first(S, 0);
+  std::vector<int64_t> live(L + 1, -1);
+  int64_t n_flagged = 0, n_replayed = 0;
+  PassOut p1;
+  CALL_TRY(read_pass(net, net->main, cat_side, p1));
+  for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
+  for (int32_t id : p1.cats) alive[id] = 1;
+  first = alive;
+
+  std::vector<int32_t> replay_ids;
+  if (cfg.precision == COSPDNN_FP16 && !p1.flags.empty()) {
+    // tier 2: the flagged rows again, layer-major in FP32 (MFMA FP32 on tiled layers)
+    const int32_t nflag = static_cast<int32_t>(p1.flags.size());
+    n_flagged = nflag;
+    CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
+    HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
+    CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), net->main.flag_count.as<int32_t>(), nflag,
+                               net->rec.in16.as<_Float16>(), net->rec.ld));
+    RunCfg rcfg;
+    rcfg.precision = COSPDNN_FP32;
+    rcfg.guard_layer = o.guard_layer32;
+    rcfg.band = 0.0f;
+    rcfg.band_from = 1 << 30;
+    rcfg.use_mfma = 0;
+    rcfg.mfma32 = 1;
+    rcfg.retire_sat = 1;
+    rcfg.drop_flagged = 1;  // the rows handed to the FP64 tier are not carried further here
+    rcfg.check_from = o.guard_layer32 >= 0 ? std::min(o.guard_layer32, L) : 16;
+    rcfg.check_every = 16;
+    rcfg.tail = o.guard_tail;
+    int rside = 0;
+    CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, st, &rside));
+    HIP_TRY(hipStreamSynchronize(st));
+    PassOut p2;
+    CALL_TRY(read_pass(net, net->rec, rside, p2));
+    for (int32_t id : p1.flags) alive[id] = 0;
+    for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
+    for (int32_t local : p2.flags) replay_ids.push_back(p1.flags[local]);
+  } else if (cfg.precision == COSPDNN_FP32) {
+    replay_ids = p1.flags;
+  }
+  if (!replay_ids.empty()) {
+    // tier 3: FP64 replay, bit-identical to the FP64 oracle
+    std::sort(replay_ids.begin(), replay_ids.end());
+    std::vector<uint8_t> a64;
+    CALL_TRY(fp64_rows(net, replay_ids, a64));
+    for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
+    n_replayed = static_cast<int64_t>(replay_ids.size());
+  }
 
   int64_t ncat = 0, changed = 0;
   for (int64_t s = 0; s < S; ++s) {
@@ -519,13 +542,10 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   res->n_categories = ncat;
   if (res->live)
     for (int l = 0; l <= L; ++l) res->live[l] = live[l];
-  res->ms_total = ms_all;
-  res->ms_recompute = ms_all - ms_first;
   res->n_flagged = n_flagged;
   res->n_replayed = n_replayed;
   res->n_changed = changed;
   res->edges = static_cast<double>(S) * static_cast<double>(net->nnz_total);
-  res->edges_per_sec = ms_all > 0 ? res->edges / (ms_all * 1e-3) : 0.0;
   const double ab = o.precision == COSPDNN_FP16 ? 2.0 : (o.precision == COSPDNN_FP32 ? 4.0 : 8.0);
   double bytes = 0.0;
   int64_t mfma_layers = 0;
@@ -540,7 +560,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   res->alg_bytes = bytes;
   res->mfma_layers = mfma_layers;
   res->ms_spmm = 0.0;
-  if (timing) {
+  if (o.layer_timing != 0) {
     double sum = 0.0;
     for (int l = 0; l < L; ++l) {
       float ms = 0.0f;
@@ -560,7 +580,7 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
     int side = 0;
     RunCfg dcfg = cfg;
     dcfg.guard_layer = -1;
-    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, dcfg, false, &side);
+    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, dcfg, false, st, &side);
     net->L = saveL;
     CALL_TRY(rc);
     HIP_TRY(hipStreamSynchronize(st));
@@ -588,6 +608,110 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
     for (int64_t r = 0; r < nr; ++r) res->dump_ids[r] = ids[r];
     res->dump_rows = nr;
   }
+  return COSPDNN_OK;
+}
+
+void set_times(cospdnn_result* res, float ms_first, float ms_all) {
+  res->ms_total = ms_all;
+  res->ms_recompute = ms_all - ms_first;
+  res->edges_per_sec = ms_all > 0 ? res->edges / (ms_all * 1e-3) : 0.0;
+}
+
+}  // namespace
+
+namespace cospdnn_detail {
+
+int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
+  hipStream_t st = net->eng->stream;
+  RunCfg cfg;
+  CALL_TRY(run_prepare(net, o, cfg));
+  HIP_TRY(hipEventRecord(net->ev[0], st));
+  int cat_side = 0;
+  CALL_TRY(enqueue_main(net, o, cfg, st, &cat_side));
+  HIP_TRY(hipEventRecord(net->ev[1], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  CALL_TRY(run_finish(net, o, cfg, cat_side, res));
+  HIP_TRY(hipEventRecord(net->ev[2], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms_first = 0.0f, ms_all = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms_first, net->ev[0], net->ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms_all, net->ev[0], net->ev[2]));
+  set_times(res, ms_first, ms_all);
+  return COSPDNN_OK;
+}
+
+// Several networks at once: their main passes run concurrently, each on a stream of its
+// own forked from (and joined back to) the engine's stream, either eagerly or from one
+// hipGraph that holds all the branches; then each network's guard tiers.  Every result
+// reports the times of the whole batch.
+int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_result* res) {
+  cospdnn_engine* e = nets[0]->eng;
+  hipStream_t st = e->stream;
+  std::vector<RunCfg> cfg(n);
+  for (int i = 0; i < n; ++i) CALL_TRY(run_prepare(nets[i], o, cfg[i]));
+  CALL_TRY(ensure_side_streams(e, n));
+  for (auto* ev : {&e->batch_ev[0], &e->batch_ev[1], &e->batch_ev[2]})
+    if (!*ev) HIP_TRY(hipEventCreateWithFlags(ev, hipEventDefault));
+  if (static_cast<int>(e->join_ev.size()) < n) {
+    const size_t old = e->join_ev.size();
+    e->join_ev.resize(n, nullptr);
+    for (size_t i = old; i < e->join_ev.size(); ++i) HIP_TRY(hipEventCreateWithFlags(&e->join_ev[i], hipEventDisableTiming));
+  }
+  if (!e->fork_ev) HIP_TRY(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
+  std::vector<int> side(n, 0);
+  const bool graph = o.use_graph != 0 && o.layer_timing == 0;
+  auto enqueue_all = [&]() -> int {
+    HIP_TRY(hipEventRecord(e->fork_ev, st));
+    for (int i = 0; i < n; ++i) {
+      hipStream_t si = e->side[i % e->side.size()];
+      HIP_TRY(hipStreamWaitEvent(si, e->fork_ev, 0));
+      side[i] = (nets[i]->L & 1) ^ 1;
+      CALL_TRY(enqueue_forward(nets[i], nets[i]->main, nets[i]->store.as<_Float16>(), nets[i]->ld_store, nets[i]->S,
+                               cfg[i], o.layer_timing != 0, si, &side[i]));
+      HIP_TRY(hipEventRecord(e->join_ev[i], si));
+    }
+    for (int i = 0; i < n; ++i) HIP_TRY(hipStreamWaitEvent(st, e->join_ev[i], 0));
+    return COSPDNN_OK;
+  };
+  HIP_TRY(hipEventRecord(e->batch_ev[0], st));
+  if (graph) {
+    BatchGraph& bg = e->batch_graph;
+    bool hit = bg.exec != nullptr && bg.nets.size() == static_cast<size_t>(n);
+    for (int i = 0; hit && i < n; ++i)
+      hit = bg.nets[i] == nets[i] && bg.plan_versions[i] == nets[i]->plan_version &&
+            graph_matches(bg.key, cfg[i], 0);
+    if (!hit) {
+      bg.reset();
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_all();
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &g);
+      if (rc != COSPDNN_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIP_TRY(ce);
+      hipError_t ie = hipGraphInstantiate(&bg.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ie);
+      bg.nets.assign(nets, nets + n);
+      bg.plan_versions.resize(n);
+      for (int i = 0; i < n; ++i) bg.plan_versions[i] = nets[i]->plan_version;
+      graph_remember(bg.key, cfg[0], 0);
+    }
+    HIP_TRY(hipGraphLaunch(bg.exec, st));
+  } else {
+    CALL_TRY(enqueue_all());
+  }
+  HIP_TRY(hipEventRecord(e->batch_ev[1], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int i = 0; i < n; ++i) CALL_TRY(run_finish(nets[i], o, cfg[i], side[i], &res[i]));
+  HIP_TRY(hipEventRecord(e->batch_ev[2], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms_first = 0.0f, ms_all = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms_first, e->batch_ev[0], e->batch_ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms_all, e->batch_ev[0], e->batch_ev[2]));
+  for (int i = 0; i < n; ++i) set_times(&res[i], ms_first, ms_all);
   return COSPDNN_OK;
 }
 
@@ -632,6 +756,8 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->dump_layer = -1;
   o->spmm_order = 1;
   o->mfma_groups_per_cu = 4;
+  o->batch_concurrent = 1;
+  o->guard_tail = 8;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {
@@ -666,6 +792,12 @@ int cospdnn_engine_create(int device, cospdnn_engine** out) {
 void cospdnn_engine_destroy(cospdnn_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
+  e->batch_graph.reset();
+  for (hipStream_t s : e->side) (void)hipStreamDestroy(s);
+  for (hipEvent_t ev : e->join_ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : e->batch_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork_ev) (void)hipEventDestroy(e->fork_ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -699,6 +831,9 @@ int cospdnn_net_create(cospdnn_engine* e, int N, int L, int64_t S, float bias, c
 void cospdnn_net_destroy(cospdnn_net* net) {
   if (!net) return;
   (void)hipSetDevice(net->eng->device);
+  // a batch graph that captured this network's buffers must not be replayed again
+  BatchGraph& bg = net->eng->batch_graph;
+  if (std::find(bg.nets.begin(), bg.nets.end(), net) != bg.nets.end()) bg.reset();
   delete net;
 }
 
@@ -819,21 +954,33 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
   HIP_TRY(hipSetDevice(net->eng->device));
   net->nnz_total = 0;
   net->weights_fp16_exact = true;
-  if (net->gen_layers) {
-    // generate + build in batches of layers on host threads, upload in order
+  {
+    // generate (when synthetic) and build the host images of a batch of layers on host
+    // threads, one layer per thread, then upload them in order
     const int L = net->L;
     const int batch = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
     for (int l0 = 0; l0 < L; l0 += batch) {
       const int l1 = std::min(L, l0 + batch);
-      std::vector<CscLayer> tmp(l1 - l0);
+      std::vector<BlockedLayer> bl(l1 - l0);
+      std::vector<char> ok(l1 - l0, 0), stable(l1 - l0, 0);
+      std::vector<std::thread> th;
       for (int l = l0; l < l1; ++l)
-        generate_layer(net->gen_seed, net->gen_kind, net->N, net->gen_K, l, net->gen_wval, tmp[l - l0]);
-      for (int l = l0; l < l1; ++l) CALL_TRY(upload_layer(net, l, tmp[l - l0], o.max_tile_waste));
-    }
-  } else {
-    for (int l = 0; l < net->L; ++l) {
-      CALL_TRY(upload_layer(net, l, net->layers[l], o.max_tile_waste));
-      net->layers[l] = CscLayer();
+        th.emplace_back([&, l]() {
+          CscLayer gen;
+          const CscLayer* csc = &net->layers[l];
+          if (net->gen_layers) {
+            generate_layer(net->gen_seed, net->gen_kind, net->N, net->gen_K, l, net->gen_wval, gen, 1);
+            csc = &gen;
+          }
+          ok[l - l0] = build_layer(*csc, net->N, o.max_tile_waste, bl[l - l0]) ? 1 : 0;
+          stable[l - l0] = all32_fixed_point(*csc, net->N, net->bias) ? 1 : 0;
+        });
+      for (auto& t : th) t.join();
+      for (int l = l0; l < l1; ++l) {
+        if (!ok[l - l0]) return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
+        CALL_TRY(upload_layer(net, l, bl[l - l0], stable[l - l0] != 0));
+        if (!net->gen_layers) net->layers[l] = CscLayer();
+      }
     }
   }
   CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));
@@ -868,6 +1015,10 @@ int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, 
     if (nets[i]->eng != nets[0]->eng) return fail(COSPDNN_ERR_MISMATCH, "networks on different engines");
   }
   HIP_TRY(hipSetDevice(nets[0]->eng->device));
+  for (int i = 0; i < n_nets; ++i)
+    for (int k = 0; k < i; ++k)
+      if (nets[i] == nets[k]) return fail(COSPDNN_ERR_ARG, "a network appears twice in one call");
+  if (n_nets > 1 && o.batch_concurrent) return run_batch(nets, n_nets, o, results);
   for (int i = 0; i < n_nets; ++i) CALL_TRY(run_one(nets[i], o, &results[i]));
   return COSPDNN_OK;
 }

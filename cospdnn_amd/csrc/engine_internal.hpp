@@ -64,6 +64,7 @@ struct LayerStore {
   DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32;
   DevLayer view;
   bool has_tiles = false;
+  bool has_tiles32 = false;  // FP32 tile image present
   double waste = 0.0;
   bool fp16_exact = true;
   bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
@@ -94,15 +95,6 @@ using cospdnn_detail::DevBuf;
 using cospdnn_detail::LayerStore;
 using cospdnn_detail::RunBufs;
 
-struct cospdnn_engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int cu_count = 0;
-  int64_t hbm_bytes = 0;
-  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
-      bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
-};
-
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
@@ -111,6 +103,37 @@ struct GraphCache {
   ~GraphCache() {
     if (exec) (void)hipGraphExecDestroy(exec);
   }
+};
+
+// One hipGraph holding the concurrent main passes of several networks (fork/join over
+// the engine's side streams), valid for that list of networks and their plans.
+struct BatchGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<cospdnn_net*> nets;
+  std::vector<int64_t> plan_versions;
+  GraphCache key;  // options of the captured passes (its exec stays null)
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    nets.clear();
+    plan_versions.clear();
+  }
+  ~BatchGraph() { reset(); }
+};
+
+struct cospdnn_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int64_t hbm_bytes = 0;
+  int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
+      bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
+  // concurrent runs of several networks
+  std::vector<hipStream_t> side;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
+  hipEvent_t batch_ev[3] = {nullptr, nullptr, nullptr};
+  BatchGraph batch_graph;
 };
 
 struct cospdnn_net {

@@ -168,7 +168,10 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
   out.kptr.assign(out.nblk + 1, 0);
   out.gidx.assign(static_cast<size_t>(nchunk) * kChunkK, 0);
   out.wt.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0);
-  out.wt32.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0.0f);
+  // the FP32 image serves only the FP32 guard pass; at high waste that pass uses the
+  // SIMT image instead and the (2x larger) FP32 tiles are not built
+  out.has_tiles32 = out.tile_waste <= kMaxWasteFp32Tiles;
+  if (out.has_tiles32) out.wt32.assign(static_cast<size_t>(nchunk) * kChunkK * kBlockOut, 0.0f);
   std::vector<float> dense;  // [kpad][32]
   int64_t ch0 = 0;
   for (int c = 0; c < out.nblk; ++c) {
@@ -203,7 +206,7 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
     }
     // A operand of v_mfma_f32_16x16x4f32, per K step kk of 4 inputs:
     // lane l holds row i = l%16 and k = 4*kk + l/16.
-    for (int ch = 0; ch < nch; ++ch)
+    for (int ch = 0; ch < (out.has_tiles32 ? nch : 0); ++ch)
       for (int m = 0; m < 2; ++m)
         for (int kk = 0; kk < 8; ++kk)
           for (int lane = 0; lane < 64; ++lane) {

@@ -41,12 +41,16 @@ struct BlockedLayer {
   std::vector<int32_t> gidx;   // nchunk*32
   std::vector<uint16_t> wt;    // nchunk*1024 FP16 bit patterns (v_mfma_f32_16x16x32_f16 A order)
   std::vector<float> wt32;     // nchunk*1024 FP32 values (v_mfma_f32_16x16x4f32 A order)
+  bool has_tiles32 = false;    // wt32 built (tile_waste <= kMaxWasteFp32Tiles)
   int64_t nchunk = 0;
   double tile_waste = 0.0;     // MFMA MACs per useful MAC (nchunk*32*32 / nnz)
   bool fp16_exact = true;      // every weight survives FP16 round trip
   float colsum_min = 0.0f;     // min over columns of the column sum
   bool nonneg = true;          // every weight >= 0
 };
+
+// FP32 tile images are built only up to this waste (they serve the FP32 guard pass).
+constexpr double kMaxWasteFp32Tiles = 4.0;
 
 // Builds both images. Tiles are built only when tile_waste <= max_waste.
 // Returns false (and leaves `out` unspecified) if the CSC is malformed.

@@ -9,8 +9,8 @@ namespace cospdnn {
 namespace {
 
 template <typename F>
-void parallel_for(int64_t n, F&& body) {
-  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+void parallel_for(int64_t n, F&& body, int threads = 0) {
+  const int64_t hw = threads > 0 ? threads : std::max(1u, std::thread::hardware_concurrency());
   const int64_t nt = std::min<int64_t>(hw, std::max<int64_t>(1, n / 256));
   if (nt <= 1) {
     body(0, n);
@@ -39,7 +39,7 @@ void floyd_rows(uint64_t key, int N, int K, int32_t* rows) {
 
 }  // namespace
 
-void generate_layer(uint64_t seed, int kind, int N, int K, int l, float wval, CscLayer& out) {
+void generate_layer(uint64_t seed, int kind, int N, int K, int l, float wval, CscLayer& out, int threads) {
   const int64_t nnz = static_cast<int64_t>(N) * K;
   out.colptr.resize(N + 1);
   out.rows.resize(nnz);
@@ -61,13 +61,13 @@ void generate_layer(uint64_t seed, int kind, int N, int K, int l, float wval, Cs
         for (int k = 0; k < K; ++k) r[k] = static_cast<int32_t>((static_cast<int64_t>(perm[j]) + static_cast<int64_t>(k) * stride) % N);
         std::sort(r, r + K);
       }
-    });
+    }, threads);
   } else {
     parallel_for(N, [&](int64_t a, int64_t b) {
       for (int64_t j = a; j < b; ++j)
         floyd_rows(stream_key(seed, kTagWeights, static_cast<uint64_t>(l), static_cast<uint64_t>(j)), N, K,
                    out.rows.data() + j * K);
-    });
+    }, threads);
   }
 }
 

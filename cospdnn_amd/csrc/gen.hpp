@@ -53,8 +53,8 @@ struct CscLayer {
   std::vector<float> vals;      // nnz
 };
 
-// Fills `out` with layer l of a generated network (multi-threaded).
-void generate_layer(uint64_t seed, int kind, int N, int K, int l, float wval, CscLayer& out);
+// Fills `out` with layer l of a generated network on `threads` host threads (0: all).
+void generate_layer(uint64_t seed, int kind, int N, int K, int l, float wval, CscLayer& out, int threads = 0);
 // Rows [s0, s0+count) of the binary input, row-major [count][N] bytes.
 void generate_input_rows(uint64_t seed, int N, int64_t s0, int64_t count, uint8_t* out);
 

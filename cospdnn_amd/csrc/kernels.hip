@@ -96,16 +96,20 @@ __device__ __forceinline__ bool flag_at(const uint4& v, int i) {
 // Rows a scan keeps, four flag bytes at a time: alive, minus retired saturated rows
 // (retire_sat: alive & !unsat), minus flagged undecided rows (drop_flagged at the
 // checkpoint: alive & unsat).
+__device__ __forceinline__ uint32_t flagged_word(const ScanArgs& a, uint32_t al, uint32_t un) {
+  return a.flag_undecided ? (a.flag_all_alive ? al : (al & un)) & 0x01010101u : 0u;
+}
 __device__ __forceinline__ uint32_t kept_word(const ScanArgs& a, uint32_t al, uint32_t un) {
   uint32_t k = al;
   if (a.retire_sat) k &= un;
-  if (a.drop_flagged && a.flag_undecided) k &= ~(al & un);
+  if (a.drop_flagged) k &= ~flagged_word(a, al, un);
   return k & 0x01010101u;
 }
 __device__ __forceinline__ uint4 kept_flags(const ScanArgs& a, int pos) {
   const uint4 v = *reinterpret_cast<const uint4*>(a.alive + pos);
   uint4 u = make_uint4(0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u);
-  if (a.retire_sat || (a.drop_flagged && a.flag_undecided)) u = *reinterpret_cast<const uint4*>(a.unsat + pos);
+  if (a.retire_sat || (a.drop_flagged && a.flag_undecided && !a.flag_all_alive))
+    u = *reinterpret_cast<const uint4*>(a.unsat + pos);
   return make_uint4(kept_word(a, v.x, u.x), kept_word(a, v.y, u.y), kept_word(a, v.z, u.z), kept_word(a, v.w, u.w));
 }
 
@@ -136,7 +140,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
     if (a.retire_sat)
       vr = make_uint4(v0.x & ~vu.x & 0x01010101u, v0.y & ~vu.y & 0x01010101u, v0.z & ~vu.z & 0x01010101u,
                       v0.w & ~vu.w & 0x01010101u);
-    if (a.flag_undecided) vf = make_uint4(v0.x & vu.x, v0.y & vu.y, v0.z & vu.z, v0.w & vu.w);
+    vf = make_uint4(flagged_word(a, v0.x, vu.x), flagged_word(a, v0.y, vu.y), flagged_word(a, v0.z, vu.z),
+                    flagged_word(a, v0.w, vu.w));
+    if (a.flag_dead)
+      vf = make_uint4(vf.x | (~v0.x & 0x01010101u), vf.y | (~v0.y & 0x01010101u), vf.z | (~v0.z & 0x01010101u),
+                      vf.w | (~v0.w & 0x01010101u));
     va = make_uint4(kept_word(a, v0.x, vu.x), kept_word(a, v0.y, vu.y), kept_word(a, v0.z, vu.z), kept_word(a, v0.w, vu.w));
     if (pc + kScanPer <= n) {
       cnt = count_flags16(va);

@@ -75,6 +75,12 @@ struct ScanArgs {
   int32_t* ret_count = nullptr;
   // With flag_undecided: the flagged rows also leave the pass (a later pass decides them).
   int drop_flagged = 0;
+  // With flag_undecided: flag every live row, saturated ones too (all-32 is not a fixed
+  // point of every remaining layer, so saturation decides nothing yet).
+  int flag_all_alive = 0;
+  // Flag the rows that died in the previous layer (deaths close to the last layer are
+  // rechecked: a later pass at higher precision may keep such a row alive to the end).
+  int flag_dead = 0;
 };
 
 enum class Dtype : int { F16 = 0, F32 = 1, F64 = 2 };

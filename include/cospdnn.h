@@ -66,7 +66,12 @@ typedef struct {
   int use_graph;         /* 1: capture the layer loop of each network in a hipGraph and replay */
   int dump_layer;        /* >= 0: copy the activations after this layer into the result */
   int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major */
-  int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..4) */
+  int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
+  int batch_concurrent;  /* several networks in one cospdnn_run: 1 = their main passes run
+                            concurrently on side streams (one hipGraph with use_graph),
+                            0 = one after another */
+  int guard_tail;        /* guarded passes: rows that die in the last guard_tail layers are
+                            recomputed at the next precision too */
 } cospdnn_opts;
 
 typedef struct {

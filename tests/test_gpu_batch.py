@@ -1,0 +1,65 @@
+"""Several networks in one cospdnn_run: concurrent main passes on side streams, eagerly
+and from one hipGraph, against the sequential run and the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import cospdnn_amd as cd
+from oracle import oracle_ctypes as orc
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [(1024, 20, 3000, -0.3, cd.GEN_RANDOM, 41), (2048, 12, 5000, -0.35, cd.GEN_RANDOM, 42),
+         (4096, 16, 2000, -0.4, cd.GEN_BANDED, 43), (512, 30, 4000, -0.3, cd.GEN_RANDOM, 44)]
+
+
+def say(msg):
+    print(f"[gpu-batch] {msg}", file=sys.stderr, flush=True)
+
+
+def make(engine, N, L, S, bias, kind, seed):
+    net = cd.Net(engine, N, L, S, bias)
+    net.generate_layers(kind, seed)
+    net.generate_input(seed)
+    net.upload(cd.default_opts(max_tile_waste=64.0))
+    return net
+
+
+def oracle_cats(N, L, S, bias, kind, seed):
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.BANDED if kind else orc.RANDOM, seed), 0, S)
+    return np.flatnonzero(alive)
+
+
+def test_concurrent_batch_matches_sequential_and_oracle(engine):
+    want = [oracle_cats(*s) for s in SPECS]
+    nets = [make(engine, *s) for s in SPECS]
+    seq = cd.run(nets, cd.default_opts(batch_concurrent=0))
+    for mode, opts in [("eager", cd.default_opts(batch_concurrent=1)),
+                       ("graph", cd.default_opts(batch_concurrent=1, use_graph=1)),
+                       ("graph again", cd.default_opts(batch_concurrent=1, use_graph=1))]:
+        say(f"batch {mode}")
+        outs = cd.run(nets, opts)
+        for s, o, w in zip(seq, outs, want):
+            assert np.array_equal(o.categories, w)
+            assert np.array_equal(o.live, s.live)
+        # every result reports the batch's time
+        assert len({round(o.ms_total, 6) for o in outs}) == 1
+    for s, w in zip(seq, want):
+        assert np.array_equal(s.categories, w)
+    # replacing a network invalidates the captured batch graph
+    nets[1].close()
+    nets[1] = make(engine, *SPECS[1][:5], 45)
+    want[1] = oracle_cats(*SPECS[1][:5], 45)
+    outs = cd.run(nets, cd.default_opts(batch_concurrent=1, use_graph=1))
+    for o, w in zip(outs, want):
+        assert np.array_equal(o.categories, w)
+    for n in nets:
+        n.close()
+
+
+def test_a_network_twice_in_one_call_is_refused(engine):
+    net = make(engine, 256, 4, 100, -0.3, cd.GEN_RANDOM, 46)
+    with pytest.raises(cd.CospdnnError):
+        cd.run([net, net])
+    net.close()

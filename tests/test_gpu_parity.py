@@ -339,3 +339,17 @@ def test_guard_retires_saturated_rows_only_where_all32_is_fixed(engine):
             out = cd.run([net], opts)[0]
             assert np.array_equal(out.categories, want), (L, opts.precision, opts.guard_layer)
         net.close()
+
+
+def test_short_net_late_deaths_are_rechecked(engine):
+    """A row that FP64 keeps alive to the last layer but that the FP16 pass lets die in one
+    of the last layers (N=2048 random, L=12, seed 45 has one: FP64 max pre-activation
+    +0.0096 at layer 11, the FP16 trajectory -0.10) is recomputed: deaths in the last
+    guard_tail layers are flagged like undecided rows."""
+    N, L, S, bias, seed = 2048, 12, 5000, -0.35, 45
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.RANDOM, seed), 0, S)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed, max_waste=64.0)
+    out = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    assert out.n_flagged > 0
+    net.close()
