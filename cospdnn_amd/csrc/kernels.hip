@@ -222,6 +222,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 // --------------------------------------------------------------------------
 constexpr int kWaveSamples = 64;
 constexpr int kWavesPerGroup = 4;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, work-groups dispatched round-robin
 constexpr int kStageLd = kWaveSamples + 4;  // FP32 staging row stride (floats): 16 rows cover all 64 banks
 constexpr int kOutLd = kWaveSamples + 8;    // FP16 output staging row stride (halves, 144 B)
 
@@ -230,20 +231,29 @@ struct StepPos {
   bool valid;
 };
 
-// Item order: block-column-major (cm) puts consecutive items on consecutive 64-sample
-// tiles of one block column, so the waves of a group and neighbouring groups read the
-// same gathered rows at adjacent addresses and write adjacent runs of the same output
-// rows; sample-tile-major puts them on different block columns of one tile.
-__device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int nblk, int ntiles, bool cm,
-                                           cint_ptr_t kptr, bool k1) {
+// Item order.  0 sample-tile-major: consecutive items are the block columns of one
+// 64-sample tile.  1 block-column-major: consecutive items are consecutive tiles of one
+// block column, so the waves of a group and neighbouring groups read the same gathered
+// rows at adjacent addresses and write adjacent runs of the same output rows.
+// 2 XCD-partitioned tile-major: the work-groups of XCD x (blockIdx % 8 under the
+// round-robin dispatch) take the tiles t = x (mod 8), tile after tile, so the input rows of a
+// tile stay in that XCD's L2 while every block column gathers from them (random layers
+// re-read each input row about K times).
+struct ItemOrder {
+  int mode, nblk, ntiles, part, nparts;
+};
+
+__device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, const ItemOrder& io, cint_ptr_t kptr,
+                                           bool k1) {
   s.valid = item < nitems;
   s.item = item;
-  if (cm) {
-    s.c = item / ntiles;
-    s.t = item - s.c * ntiles;
+  if (io.mode == 1) {
+    s.c = item / io.ntiles;
+    s.t = item - s.c * io.ntiles;
   } else {
-    s.t = item / nblk;
-    s.c = item - s.t * nblk;
+    const int it = item / io.nblk;
+    s.c = item - it * io.nblk;
+    s.t = io.mode == 2 ? io.part + io.nparts * it : it;
   }
   if (!s.valid) {
     s.ch = 0;
@@ -257,14 +267,14 @@ __device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, int
   }
 }
 
-__device__ __forceinline__ void step_next(StepPos& s, int stride, int nitems, int nblk, int ntiles, bool cm,
-                                          cint_ptr_t kptr, bool k1) {
+__device__ __forceinline__ void step_next(StepPos& s, int stride, int nitems, const ItemOrder& io, cint_ptr_t kptr,
+                                          bool k1) {
   if (!s.valid) return;
   if (s.ch + 1 < s.ch_end) {
     ++s.ch;
     return;
   }
-  step_first(s, s.item + stride, nitems, nblk, ntiles, cm, kptr, k1);
+  step_first(s, s.item + stride, nitems, io, kptr, k1);
 }
 
 // OR over the 16 lanes of each DPP row (row_ror by 1, 2, 4, 8).
@@ -319,11 +329,19 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   const int n = a.counts[a.layer];
   const int nblk = w.nblk;
   const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
-  const int nitems = ntiles * nblk;
-  const bool cm = a.order != 0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int stride = gridDim.x * kWavesPerGroup;
+  ItemOrder io{a.order, nblk, ntiles, 0, 1};
+  int nitems = ntiles * nblk, first = blockIdx.x * kWavesPerGroup + wave, stride = gridDim.x * kWavesPerGroup;
+  if (a.order == 2) {
+    // this XCD's groups and tiles
+    io.nparts = kXcds;
+    io.part = blockIdx.x % kXcds;
+    const int groups = (static_cast<int>(gridDim.x) - io.part + kXcds - 1) / kXcds;
+    nitems = (ntiles - io.part + kXcds - 1) / kXcds * nblk;
+    first = (blockIdx.x / kXcds) * kWavesPerGroup + wave;
+    stride = groups * kWavesPerGroup;
+  }
   const bool k1 = w.one_chunk != 0;
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
   const cint_ptr_t out_idx = (cint_ptr_t)w.out_idx;
@@ -331,12 +349,12 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
 
   StepPos cur, nx1, nx2;
-  step_first(cur, blockIdx.x * kWavesPerGroup + wave, nitems, nblk, ntiles, cm, kptr, k1);
+  step_first(cur, first, nitems, io, kptr, k1);
   if (!cur.valid) return;  // wave-uniform; nothing below synchronises across waves
   nx1 = cur;
-  step_next(nx1, stride, nitems, nblk, ntiles, cm, kptr, k1);
+  step_next(nx1, stride, nitems, io, kptr, k1);
   nx2 = nx1;
-  step_next(nx2, stride, nitems, nblk, ntiles, cm, kptr, k1);
+  step_next(nx2, stride, nitems, io, kptr, k1);
 
   _Float16* tile = reinterpret_cast<_Float16*>(wlds[wave]);  // [32][64] FP16 input tile (bytes 0..4095)
   _Float16* ostage = tile + 32 * kWaveSamples;               // [32][kOutLd] FP16 outputs (bytes 4096..8703)
@@ -534,7 +552,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     if (!nx1.valid) break;
     cur = nx1;
     nx1 = nx2;
-    step_next(nx2, stride, nitems, nblk, ntiles, cm, kptr, k1);
+    step_next(nx2, stride, nitems, io, kptr, k1);
   }
 }
 

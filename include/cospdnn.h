@@ -65,7 +65,8 @@ typedef struct {
   int layer_timing;      /* 1: time every layer's SpMM with events (reference-style sum) */
   int use_graph;         /* 1: capture the layer loop of each network in a hipGraph and replay */
   int dump_layer;        /* >= 0: copy the activations after this layer into the result */
-  int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major */
+  int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major,
+                            2 sample tiles partitioned over the XCDs (L2 reuse on random layers) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
   int batch_concurrent;  /* several networks in one cospdnn_run: 1 = their main passes run
                             concurrently on side streams (one hipGraph with use_graph),
