@@ -256,8 +256,14 @@ def bench_batch(cd, np, args, specs):
     edges = float(sum(sp.edges for sp in specs))
     seq_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=0)
     outs_s, wall_s = timed_runs_many(cd, nets, seq_opts, args.steps, args.warmup)
-    bat_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=1)
-    outs_b, wall_b = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
+    best = None
+    for chains in (2, 3):
+        bat_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=chains)
+        outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
+        log(f"[bench] {chains} chains: {wall_c * 1e3 / args.steps:.2f} ms")
+        if best is None or wall_c < best[1]:
+            best = (outs_c, wall_c, chains)
+    outs_b, wall_b, chains = best
     same = all(np.array_equal(a.categories, b.categories) for a, b in zip(outs_s[0], outs_b[0]))
     ms_s = wall_s * 1e3 / args.steps
     ms_b = wall_b * 1e3 / args.steps
@@ -280,6 +286,7 @@ def bench_batch(cd, np, args, specs):
         "config": {"workload": f"{args.config}: " + ", ".join(f"(N={sp.N}, L={sp.L}, samples={sp.S})" for sp in specs)
                    + ", random 32 nnz/column of 0.0625", "networks": len(specs), "parallelism": "single GPU"},
         "sequential": {"ms_per_step": ms_s, "value": edges / (ms_s * 1e-3)},
+        "concurrent_chains": chains,
         "concurrent_over_sequential": ms_s / ms_b,
         "categories_equal": same,
         "tuning": tuning,

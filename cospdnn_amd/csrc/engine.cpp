@@ -637,19 +637,40 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   HIP_TRY(hipEventElapsedTime(&ms_first, net->ev[0], net->ev[1]));
   HIP_TRY(hipEventElapsedTime(&ms_all, net->ev[0], net->ev[2]));
   set_times(res, ms_first, ms_all);
+  net->est_main_ms = ms_first;
   return COSPDNN_OK;
 }
 
-// Several networks at once: their main passes run concurrently, each on a stream of its
-// own forked from (and joined back to) the engine's stream, either eagerly or from one
-// hipGraph that holds all the branches; then each network's guard tiers.  Every result
-// reports the times of the whole batch.
+// Several networks at once: their main passes run in `chains` concurrent chains, each a
+// stream of its own forked from (and joined back to) the engine's stream, either eagerly
+// or from one hipGraph that holds all the branches; then each network's guard tiers.
+// Networks go to chains longest first, each to the chain with the least estimated work
+// (last measured main-pass time, else edges), so that short networks run in the shadow
+// of long ones without the long ones, which lean on L2, running against each other.
+// Every result reports the times of the whole batch.
 int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_result* res) {
   cospdnn_engine* e = nets[0]->eng;
   hipStream_t st = e->stream;
   std::vector<RunCfg> cfg(n);
   for (int i = 0; i < n; ++i) CALL_TRY(run_prepare(nets[i], o, cfg[i]));
-  CALL_TRY(ensure_side_streams(e, n));
+  const int chains = std::max(1, std::min(n, o.batch_concurrent));
+  CALL_TRY(ensure_side_streams(e, chains));
+  std::vector<int> chain(n, 0);
+  {
+    std::vector<double> est(n);
+    for (int i = 0; i < n; ++i)
+      est[i] = nets[i]->est_main_ms > 0.0 ? nets[i]->est_main_ms
+                                          : 1e-10 * static_cast<double>(nets[i]->S) * static_cast<double>(nets[i]->nnz_total);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return est[x] > est[y]; });
+    std::vector<double> load(chains, 0.0);
+    for (int i : order) {
+      const int c = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      chain[i] = c;
+      load[c] += est[i];
+    }
+  }
   for (auto* ev : {&e->batch_ev[0], &e->batch_ev[1], &e->batch_ev[2]})
     if (!*ev) HIP_TRY(hipEventCreateWithFlags(ev, hipEventDefault));
   if (static_cast<int>(e->join_ev.size()) < n) {
@@ -662,15 +683,17 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
   const bool graph = o.use_graph != 0 && o.layer_timing == 0;
   auto enqueue_all = [&]() -> int {
     HIP_TRY(hipEventRecord(e->fork_ev, st));
+    for (int c = 0; c < chains; ++c) HIP_TRY(hipStreamWaitEvent(e->side[c], e->fork_ev, 0));
     for (int i = 0; i < n; ++i) {
-      hipStream_t si = e->side[i % e->side.size()];
-      HIP_TRY(hipStreamWaitEvent(si, e->fork_ev, 0));
+      hipStream_t si = e->side[chain[i]];
       side[i] = (nets[i]->L & 1) ^ 1;
       CALL_TRY(enqueue_forward(nets[i], nets[i]->main, nets[i]->store.as<_Float16>(), nets[i]->ld_store, nets[i]->S,
                                cfg[i], o.layer_timing != 0, si, &side[i]));
-      HIP_TRY(hipEventRecord(e->join_ev[i], si));
     }
-    for (int i = 0; i < n; ++i) HIP_TRY(hipStreamWaitEvent(st, e->join_ev[i], 0));
+    for (int c = 0; c < chains; ++c) {
+      HIP_TRY(hipEventRecord(e->join_ev[c], e->side[c]));
+      HIP_TRY(hipStreamWaitEvent(st, e->join_ev[c], 0));
+    }
     return COSPDNN_OK;
   };
   HIP_TRY(hipEventRecord(e->batch_ev[0], st));
@@ -678,7 +701,7 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
     BatchGraph& bg = e->batch_graph;
     bool hit = bg.exec != nullptr && bg.nets.size() == static_cast<size_t>(n);
     for (int i = 0; hit && i < n; ++i)
-      hit = bg.nets[i] == nets[i] && bg.plan_versions[i] == nets[i]->plan_version &&
+      hit = bg.nets[i] == nets[i] && bg.plan_versions[i] == nets[i]->plan_version && bg.chain[i] == chain[i] &&
             graph_matches(bg.key, cfg[i], 0);
     if (!hit) {
       bg.reset();
@@ -697,6 +720,7 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
       bg.nets.assign(nets, nets + n);
       bg.plan_versions.resize(n);
       for (int i = 0; i < n; ++i) bg.plan_versions[i] = nets[i]->plan_version;
+      bg.chain = chain;
       graph_remember(bg.key, cfg[0], 0);
     }
     HIP_TRY(hipGraphLaunch(bg.exec, st));
@@ -756,7 +780,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->dump_layer = -1;
   o->spmm_order = 1;
   o->mfma_groups_per_cu = 4;
-  o->batch_concurrent = 1;
+  o->batch_concurrent = 2;
   o->guard_tail = 8;
 }
 
@@ -1018,7 +1042,8 @@ int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, 
   for (int i = 0; i < n_nets; ++i)
     for (int k = 0; k < i; ++k)
       if (nets[i] == nets[k]) return fail(COSPDNN_ERR_ARG, "a network appears twice in one call");
-  if (n_nets > 1 && o.batch_concurrent) return run_batch(nets, n_nets, o, results);
+  if (o.batch_concurrent < 0) return fail(COSPDNN_ERR_ARG, "batch_concurrent must be >= 0");
+  if (n_nets > 1 && o.batch_concurrent > 0) return run_batch(nets, n_nets, o, results);
   for (int i = 0; i < n_nets; ++i) CALL_TRY(run_one(nets[i], o, &results[i]));
   return COSPDNN_OK;
 }

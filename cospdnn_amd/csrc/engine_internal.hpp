@@ -111,12 +111,14 @@ struct BatchGraph {
   hipGraphExec_t exec = nullptr;
   std::vector<cospdnn_net*> nets;
   std::vector<int64_t> plan_versions;
+  std::vector<int> chain;  // chain (side stream) of each network
   GraphCache key;  // options of the captured passes (its exec stays null)
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     exec = nullptr;
     nets.clear();
     plan_versions.clear();
+    chain.clear();
   }
   ~BatchGraph() { reset(); }
 };
@@ -169,6 +171,7 @@ struct cospdnn_net {
   RunBufs rep;                       // FP64 pass over the rows the FP32 pass leaves undecided
   std::vector<hipEvent_t> layer_ev;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double est_main_ms = -1.0;  // last measured main-pass time (chains of a concurrent run)
   ~cospdnn_net() {
     for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev)

@@ -68,9 +68,9 @@ typedef struct {
   int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major,
                             2 sample tiles partitioned over the XCDs (L2 reuse on random layers) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
-  int batch_concurrent;  /* several networks in one cospdnn_run: 1 = their main passes run
-                            concurrently on side streams (one hipGraph with use_graph),
-                            0 = one after another */
+  int batch_concurrent;  /* several networks in one cospdnn_run: k > 0 = their main passes
+                            run in k concurrent chains on side streams (one hipGraph with
+                            use_graph), networks assigned longest first; 0 = one after another */
   int guard_tail;        /* guarded passes: rows that die in the last guard_tail layers are
                             recomputed at the next precision too */
 } cospdnn_opts;

@@ -35,9 +35,11 @@ def test_concurrent_batch_matches_sequential_and_oracle(engine):
     want = [oracle_cats(*s) for s in SPECS]
     nets = [make(engine, *s) for s in SPECS]
     seq = cd.run(nets, cd.default_opts(batch_concurrent=0))
-    for mode, opts in [("eager", cd.default_opts(batch_concurrent=1)),
-                       ("graph", cd.default_opts(batch_concurrent=1, use_graph=1)),
-                       ("graph again", cd.default_opts(batch_concurrent=1, use_graph=1))]:
+    for mode, opts in [("1 chain, eager", cd.default_opts(batch_concurrent=1)),
+                       ("2 chains, eager", cd.default_opts(batch_concurrent=2)),
+                       ("4 chains, graph", cd.default_opts(batch_concurrent=4, use_graph=1)),
+                       ("4 chains, graph again", cd.default_opts(batch_concurrent=4, use_graph=1)),
+                       ("2 chains, graph", cd.default_opts(batch_concurrent=2, use_graph=1))]:
         say(f"batch {mode}")
         outs = cd.run(nets, opts)
         for s, o, w in zip(seq, outs, want):
@@ -51,7 +53,7 @@ def test_concurrent_batch_matches_sequential_and_oracle(engine):
     nets[1].close()
     nets[1] = make(engine, *SPECS[1][:5], 45)
     want[1] = oracle_cats(*SPECS[1][:5], 45)
-    outs = cd.run(nets, cd.default_opts(batch_concurrent=1, use_graph=1))
+    outs = cd.run(nets, cd.default_opts(batch_concurrent=2, use_graph=1))
     for o, w in zip(outs, want):
         assert np.array_equal(o.categories, w)
     for n in nets:
