@@ -74,6 +74,26 @@ def cpu_baseline(spec, samples, gpu_cats):
     }
 
 
+def max_abs_dy(cd, np, net, spec, samples):
+    """max |Y_l - Y_l(FP64)| of the FP16 pass over the first `samples` samples, after
+    layers l in {3, 6, 12, L} (the activations of the FP16 pass before any guard
+    recompute, against the FP64 oracle truncated to l layers).  Returns {l: value}."""
+    from oracle import oracle_ctypes as orc
+
+    out = {}
+    for l in sorted({min(x, spec.L) for x in (3, 6, 12, spec.L)}):
+        o = cd.default_opts(precision=cd.FP16, dump_layer=l - 1)
+        r = cd.run([net], o, dump_cap_rows=samples)[0]
+        ids = r.dump_ids
+        keep = ids < samples
+        got = np.zeros((samples, spec.N), dtype=np.float64)
+        got[ids[keep]] = r.dump_vals[keep]
+        onet = orc.OracleNet.generated(spec.N, l, spec.bias, spec.kind, spec.seed)
+        _, _, want = orc.forward_fp64(onet, 0, samples, final_ids=np.arange(samples))
+        out[l] = float(np.abs(got - want).max())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +105,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the FP16 layer loop from a hipGraph")
     ap.add_argument("--tune", type=int, default=1, help="1: autotune the per-layer launch plan before timing")
+    ap.add_argument("--dy-samples", type=int, default=1000, help="samples for max |dY| against the FP64 oracle (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -107,7 +128,10 @@ def main():
     net = cd.Net(eng, spec.N, spec.L, spec.S, spec.bias)
     net.generate_layers(spec.kind, spec.seed, K=spec.K, wval=spec.wval)
     net.generate_input(spec.seed)
-    net.upload(cd.default_opts())
+    # tile images up to waste 64 (random layers up to N=16384 may then take the MFMA kernel
+    # when the tuner finds it faster); beyond that the SIMT image only (a random N=65536
+    # layer's tile image would be 128 MB)
+    net.upload(cd.default_opts(max_tile_waste=64.0 if spec.N <= 16384 else 4.0))
     log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
 
     # Autotuning (untimed): exhaustive per-layer search over the launch candidates, then
@@ -183,6 +207,13 @@ def main():
         cpu = cpu_baseline(spec, min(args.cpu_samples, spec.S), cats16)
         log(f"[bench] cpu: {cpu['value']:.3e} edges/s on {cpu['cores']} threads")
 
+    dy = None
+    if args.dy_samples > 0:
+        t0 = time.perf_counter()
+        dy = max_abs_dy(cd, np, net, spec, min(args.dy_samples, spec.S))
+        log(f"[bench] max |dY| of the FP16 pass vs FP64 over {min(args.dy_samples, spec.S)} samples, by layer: "
+            + ", ".join(f"{l}: {v:.3e}" for l, v in dy.items()) + f" ({time.perf_counter() - t0:.1f}s)")
+
     traffic, traffic_src = profiled_traffic(args.config, "spmm_mfma_kernel")
     line = {
         "metric": METRIC,
@@ -226,7 +257,10 @@ def main():
         "recomputed_rows_fp32": int(outs16[0].n_flagged),
         "replayed_rows_fp64": int(outs16[0].n_replayed),
         "guard_tiers_ms": float(outs16[0].ms_recompute),
-        "live_per_layer_head": [int(x) for x in outs16[0].live[:24]],
+        "live_per_layer": [int(x) for x in outs16[0].live],
+        "max_abs_dy_fp16_vs_fp64": max(dy.values()) if dy else None,
+        "max_abs_dy_by_layer": {str(k): v for k, v in dy.items()} if dy else None,
+        "dy_samples": min(args.dy_samples, spec.S) if dy is not None else 0,
     }
     print(json.dumps(line), flush=True)
 

@@ -353,3 +353,17 @@ def test_short_net_late_deaths_are_rechecked(engine):
     assert np.array_equal(out.categories, np.flatnonzero(alive))
     assert out.n_flagged > 0
     net.close()
+
+
+def test_c4_shape_on_a_sample_slice(engine):
+    """The largest network (N=65536, L=480: ~1e9 weights) on the first 1500 samples: the
+    category set equals the FP64 oracle's."""
+    spec = cd.config_nets("c4")[0]
+    S = 1500
+    say(f"c4 shape N={spec.N} L={spec.L} S={S}")
+    alive, live = orc.forward_fp64(orc.OracleNet.generated(spec.N, spec.L, spec.bias, orc.RANDOM, spec.seed), 0, S)
+    net = gen_net(engine, spec.N, spec.L, S, spec.bias, cd.GEN_RANDOM, spec.seed)
+    out = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    assert_live_close(out.live, live, out)
+    net.close()
