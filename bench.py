@@ -242,7 +242,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "spmm_mfma_kernel",
+            "kernel": "spmm_mfma_kernel" if outs16[0].mfma_layers == spec.L else
+                      ("spmm_simt_kernel<half,half>" if outs16[0].mfma_layers == 0 else "spmm_mfma_kernel + spmm_simt_kernel"),
             "avg_launch_ms": spmm_ms / launches,
             "alg_bytes_per_launch": alg_bytes / launches,
         },
