@@ -1,0 +1,156 @@
+// HBM access-pattern probe: read, write and copy bandwidth of plain streams, and of copies
+// shaped like the SpMM's activation traffic (32 rows x 64 samples of FP16 per wave step,
+// rows one activation row apart) with two lane mappings: one lane per 64 bytes of a row,
+// and 8 lanes x 16 bytes per row (every instruction moves whole 128-byte segments).
+// Prints GB/s per pattern; used to choose the SpMM's load/store mapping.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdint>
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+#define CK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("err %s line %d\n", hipGetErrorString(e), __LINE__); return 1; } } while (0)
+
+__global__ void rd(const uint4* __restrict__ a, size_t n, uint32_t* out) {
+  uint32_t s = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint4 v = a[i]; s ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (s == 0x12345678u) out[0] = s;
+}
+__global__ void rd_nt(const uint4* __restrict__ a, size_t n, uint32_t* out) {
+  uint32_t s = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    u4v v = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(a) + i); s ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (s == 0x12345678u) out[0] = s;
+}
+__global__ void wr(uint4* __restrict__ a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = make_uint4((uint32_t)i, 1, 2, 3);
+}
+__global__ void wr_nt(uint4* __restrict__ a, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(u4v{(uint32_t)i, 1u, 2u, 3u}, reinterpret_cast<u4v*>(a) + i);
+}
+__global__ void cp(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+__global__ void cp_nt(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const u4v*>(a) + i), reinterpret_cast<u4v*>(b) + i);
+}
+// unrolled copy: each thread moves 4 x 16 B per iteration (more bytes in flight)
+__global__ void cp4(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i + 3 * stride < n; i += 4 * stride) {
+    uint4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+    b[i] = v0; b[i + stride] = v1; b[i + 2 * stride] = v2; b[i + 3 * stride] = v3;
+  }
+}
+// the SpMM's pattern: a wave reads 32 rows x 128 B (rows ld apart) and writes 32 rows x 128 B
+// work item = (tile t of 64 samples, block c); rows of block c = 32c..32c+31 (identity gather)
+__global__ void tilecopy(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles, int nblk, int order) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lrow = lane >> 1, lcol = (lane & 1) * 32;
+  const int nitems = ntiles * nblk;
+  for (int item = blockIdx.x * 4 + wave; item < nitems; item += gridDim.x * 4) {
+    int t, c;
+    if (order) { c = item / ntiles; t = item - c * ntiles; } else { t = item / nblk; c = item - t * nblk; }
+    const uint4* src = reinterpret_cast<const uint4*>(x + (int64_t)(32 * c + lrow) * ld + t * 64 + lcol);
+    uint4* dst = reinterpret_cast<uint4*>(y + (int64_t)(32 * c + lrow) * ld + t * 64 + lcol);
+    uint4 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+    dst[0] = v0; dst[1] = v1; dst[2] = v2; dst[3] = v3;
+  }
+}
+
+// coalesced mapping: load instruction k covers rows 8k..8k+7, lane -> row 8k + lane/8, 16 B at (lane%8)*16
+template <bool NT>
+__global__ void tilecopy_co(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles, int nblk, int order) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
+  const int nitems = ntiles * nblk;
+  for (int item = blockIdx.x * 4 + wave; item < nitems; item += gridDim.x * 4) {
+    int t, c;
+    if (order) { c = item / ntiles; t = item - c * ntiles; } else { t = item / nblk; c = item - t * nblk; }
+    u4v v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u4v* src = reinterpret_cast<const u4v*>(x + (int64_t)(32 * c + 8 * k + r8) * ld + t * 64 + c8);
+      v[k] = NT ? __builtin_nontemporal_load(src) : *src;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      u4v* dst = reinterpret_cast<u4v*>(y + (int64_t)(32 * c + 8 * k + r8) * ld + t * 64 + c8);
+      if (NT) __builtin_nontemporal_store(v[k], dst); else *dst = v[k];
+    }
+  }
+}
+// 128 samples per wave-step: 16 rows x 256 B per instruction pair
+__global__ void tilecopy_w128(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles2, int nblk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r4 = lane >> 4, c16 = (lane & 15) * 8;
+  const int nitems = ntiles2 * nblk;
+  for (int item = blockIdx.x * 4 + wave; item < nitems; item += gridDim.x * 4) {
+    const int c = item / ntiles2, t = item - c * ntiles2;
+    u4v v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      v[k] = *reinterpret_cast<const u4v*>(x + (int64_t)(32 * c + 4 * k + r4) * ld + t * 128 + c16);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      *reinterpret_cast<u4v*>(y + (int64_t)(32 * c + 4 * k + r4) * ld + t * 128 + c16) = v[k];
+  }
+}
+
+int main() {
+  const size_t bytes = size_t{2} << 30, n = bytes / 16;
+  uint4 *a, *b; uint32_t* o;
+  CK(hipMalloc(&a, bytes)); CK(hipMalloc(&b, bytes)); CK(hipMalloc(&o, 64));
+  CK(hipMemset(a, 1, bytes)); CK(hipMemset(b, 0, bytes));
+  hipEvent_t e0, e1; CK(hipEventCreate(&e0)); CK(hipEventCreate(&e1));
+  auto run = [&](const char* name, double moved, auto launch) -> int {
+    float best = 1e30f;
+    for (int it = 0; it < 4; ++it) {
+      CK(hipEventRecord(e0, 0)); launch(); CK(hipGetLastError()); CK(hipEventRecord(e1, 0)); CK(hipEventSynchronize(e1));
+      float ms; CK(hipEventElapsedTime(&ms, e0, e1)); if (it && ms < best) best = ms;
+    }
+    printf("%-28s %8.3f ms  %7.0f GB/s\n", name, best, moved / (best * 1e-3) / 1e9);
+    return 0;
+  };
+  for (int grid : {1024, 4096, 16384}) {
+    char nm[64];
+    snprintf(nm, 64, "read g=%d", grid); run(nm, bytes, [&] { hipLaunchKernelGGL(rd, dim3(grid), dim3(256), 0, 0, a, n, o); });
+    snprintf(nm, 64, "read_nt g=%d", grid); run(nm, bytes, [&] { hipLaunchKernelGGL(rd_nt, dim3(grid), dim3(256), 0, 0, a, n, o); });
+    snprintf(nm, 64, "write g=%d", grid); run(nm, bytes, [&] { hipLaunchKernelGGL(wr, dim3(grid), dim3(256), 0, 0, b, n); });
+    snprintf(nm, 64, "write_nt g=%d", grid); run(nm, bytes, [&] { hipLaunchKernelGGL(wr_nt, dim3(grid), dim3(256), 0, 0, b, n); });
+    snprintf(nm, 64, "copy g=%d", grid); run(nm, 2.0 * bytes, [&] { hipLaunchKernelGGL(cp, dim3(grid), dim3(256), 0, 0, a, b, n); });
+    snprintf(nm, 64, "copy_nt g=%d", grid); run(nm, 2.0 * bytes, [&] { hipLaunchKernelGGL(cp_nt, dim3(grid), dim3(256), 0, 0, a, b, n); });
+    snprintf(nm, 64, "copy4 g=%d", grid); run(nm, 2.0 * bytes, [&] { hipLaunchKernelGGL(cp4, dim3(grid), dim3(256), 0, 0, a, b, n); });
+  }
+  // tile copy shaped like the SpMM at N=16384 rows, 9984 samples (ld 10240) and 60032 (ld 60160)
+  for (int S : {9984, 60032}) {
+    const int64_t ld = (S + 255) / 256 * 256;
+    const int N = 16384;
+    if ((size_t)N * ld * 2 > bytes) { printf("skip\n"); continue; }
+    const int ntiles = S / 64, nblk = N / 32;
+    for (int order : {0, 1}) for (int grid : {512, 1024, 2048}) {
+      char nm[64]; snprintf(nm, 64, "tilecopy S=%d o=%d g=%d", S, order, grid);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, order); });
+    }
+  }
+  for (int S : {9984, 60032}) {
+    const int64_t ld = (S + 255) / 256 * 256;
+    const int N = 16384;
+    const int ntiles = S / 64, nblk = N / 32;
+    for (int order : {0, 1}) for (int grid : {512, 1024, 2048}) {
+      char nm[64]; snprintf(nm, 64, "tilecopy_co S=%d o=%d g=%d", S, order, grid);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co<false>, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, order); });
+      snprintf(nm, 64, "tilecopy_co_nt S=%d o=%d g=%d", S, order, grid);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co<true>, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, order); });
+    }
+    for (int grid : {512, 1024}) {
+      char nm[64]; snprintf(nm, 64, "tilecopy_w128 S=%d g=%d", S, grid);
+      run(nm, 2.0 * N * (S / 128 * 128) * 2, [&] { hipLaunchKernelGGL(tilecopy_w128, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, S / 128, nblk); });
+    }
+  }
+  return 0;
+}
