@@ -62,7 +62,9 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     rb.in16.reset();
     rb.cap = cap;
     rb.L = L;
-    rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS);
+    // room for act_ld(rows) of every rows <= cap, and for the FP32 MFMA kernel's reads up to
+    // the next multiple of its 256-sample tile
+    rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS) + 64;
     CALL_TRY(dalloc(rb.flags, static_cast<size_t>(6 * rb.ld)));
     CALL_TRY(dalloc(rb.ids, sizeof(int32_t) * 2 * rb.ld));
     CALL_TRY(dalloc(rb.qmap, sizeof(int32_t) * rb.ld));
@@ -154,7 +156,7 @@ struct RunCfg {
   int band_from;
   int use_mfma;     // FP16 pass: MFMA kernel for tiled layers
   int mfma32;       // FP32 pass: MFMA FP32 kernel for tiled layers
-  int order = 1;    // MFMA SpMM item order
+  int order = 3;    // MFMA SpMM item order
   int groups_per_cu = kMfmaGroupsPerCu;
   int retire_sat = 0;  // guard passes: saturated rows leave the pass once all-32 is a fixed point of the rest
   int drop_flagged = 0;  // rows flagged at the checkpoint leave the pass (a later pass decides them)
@@ -241,9 +243,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     const bool fp64 = cfg.precision == COSPDNN_FP64;
     auto side_ptr = [&](int sd) -> void* { return fp16 ? rb.act16[sd].p : (fp64 ? rb.act64[sd].p : rb.act32[sd].p); };
     a.x = l == 0 ? static_cast<const void*>(input) : side_ptr(cur);
-    a.ldx = l == 0 ? ld_in : rb.ld;
+    a.ld_in = ld_in;  // layers >= 1: compacted buffers, act_ld(rows)
     a.y = side_ptr(nxt);
-    a.ldy = rb.ld;
     a.bias = net->bias;
     a.counts = rb.counts.as<int32_t>();
     a.layer = l;
@@ -271,7 +272,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     if (mfma_l) {
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
-      const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{groups_l} * e->cu_count)));
+      int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{groups_l} * e->cu_count)));
+      if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
       a.order = order_l;
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
       if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
@@ -394,7 +396,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
-  if (o.spmm_order < 0 || o.spmm_order > 2) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0, 1 or 2");
+  if (o.spmm_order < 0 || o.spmm_order > 3) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..3");
   if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
     return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
   if (o.guard_tail < 0) return fail(COSPDNN_ERR_ARG, "guard_tail must be >= 0");
@@ -591,7 +593,7 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     const int64_t nr = std::min<int64_t>(rows, res->dump_cap_rows);
     std::vector<int32_t> ids(nr);
     if (nr > 0) HIP_TRY(hipMemcpy(ids.data(), net->main.id(out_side), sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
-    const int64_t ld = net->main.ld;
+    const int64_t ld = act_ld(rows);  // compacted buffer of `rows` rows
     if (nr > 0 && cfg.precision == COSPDNN_FP16) {
       std::vector<_Float16> buf(static_cast<size_t>(N) * nr);
       HIP_TRY(hipMemcpy2D(buf.data(), sizeof(_Float16) * nr, net->main.act16[out_side].p, sizeof(_Float16) * ld,
@@ -778,7 +780,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->layer_timing = 0;
   o->use_graph = 0;
   o->dump_layer = -1;
-  o->spmm_order = 1;
+  o->spmm_order = 3;
   o->mfma_groups_per_cu = 4;
   o->batch_concurrent = 2;
   o->guard_tail = 8;

@@ -84,6 +84,33 @@ __global__ void tilecopy_co(const uint16_t* __restrict__ x, uint16_t* __restrict
     }
   }
 }
+// the coalesced nontemporal copy with two items per loop trip (8 loads in flight per lane)
+__global__ void tilecopy_co2(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles, int nblk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
+  const int nitems = ntiles * nblk;
+  const int stride = gridDim.x * 4;
+  for (int item = blockIdx.x * 4 + wave; item < nitems; item += 2 * stride) {
+    const int item2 = item + stride < nitems ? item + stride : item;
+    u4v v[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int it = h ? item2 : item;
+      const int c = it / ntiles, t = it - c * ntiles;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        v[4 * h + k] = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(x + (int64_t)(32 * c + 8 * k + r8) * ld + t * 64 + c8));
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int it = h ? item2 : item;
+      const int c = it / ntiles, t = it - c * ntiles;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        __builtin_nontemporal_store(v[4 * h + k], reinterpret_cast<u4v*>(y + (int64_t)(32 * c + 8 * k + r8) * ld + t * 64 + c8));
+    }
+  }
+}
 // 128 samples per wave-step: 16 rows x 256 B per instruction pair
 __global__ void tilecopy_w128(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles2, int nblk) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -150,6 +177,17 @@ int main() {
     for (int grid : {512, 1024}) {
       char nm[64]; snprintf(nm, 64, "tilecopy_w128 S=%d g=%d", S, grid);
       run(nm, 2.0 * N * (S / 128 * 128) * 2, [&] { hipLaunchKernelGGL(tilecopy_w128, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, S / 128, nblk); });
+    }
+  }
+  // the steady-state layer shape: 9984 live rows in buffers laid out for 60000 (ld 60160),
+  // against the same rows stored compactly (ld 10240); one and two items per loop trip
+  for (int64_t ld : {int64_t{10240}, int64_t{60160}}) {
+    const int N = 16384, S = 9984, ntiles = S / 64, nblk = N / 32;
+    for (int grid : {1024, 2048}) {
+      char nm[80]; snprintf(nm, 80, "steady co_nt ld=%lld g=%d", (long long)ld, grid);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co<true>, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, 1); });
+      snprintf(nm, 80, "steady co2_nt ld=%lld g=%d", (long long)ld, grid);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co2, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk); });
     }
   }
   return 0;

@@ -222,9 +222,16 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 // --------------------------------------------------------------------------
 constexpr int kWaveSamples = 64;
 constexpr int kWavesPerGroup = 4;
-constexpr int kXcds = 8;  // MI355X: 8 XCDs, work-groups dispatched round-robin
 constexpr int kStageLd = kWaveSamples + 4;  // FP32 staging row stride (floats): 16 rows cover all 64 banks
 constexpr int kOutLd = kWaveSamples + 8;    // FP16 output staging row stride (halves, 144 B)
+
+// leading dimensions of a layer's input and output buffers
+__device__ __forceinline__ int64_t x_ld(const StepArgs& a) {
+  return a.layer == 0 ? a.ld_in : act_ld(__builtin_amdgcn_readfirstlane(a.counts[a.layer]));
+}
+__device__ __forceinline__ int64_t y_ld(const StepArgs& a) {
+  return act_ld(__builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]));
+}
 
 struct StepPos {
   int item, t, c, ch, ch_end;
@@ -238,23 +245,34 @@ struct StepPos {
 // 2 XCD-partitioned tile-major: the work-groups of XCD x (blockIdx % 8 under the
 // round-robin dispatch) take the tiles t = x (mod 8), tile after tile, so the input rows of a
 // tile stay in that XCD's L2 while every block column gathers from them (random layers
-// re-read each input row about K times).
+// re-read each input row about K times).  3 XCD-partitioned block-column-major: XCD x takes
+// the block columns [x nblk / 8, (x + 1) nblk / 8) in block-column-major order, so the
+// rows that neighbouring block columns of a banded layer share are gathered through the
+// same L2.  Orders 2 and 3 need a grid of at least 8 work-groups.
 struct ItemOrder {
-  int mode, nblk, ntiles, part, nparts;
+  int mode, nblk, ntiles, part, nparts, c0;
 };
 
 __device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, const ItemOrder& io, cint_ptr_t kptr,
                                            bool k1) {
+  // wave-uniform by construction; readfirstlane keeps the position arithmetic scalar
+  item = __builtin_amdgcn_readfirstlane(item);
   s.valid = item < nitems;
   s.item = item;
-  if (io.mode == 1) {
-    s.c = item / io.ntiles;
-    s.t = item - s.c * io.ntiles;
+  // (c, t) computed into locals and stored once: stores to different fields on the two
+  // branches get merged into one store with a selected address, which keeps s in memory
+  int c, t;
+  if (io.mode == 1 || io.mode == 3) {
+    const int cc = item / io.ntiles;
+    t = item - cc * io.ntiles;
+    c = io.c0 + cc;
   } else {
     const int it = item / io.nblk;
-    s.c = item - it * io.nblk;
-    s.t = io.mode == 2 ? io.part + io.nparts * it : it;
+    c = item - it * io.nblk;
+    t = io.mode == 2 ? io.part + io.nparts * it : it;
   }
+  s.c = c;
+  s.t = t;
   if (!s.valid) {
     s.ch = 0;
     s.ch_end = 1;
@@ -326,12 +344,12 @@ __device__ __forceinline__ void store_stream(_Float16* p, uint4 v) {
 
 __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
   __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];  // 8.5 KB per wave
-  const int n = a.counts[a.layer];
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
   const int nblk = w.nblk;
   const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  ItemOrder io{a.order, nblk, ntiles, 0, 1};
+  ItemOrder io{a.order, nblk, ntiles, 0, 1, 0};
   int nitems = ntiles * nblk, first = blockIdx.x * kWavesPerGroup + wave, stride = gridDim.x * kWavesPerGroup;
   if (a.order == 2) {
     // this XCD's groups and tiles
@@ -341,12 +359,22 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     nitems = (ntiles - io.part + kXcds - 1) / kXcds * nblk;
     first = (blockIdx.x / kXcds) * kWavesPerGroup + wave;
     stride = groups * kWavesPerGroup;
+  } else if (a.order == 3) {
+    // this XCD's block columns
+    io.nparts = kXcds;
+    io.part = blockIdx.x % kXcds;
+    const int groups = (static_cast<int>(gridDim.x) - io.part + kXcds - 1) / kXcds;
+    io.c0 = io.part * nblk / kXcds;
+    nitems = ((io.part + 1) * nblk / kXcds - io.c0) * ntiles;
+    first = (blockIdx.x / kXcds) * kWavesPerGroup + wave;
+    stride = groups * kWavesPerGroup;
   }
   const bool k1 = w.one_chunk != 0;
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
   const cint_ptr_t out_idx = (cint_ptr_t)w.out_idx;
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
 
   StepPos cur, nx1, nx2;
   step_first(cur, first, nitems, io, kptr, k1);
@@ -379,23 +407,19 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   }
   uint4 d0, d1, d2, d3;
   half8_t w0, w1;
-  int qv, ov0, ov1, oa0, oa1, oa2, oa3;
+  int qv, ov0, ov1;
   {
     const int ch = cur.ch * kChunk + r8;
     const _Float16* base = X + cur.t * kWaveSamples + c8;
-    d0 = load_stream(base + static_cast<int64_t>(gidx[ch]) * a.ldx);
-    d1 = load_stream(base + static_cast<int64_t>(gidx[ch + 8]) * a.ldx);
-    d2 = load_stream(base + static_cast<int64_t>(gidx[ch + 16]) * a.ldx);
-    d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * a.ldx);
+    d0 = load_stream(base + static_cast<int64_t>(gidx[ch]) * ldx);
+    d1 = load_stream(base + static_cast<int64_t>(gidx[ch + 8]) * ldx);
+    d2 = load_stream(base + static_cast<int64_t>(gidx[ch + 16]) * ldx);
+    d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * ldx);
     w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
     w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
     qv = a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
     ov0 = oidx[cur.c * 32 + j];
     ov1 = oidx[cur.c * 32 + 16 + j];
-    oa0 = oidx[cur.c * 32 + r8];
-    oa1 = oidx[cur.c * 32 + r8 + 8];
-    oa2 = oidx[cur.c * 32 + r8 + 16];
-    oa3 = oidx[cur.c * 32 + r8 + 24];
   }
   // accumulators start at the bias: D' + b comes out of the MFMA chain
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
@@ -408,13 +432,43 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
 #pragma unroll
     for (int m = 0; m < 2; ++m) acc[t][m] = bias4;
 
+  // The stores of a fast-path epilogue wait in the wave's output staging area (LDS) and
+  // go out at the start of the next step, before that step's prefetch: a VMEM wait with
+  // loads and stores pending waits for everything, so stores issued at the end of a step
+  // would put a whole store latency in front of every step.  Pending: the tile's first
+  // sample position (wave-uniform; lane i's sample is that + i) and the lane's flag bits.
+  // The 32 output ids sit in the row padding of the staging area (int at half 64 of a row).
+  bool pend = false;
+  int pq0 = 0;
+  uint32_t pfl = 0;
+  auto flush = [&]() {
+    if (!pend) return;
+    const _Float16* orow = ostage + r8 * kOutLd;
+    const uint4 v0 = *reinterpret_cast<const uint4*>(orow + c8);
+    const uint4 v1 = *reinterpret_cast<const uint4*>(orow + 8 * kOutLd + c8);
+    const uint4 v2 = *reinterpret_cast<const uint4*>(orow + 16 * kOutLd + c8);
+    const uint4 v3 = *reinterpret_cast<const uint4*>(orow + 24 * kOutLd + c8);
+    const int po0 = *reinterpret_cast<const int*>(orow + kWaveSamples);
+    const int po1 = *reinterpret_cast<const int*>(orow + 8 * kOutLd + kWaveSamples);
+    const int po2 = *reinterpret_cast<const int*>(orow + 16 * kOutLd + kWaveSamples);
+    const int po3 = *reinterpret_cast<const int*>(orow + 24 * kOutLd + kWaveSamples);
+    _Float16* ybase = Y + pq0 + c8;
+    if (full || po0 >= 0) store_stream(ybase + static_cast<int64_t>(po0) * ldy, v0);
+    if (full || po1 >= 0) store_stream(ybase + static_cast<int64_t>(po1) * ldy, v1);
+    if (full || po2 >= 0) store_stream(ybase + static_cast<int64_t>(po2) * ldy, v2);
+    if (full || po3 >= 0) store_stream(ybase + static_cast<int64_t>(po3) * ldy, v3);
+    if (pfl & 1u) a.alive_next[pq0 + lane] = 1;
+    if (pfl & 2u) a.unsat_next[pq0 + lane] = 1;
+    if (pfl & 4u) a.band_next[pq0 + lane] = 1;
+    pend = false;
+  };
+
   while (true) {
     const bool last = cur.ch + 1 == cur.ch_end;
     const int p_own = cur.t * kWaveSamples + lane;
     const int q_own = p_own < n ? qv : -1;
     const half8_t cw0 = w0, cw1 = w1;
     const int cov0 = ov0, cov1 = ov1;
-    const int coa0 = oa0, coa1 = oa1, coa2 = oa2, coa3 = oa3;
     const int cur_c = cur.c;
     // this step's tile into the wave's LDS slice (waits for the loads issued last step)
     {
@@ -424,25 +478,29 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
       dst[2 * 8 * kWaveSamples / 8] = d2;
       dst[3 * 8 * kWaveSamples / 8] = d3;
     }
-    // next step's loads
+    // Next step's tile rows, last step's stores, then the rest of the next step's loads.
+    // Every global access of a step is issued here, at its start, so that the wait at the
+    // top of the next step finds them done; the tile loads go first as they need this
+    // step's row ids (no load may be waited for after a store: the wait would cover it).
+    const int pt = nx1.valid ? nx1.t : cur.t;
+    const int pch = nx1.valid ? nx1.ch : cur.ch;
+    const int pcn = nx1.valid ? nx1.c : cur.c;
     {
-      const int pt = nx1.valid ? nx1.t : cur.t;
-      const int pch = nx1.valid ? nx1.ch : cur.ch;
-      const int pcn = nx1.valid ? nx1.c : cur.c;
       const _Float16* base = X + pt * kWaveSamples + c8;
-      d0 = load_stream(base + static_cast<int64_t>(rn0) * a.ldx);
-      d1 = load_stream(base + static_cast<int64_t>(rn1) * a.ldx);
-      d2 = load_stream(base + static_cast<int64_t>(rn2) * a.ldx);
-      d3 = load_stream(base + static_cast<int64_t>(rn3) * a.ldx);
+      d0 = load_stream(base + static_cast<int64_t>(rn0) * ldx);
+      d1 = load_stream(base + static_cast<int64_t>(rn1) * ldx);
+      d2 = load_stream(base + static_cast<int64_t>(rn2) * ldx);
+      d3 = load_stream(base + static_cast<int64_t>(rn3) * ldx);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    flush();
+    __builtin_amdgcn_sched_barrier(0);
+    {
       w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
       w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
       qv = a.qmap[min(pt * kWaveSamples + lane, n - 1)];
       ov0 = oidx[pcn * 32 + j];
       ov1 = oidx[pcn * 32 + 16 + j];
-      oa0 = oidx[pcn * 32 + r8];
-      oa1 = oidx[pcn * 32 + r8 + 8];
-      oa2 = oidx[pcn * 32 + r8 + 16];
-      oa3 = oidx[pcn * 32 + r8 + 24];
       const int ch2 = (nx2.valid ? nx2.ch : pch) * kChunk + r8;
       rn0 = gidx[ch2];
       rn1 = gidx[ch2 + 8];
@@ -491,31 +549,19 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
           o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
           o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
         }
-        {
-          const _Float16* orow = ostage + r8 * kOutLd + c8;
-          const uint4 v0 = *reinterpret_cast<const uint4*>(orow);
-          const uint4 v1 = *reinterpret_cast<const uint4*>(orow + 8 * kOutLd);
-          const uint4 v2 = *reinterpret_cast<const uint4*>(orow + 16 * kOutLd);
-          const uint4 v3 = *reinterpret_cast<const uint4*>(orow + 24 * kOutLd);
-          _Float16* ybase = Y + q0 + c8;
-          if (full || coa0 >= 0) store_stream(ybase + static_cast<int64_t>(coa0) * a.ldy, v0);
-          if (full || coa1 >= 0) store_stream(ybase + static_cast<int64_t>(coa1) * a.ldy, v1);
-          if (full || coa2 >= 0) store_stream(ybase + static_cast<int64_t>(coa2) * a.ldy, v2);
-          if (full || coa3 >= 0) store_stream(ybase + static_cast<int64_t>(coa3) * a.ldy, v3);
-        }
+        // output ids of rows j and 16 + j (the four lanes of a row write the same value)
+        *reinterpret_cast<int*>(ostage + j * kOutLd + kWaveSamples) = cov0;
+        *reinterpret_cast<int*>(ostage + (16 + j) * kOutLd + kWaveSamples) = cov1;
         // liveness bits of the lane's 16 samples (bit 4t+r), OR-reduced over the 16 lanes of
         // the row group: lane j + 16g owns sample 16g + j
         const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
         const uint32_t ba = full ? flag_bits<0, true>(acc, true, true, 0.0f) : flag_bits<0, false>(acc, ok0, ok1, 0.0f);
-        if ((ba >> j) & 1u) a.alive_next[q_own] = 1;
-        if (want_unsat) {
-          const uint32_t bu = flag_bits<1, false>(acc, ok0, ok1, 0.0f);
-          if ((bu >> j) & 1u) a.unsat_next[q_own] = 1;
-        }
-        if (want_band) {
-          const uint32_t bb = flag_bits<2, false>(acc, ok0, ok1, a.band);
-          if ((bb >> j) & 1u) a.band_next[q_own] = 1;
-        }
+        uint32_t fl = (ba >> j) & 1u;
+        if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
+        if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
+        pend = true;
+        pq0 = q0;
+        pfl = fl;
       } else {
         // rows died in this tile: stage the pre-activations, then each lane stores the 32
         // outputs of its own sample
@@ -536,7 +582,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
               fl |= v > 0.0f ? 1u : 0u;
               fl |= v < 32.0f ? 2u : 0u;
               fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4u : 0u;
-              Y[static_cast<int64_t>(out) * a.ldy + q_own] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(v, 0.0f, 32.0f));
+              Y[static_cast<int64_t>(out) * ldy + q_own] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(v, 0.0f, 32.0f));
             }
           }
           if (fl & 1u) a.alive_next[q_own] = 1;
@@ -554,6 +600,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     nx1 = nx2;
     step_next(nx2, stride, nitems, io, kptr, k1);
   }
+  flush();
 }
 
 // --------------------------------------------------------------------------
@@ -579,6 +626,7 @@ __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) 
   const int64_t nitems = static_cast<int64_t>(ntiles) * w.nblk;
   const TIn* __restrict__ X = static_cast<const TIn*>(a.x);
   TOut* __restrict__ Y = static_cast<TOut*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
   for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
     const int t = static_cast<int>(item / w.nblk);
     const int c = static_cast<int>(item - static_cast<int64_t>(t) * w.nblk);
@@ -597,18 +645,18 @@ __global__ __launch_bounds__(256) void spmm_simt_kernel(StepArgs a, DevLayer w) 
       for (; k + 8 <= k1; k += 8) {
         TIn xv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * a.ldx];
+        for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * ldx];
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc = madd<TAcc>(acc, static_cast<TAcc>(xv[u]), w.oval[k + u]);
       }
-      for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * a.ldx]), w.oval[k]);
+      for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * ldx]), w.oval[k]);
       const TAcc v = std::is_same<TAcc, double>::value ? static_cast<TAcc>(__dadd_rn(acc, static_cast<double>(a.bias)))
                                                        : acc + static_cast<TAcc>(a.bias);
       fl |= (v > 0) ? 1 : 0;
       fl |= (v < 32) ? 2 : 0;
       fl |= (fabs(static_cast<double>(v)) < a.band || fabs(static_cast<double>(v) - 32.0) < a.band) ? 4 : 0;
       const TAcc y = v < 0 ? TAcc(0) : (v > 32 ? TAcc(32) : v);
-      Y[static_cast<int64_t>(out) * a.ldy + q] = static_cast<TOut>(y);
+      Y[static_cast<int64_t>(out) * ldy + q] = static_cast<TOut>(y);
     }
     if (fl & 1) a.alive_next[q] = 1;
     if (fl & 2) a.unsat_next[q] = 1;
@@ -630,6 +678,7 @@ __global__ __launch_bounds__(256) void spmm_simt_rows_kernel(StepArgs a, DevLaye
   const int64_t nitems = static_cast<int64_t>(ntiles) * w.nblk;
   const TIn* __restrict__ X = static_cast<const TIn*>(a.x);
   TOut* __restrict__ Y = static_cast<TOut*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
   const int o = static_cast<int>(threadIdx.x) / kFewRowsTile;
   const int sp = static_cast<int>(threadIdx.x) % kFewRowsTile;
   for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
@@ -647,15 +696,15 @@ __global__ __launch_bounds__(256) void spmm_simt_rows_kernel(StepArgs a, DevLaye
     for (; k + 8 <= k1; k += 8) {
       TIn xv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * a.ldx];
+      for (int u = 0; u < 8; ++u) xv[u] = xcol[static_cast<int64_t>(w.orow[k + u]) * ldx];
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc = madd<TAcc>(acc, static_cast<TAcc>(xv[u]), w.oval[k + u]);
     }
-    for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * a.ldx]), w.oval[k]);
+    for (; k < k1; ++k) acc = madd<TAcc>(acc, static_cast<TAcc>(xcol[static_cast<int64_t>(w.orow[k]) * ldx]), w.oval[k]);
     const TAcc v = std::is_same<TAcc, double>::value ? static_cast<TAcc>(__dadd_rn(acc, static_cast<double>(a.bias)))
                                                      : acc + static_cast<TAcc>(a.bias);
     const TAcc y = v < 0 ? TAcc(0) : (v > 32 ? TAcc(32) : v);
-    Y[static_cast<int64_t>(out) * a.ldy + q] = static_cast<TOut>(y);
+    Y[static_cast<int64_t>(out) * ldy + q] = static_cast<TOut>(y);
     if (v > 0) a.alive_next[q] = 1;
     if (v < 32) a.unsat_next[q] = 1;
     if (a.band_next != nullptr && (fabs(static_cast<double>(v)) < a.band || fabs(static_cast<double>(v) - 32.0) < a.band))
@@ -679,6 +728,7 @@ __device__ __forceinline__ void spmm_mfma32_body(const StepArgs& a, const DevLay
   const int j = lane & 15, g = lane >> 4;
   const TIn* __restrict__ X = static_cast<const TIn*>(a.x);
   float* __restrict__ Y = static_cast<float*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
   constexpr int kLanesPerRow = kTileS * static_cast<int>(sizeof(TIn)) / 16;
   constexpr int kElemsPerLane = 16 / static_cast<int>(sizeof(TIn));
 
@@ -698,7 +748,7 @@ __device__ __forceinline__ void spmm_mfma32_body(const StepArgs& a, const DevLay
       for (int i = 0; i < kLoads; ++i) {
         const int r = wave * (kChunk / 4) + i * kRowsPerLoad;  // wave-uniform first row
         const int row = w.gidx[ch * kChunk + r + lane / kLanesPerRow];
-        const TIn* src = X + static_cast<int64_t>(row) * a.ldx + s0 + (lane % kLanesPerRow) * kElemsPerLane;
+        const TIn* src = X + static_cast<int64_t>(row) * ldx + s0 + (lane % kLanesPerRow) * kElemsPerLane;
         __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(xs + r * kTileS), 16, 0, 0);
       }
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -735,7 +785,7 @@ __device__ __forceinline__ void spmm_mfma32_body(const StepArgs& a, const DevLay
             fl |= (v > 0.0f) ? 1 : 0;
             fl |= (v < 32.0f) ? 2 : 0;
             fl |= (fabsf(v) < a.band || fabsf(v - 32.0f) < a.band) ? 4 : 0;
-            if (q >= 0) Y[static_cast<int64_t>(outs[m][r]) * a.ldy + q] = fminf(fmaxf(v, 0.0f), 32.0f);
+            if (q >= 0) Y[static_cast<int64_t>(outs[m][r]) * ldy + q] = fminf(fmaxf(v, 0.0f), 32.0f);
           }
         }
       }

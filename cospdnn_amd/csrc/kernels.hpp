@@ -18,6 +18,7 @@
 namespace cospdnn {
 
 constexpr int kTileS = 256;  // samples per work tile (both SpMM kernels)
+constexpr int kXcds = 8;     // MI355X: 8 XCDs, work-groups dispatched to them round-robin
 
 struct DevLayer {
   int nblk = 0;
@@ -33,11 +34,20 @@ struct DevLayer {
   int full_blocks = 0;               // every out_idx slot holds an output (no padding slots)
 };
 
+// Activation buffers are compacted layer by layer: the buffer a layer >= 1 reads holds
+// counts[layer] rows of samples at leading dimension act_ld(counts[layer]), the row count
+// rounded up to an odd multiple of 64 samples (whole 128-byte FP16 segments; an odd multiple
+// keeps the neurons' rows from falling on the same HBM channels).  Only the layer-0 input
+// store has a leading dimension of its own.  act_ld(rows) <= round_up(rows, 256) + 64.
+__host__ __device__ inline int64_t act_ld(int64_t rows) {
+  const int64_t m = ((rows > 1 ? rows : 1) + 63) / 64;
+  return 64 * (m | 1);
+}
+
 struct StepArgs {
   const void* x = nullptr;
-  int64_t ldx = 0;
+  int64_t ld_in = 0;                 // leading dimension of x at layer 0 (the input store)
   void* y = nullptr;
-  int64_t ldy = 0;
   float bias = 0.0f;
   const int32_t* counts = nullptr;
   int layer = 0;                     // n = counts[layer]
@@ -47,7 +57,7 @@ struct StepArgs {
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
   float band = 0.0f;
   int32_t* work = nullptr;           // work-item counter of the MFMA kernel (zeroed by the scan)
-  int order = 1;                     // MFMA kernel: 0 = items in sample-tile-major order, 1 = block-column-major
+  int order = 1;                     // MFMA kernel item order (0..3)
 };
 
 struct ScanArgs {

@@ -139,12 +139,36 @@ def test_edge_shapes(engine):
             net.set_layer_csc(l, cp, rw, vl)
         net.set_input_dense(y0)
         net.upload(cd.default_opts(max_tile_waste=1e9))
-        for prec in (cd.FP32, cd.FP16, cd.FP64):
-            out = cd.run([net], cd.default_opts(precision=prec))[0]
-            assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec)
+        # every MFMA item order: the XCD-partitioned ones on grids smaller than 8 XCDs too
+        for prec, order in [(cd.FP32, 1), (cd.FP16, 0), (cd.FP16, 1), (cd.FP16, 2), (cd.FP16, 3), (cd.FP64, 1)]:
+            out = cd.run([net], cd.default_opts(precision=prec, spmm_order=order))[0]
+            assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec, order)
             if prec == cd.FP32 and L > 0:
                 assert_live_close(out.live, live, out)
         net.close()
+
+
+@pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
+def test_item_orders_and_grids_agree(engine, kind):
+    """Every MFMA work order and grid size computes the same layers: categories equal to the
+    oracle's, final activations identical across orders (each output is one MFMA chain)."""
+    N, L, S, bias, seed = (2048, 24, 5000, -0.4, 31) if kind == cd.GEN_BANDED else (1024, 24, 5000, -0.3, 32)
+    say(f"orders kind={kind}")
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+    net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
+    ref = None
+    for order in (0, 1, 2, 3):
+        for groups in (1, 4):
+            o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1)
+            out = cd.run([net], o, dump_cap_rows=S)[0]
+            assert out.mfma_layers == L
+            assert np.array_equal(out.categories, np.flatnonzero(alive)), (order, groups)
+            got = np.zeros((S, N), np.float32)
+            got[out.dump_ids] = out.dump_vals
+            if ref is None:
+                ref = got
+            assert np.array_equal(got, ref), (order, groups)
+    net.close()
 
 
 def test_fp16_refuses_inexact_values(engine):
