@@ -109,10 +109,27 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
 
 namespace {
 
-// Device images of one layer from its host images (built by build_layer).
+// Activation rows are stored in each layer's block order: the buffer layer l writes holds
+// output slot s of that layer in row s, so the 32 outputs of a block column are 32
+// consecutive rows and a wave's stores stream through neighbouring rows (a copy probe of
+// the SpMM's traffic runs 10% faster into consecutive rows than into the neurons' own
+// rows).  Layer l >= 1 therefore reads its inputs through the previous layer's order;
+// the input store keeps neuron order.  prev_order: the previous layer's out_idx.
+void remap_inputs(BlockedLayer& bl, const std::vector<int32_t>& prev_order, int N) {
+  std::vector<int32_t> row_of(N);
+  for (int s = 0; s < N; ++s) row_of[prev_order[s]] = s;
+  for (int32_t& r : bl.orow) r = row_of[r];
+  for (int32_t& g : bl.gidx) g = row_of[g];
+}
+
+// Device images of one layer from its host images (built by build_layer, inputs remapped).
 int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stable) {
   auto ls = std::make_unique<LayerStore>();
-  CALL_TRY(dupload(ls->out_idx, bl.out_idx));
+  // the first N slots hold a permutation of the neurons, the rest is padding
+  std::vector<int32_t> rows(bl.out_idx.size());
+  for (size_t s = 0; s < rows.size(); ++s) rows[s] = bl.out_idx[s] >= 0 ? static_cast<int32_t>(s) : -1;
+  CALL_TRY(dupload(ls->out_idx, rows));
+  CALL_TRY(dupload(ls->out_logical, std::vector<int32_t>(bl.out_idx.begin(), bl.out_idx.begin() + net->N)));
   CALL_TRY(dupload(ls->optr, bl.optr));
   CALL_TRY(dupload(ls->orow, bl.orow));
   CALL_TRY(dupload(ls->oval, bl.oval));
@@ -156,7 +173,7 @@ struct RunCfg {
   int band_from;
   int use_mfma;     // FP16 pass: MFMA kernel for tiled layers
   int mfma32;       // FP32 pass: MFMA FP32 kernel for tiled layers
-  int order = 3;    // MFMA SpMM item order
+  int order = 1;    // MFMA SpMM item order
   int groups_per_cu = kMfmaGroupsPerCu;
   int retire_sat = 0;  // guard passes: saturated rows leave the pass once all-32 is a fixed point of the rest
   int drop_flagged = 0;  // rows flagged at the checkpoint leave the pass (a later pass decides them)
@@ -594,18 +611,22 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     std::vector<int32_t> ids(nr);
     if (nr > 0) HIP_TRY(hipMemcpy(ids.data(), net->main.id(out_side), sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
     const int64_t ld = act_ld(rows);  // compacted buffer of `rows` rows
+    // activation row k holds neuron neuron_of[k] (the layer's block order)
+    std::vector<int32_t> neuron_of(N);
+    HIP_TRY(hipMemcpy(neuron_of.data(), net->dl[dl]->out_logical.p, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
     if (nr > 0 && cfg.precision == COSPDNN_FP16) {
       std::vector<_Float16> buf(static_cast<size_t>(N) * nr);
       HIP_TRY(hipMemcpy2D(buf.data(), sizeof(_Float16) * nr, net->main.act16[out_side].p, sizeof(_Float16) * ld,
                           sizeof(_Float16) * nr, N, hipMemcpyDeviceToHost));
       for (int64_t r = 0; r < nr; ++r)
-        for (int nn = 0; nn < N; ++nn) res->dump_vals[r * N + nn] = static_cast<float>(buf[static_cast<size_t>(nn) * nr + r]);
+        for (int k = 0; k < N; ++k)
+          res->dump_vals[r * N + neuron_of[k]] = static_cast<float>(buf[static_cast<size_t>(k) * nr + r]);
     } else if (nr > 0) {
       std::vector<float> buf(static_cast<size_t>(N) * nr);
       HIP_TRY(hipMemcpy2D(buf.data(), sizeof(float) * nr, net->main.act32[out_side].p, sizeof(float) * ld,
                           sizeof(float) * nr, N, hipMemcpyDeviceToHost));
       for (int64_t r = 0; r < nr; ++r)
-        for (int nn = 0; nn < N; ++nn) res->dump_vals[r * N + nn] = buf[static_cast<size_t>(nn) * nr + r];
+        for (int k = 0; k < N; ++k) res->dump_vals[r * N + neuron_of[k]] = buf[static_cast<size_t>(k) * nr + r];
     }
     for (int64_t r = 0; r < nr; ++r) res->dump_ids[r] = ids[r];
     res->dump_rows = nr;
@@ -780,7 +801,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->layer_timing = 0;
   o->use_graph = 0;
   o->dump_layer = -1;
-  o->spmm_order = 3;
+  o->spmm_order = 1;
   o->mfma_groups_per_cu = 4;
   o->batch_concurrent = 2;
   o->guard_tail = 8;
@@ -985,6 +1006,7 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
     // threads, one layer per thread, then upload them in order
     const int L = net->L;
     const int batch = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+    std::vector<int32_t> carry;  // out_idx of the last layer of the previous batch
     for (int l0 = 0; l0 < L; l0 += batch) {
       const int l1 = std::min(L, l0 + batch);
       std::vector<BlockedLayer> bl(l1 - l0);
@@ -1002,11 +1024,18 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
           stable[l - l0] = all32_fixed_point(*csc, net->N, net->bias) ? 1 : 0;
         });
       for (auto& t : th) t.join();
-      for (int l = l0; l < l1; ++l) {
+      for (int l = l0; l < l1; ++l)
         if (!ok[l - l0]) return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
+      // inputs of layer l through the row order of layer l-1 (rows of the input store: neurons)
+      th.clear();
+      for (int l = std::max(l0, 1); l < l1; ++l)
+        th.emplace_back([&, l]() { remap_inputs(bl[l - l0], l > l0 ? bl[l - 1 - l0].out_idx : carry, net->N); });
+      for (auto& t : th) t.join();
+      for (int l = l0; l < l1; ++l) {
         CALL_TRY(upload_layer(net, l, bl[l - l0], stable[l - l0] != 0));
         if (!net->gen_layers) net->layers[l] = CscLayer();
       }
+      carry = std::move(bl[l1 - 1 - l0].out_idx);
     }
   }
   CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));

@@ -62,6 +62,7 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct LayerStore {
   DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32;
+  DevBuf out_logical;  // N int32: the neuron id of each activation row this layer writes
   DevLayer view;
   bool has_tiles = false;
   bool has_tiles32 = false;  // FP32 tile image present

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdint>
+#include <utility>
+#include <vector>
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 #define CK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("err %s line %d\n", hipGetErrorString(e), __LINE__); return 1; } } while (0)
 
@@ -111,6 +113,24 @@ __global__ void tilecopy_co2(const uint16_t* __restrict__ x, uint16_t* __restric
     }
   }
 }
+// the coalesced nontemporal copy through row tables: block c reads rows rin[32c..32c+31]
+// and writes rows rout[32c..32c+31] (block-column-major items)
+__global__ void tilecopy_tab(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles, int nblk,
+                             const int* __restrict__ rin, const int* __restrict__ rout) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
+  const int nitems = ntiles * nblk;
+  for (int item = blockIdx.x * 4 + wave; item < nitems; item += gridDim.x * 4) {
+    const int c = item / ntiles, t = item - c * ntiles;
+    u4v v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      v[k] = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(x + (int64_t)rin[32 * c + 8 * k + r8] * ld + t * 64 + c8));
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      __builtin_nontemporal_store(v[k], reinterpret_cast<u4v*>(y + (int64_t)rout[32 * c + 8 * k + r8] * ld + t * 64 + c8));
+  }
+}
 // 128 samples per wave-step: 16 rows x 256 B per instruction pair
 __global__ void tilecopy_w128(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles2, int nblk) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -188,6 +208,45 @@ int main() {
       run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co<true>, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, 1); });
       snprintf(nm, 80, "steady co2_nt ld=%lld g=%d", (long long)ld, grid);
       run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_co2, dim3(grid), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk); });
+    }
+  }
+  // row tables of a banded layer (block c gathers rows c + 512 k, k < 32) against
+  // consecutive rows, and the banded gather through a bit-reversed row placement
+  {
+    const int N = 16384, S = 9984, ntiles = S / 64, nblk = N / 32;
+    const int64_t ld = 10048;  // act_ld(9953): odd multiple of 64
+    std::vector<int> seq(N), band(N), perm(N), bandp(N);
+    for (int i = 0; i < N; ++i) seq[i] = i;
+    for (int c = 0; c < nblk; ++c)
+      for (int k = 0; k < 32; ++k) band[32 * c + k] = c + nblk * k;
+    for (int i = 0; i < N; ++i) {
+      int r = 0;
+      for (int b = 0; b < 14; ++b) r |= ((i >> b) & 1) << (13 - b);
+      perm[i] = r;
+    }
+    for (int i = 0; i < N; ++i) bandp[i] = perm[band[i]];
+    // outputs: a fixed pseudo-random permutation (the banded generator's output order)
+    std::vector<int> rnd(N);
+    for (int i = 0; i < N; ++i) rnd[i] = i;
+    uint64_t st = 12345;
+    for (int i = N - 1; i > 0; --i) {
+      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+      const int j = (int)((st >> 33) % (uint64_t)(i + 1));
+      std::swap(rnd[i], rnd[j]);
+    }
+    int *d_seq, *d_band, *d_bandp, *d_rnd;
+    CK(hipMalloc(&d_seq, N * 4)); CK(hipMalloc(&d_band, N * 4)); CK(hipMalloc(&d_bandp, N * 4)); CK(hipMalloc(&d_rnd, N * 4));
+    CK(hipMemcpy(d_seq, seq.data(), N * 4, hipMemcpyHostToDevice));
+    CK(hipMemcpy(d_band, band.data(), N * 4, hipMemcpyHostToDevice));
+    CK(hipMemcpy(d_bandp, bandp.data(), N * 4, hipMemcpyHostToDevice));
+    CK(hipMemcpy(d_rnd, rnd.data(), N * 4, hipMemcpyHostToDevice));
+    struct { const char* name; const int* in; const int* out; } cases[] = {
+        {"rows seq -> seq", d_seq, d_seq}, {"rows seq -> rnd", d_seq, d_rnd},
+        {"rows band -> rnd", d_band, d_rnd}, {"rows bitrev(band) -> rnd", d_bandp, d_rnd},
+        {"rows band -> seq", d_band, d_seq}};
+    for (auto& cs : cases) {
+      char nm[80]; snprintf(nm, 80, "%s ld=%lld", cs.name, (long long)ld);
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_tab, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, cs.in, cs.out); });
     }
   }
   return 0;

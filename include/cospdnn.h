@@ -65,9 +65,9 @@ typedef struct {
   int layer_timing;      /* 1: time every layer's SpMM with events (reference-style sum) */
   int use_graph;         /* 1: capture the layer loop of each network in a hipGraph and replay */
   int dump_layer;        /* >= 0: copy the activations after this layer into the result */
-  int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major,
-                            2 sample tiles partitioned over the XCDs (L2 reuse on random layers),
-                            3 block columns partitioned over the XCDs (default) */
+  int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major
+                            (default), 2 sample tiles partitioned over the XCDs (L2 reuse on
+                            random layers), 3 block columns partitioned over the XCDs */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
   int batch_concurrent;  /* several networks in one cospdnn_run: k > 0 = their main passes
                             run in k concurrent chains on side streams (one hipGraph with

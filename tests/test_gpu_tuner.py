@@ -63,14 +63,13 @@ def make_net(engine, N, L, S, bias, kind, seed, waste=64.0):
 
 
 def default_plan(net, max_waste=4.0):
-    """The engine's choice without a tuner: the MFMA kernel (default order 3, XCD-partitioned
-    block columns, 4 groups per CU) on layers whose tile waste is at most max_waste (the upload
-    default), else SIMT."""
+    """The engine's choice without a tuner: the MFMA kernel (block-column order, 4 groups
+    per CU) on layers whose tile waste is at most max_waste (the upload default), else SIMT."""
     plan = []
     for l in range(net.L):
         f = net.layer_features(l, net.S, (cd.PLAN_SIMT, 0, 0))
         waste = 2.0 ** (6.0 * float(f[4])) if f[3] > 0 else float("inf")
-        plan.append((cd.PLAN_MFMA, 3, 4) if waste <= max_waste * (1 + 1e-6) else (cd.PLAN_SIMT, 0, 0))
+        plan.append((cd.PLAN_MFMA, 1, 4) if waste <= max_waste * (1 + 1e-6) else (cd.PLAN_SIMT, 0, 0))
     return plan
 
 
