@@ -42,6 +42,7 @@ class Opts(ctypes.Structure):
         ("mfma_groups_per_cu", ctypes.c_int),
         ("batch_concurrent", ctypes.c_int),
         ("guard_tail", ctypes.c_int),
+        ("guard_concurrent", ctypes.c_int),
     ]
 
 

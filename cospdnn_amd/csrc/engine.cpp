@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -62,6 +63,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     rb.in16.reset();
     rb.cap = cap;
     rb.L = L;
+    ++rb.gen;
     // room for act_ld(rows) of every rows <= cap, and for the FP32 MFMA kernel's reads up to
     // the next multiple of its 256-sample tile
     rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS) + 64;
@@ -181,6 +183,8 @@ struct RunCfg {
   // and every check_every layers after it, read the kept-row count and stop once it is 0.
   int check_from = -1, check_every = 16;
   int tail = 0;  // guarded passes: rows that die in the last `tail` layers are flagged too
+  int stop_after_scan = -1;  // >= 0: the pass ends with the scan of this layer (no rows can be left)
+  int grid_cap = 0;          // > 0: at most this many work-groups per SpMM launch
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -197,10 +201,20 @@ int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   return static_cast<int>(g);
 }
 
+// Optional parts of a pass: its row count read from device memory (S_run is then only an
+// upper bound, used for grids and buffer clears), and a callback enqueued right after the
+// scan of fork_layer (the guard pipeline forks from there).
+struct PassHooks {
+  const int32_t* rows_dev = nullptr;
+  int32_t* overflow = nullptr;  // set to 1 when *rows_dev > S_run (the pass then takes S_run rows)
+  int fork_layer = -1;
+  std::function<int()> at_fork;
+};
+
 // Enqueues one layer-major pass (no synchronisation). *cat_side receives the ids side
 // holding the categories after the final scan.
 int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_t ld_in, int64_t S_run,
-                    const RunCfg& cfg, bool timing, hipStream_t st, int* cat_side) {
+                    const RunCfg& cfg, bool timing, hipStream_t st, int* cat_side, const PassHooks* hooks = nullptr) {
   cospdnn_engine* e = net->eng;
   const int L = net->L;
   HIP_TRY(launch_fill_u8(rb.f(0, 0), 1, S_run, st));
@@ -209,7 +223,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   if (L == 0) HIP_TRY(launch_input_alive(input, ld_in, net->N, S_run, rb.f(0, 0), st));
   HIP_TRY(launch_iota_i32(rb.id(0), S_run, st));
   HIP_TRY(launch_fill_i32(rb.counts.as<int32_t>() + 1, 0, L + 1, st));  // targets of the scans' atomics
-  HIP_TRY(launch_set_i32(rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), st));
+  if (hooks && hooks->rows_dev)
+    HIP_TRY(launch_clamp_count(hooks->rows_dev, rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), hooks->overflow, st));
+  else
+    HIP_TRY(launch_set_i32(rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), 0, st));
   HIP_TRY(launch_fill_u8(rb.flag_mark.as<uint8_t>(), 0, S_run, st));
   HIP_TRY(launch_set_i32(rb.ret_count.as<int32_t>(), 0, st));
@@ -219,6 +236,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   std::vector<char> sat_from(L + 1, 1);
   for (int l = L - 1; l >= 0; --l) sat_from[l] = sat_from[l + 1] && net->dl[l]->sat_stable;
   const bool few = S_run <= kFewRows;
+  auto capg = [&](int g) { return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g; };
   for (int l = 0; l <= L; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     ScanArgs sa;
@@ -247,7 +265,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     // scan l sees the deaths of layer l-1
     sa.flag_dead = (D >= 0 && cfg.tail > 0 && l >= 1 && l - 1 >= L - cfg.tail) ? 1 : 0;
     HIP_TRY(launch_scan(sa, S_run, st));
-    if (l == L) break;
+    if (hooks && hooks->at_fork && l == hooks->fork_layer) CALL_TRY(hooks->at_fork());
+    if (l == L || l == cfg.stop_after_scan) break;
     if (cfg.check_from >= 0 && l >= cfg.check_from && (l - cfg.check_from) % std::max(1, cfg.check_every) == 0) {
       int32_t kept = 0;
       HIP_TRY(hipMemcpyAsync(&kept, rb.counts.as<int32_t>() + l + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -290,6 +309,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
       int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{groups_l} * e->cu_count)));
+      grid = capg(grid);
       if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
       a.order = order_l;
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
@@ -297,25 +317,25 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
     } else if (fp16) {
       if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, ls.view, Dtype::F16, Dtype::F16, grid_rows(e, e->bpc_rows, S_run, nblk), st));
+        HIP_TRY(launch_spmm_simt_rows(a, ls.view, Dtype::F16, Dtype::F16, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
       else
-        HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, grid_for(e, e->bpc_simt16, S_run, nblk), st));
+        HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, capg(grid_for(e, e->bpc_simt16, S_run, nblk)), st));
     } else if (fp64) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F64;
       if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F64, grid_rows(e, e->bpc_rows, S_run, nblk), st));
+        HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F64, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
       else
-        HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk), st));
+        HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, capg(grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk)), st));
     } else if (few && !(ls.has_tiles32 && cfg.mfma32)) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F32, grid_rows(e, e->bpc_rows, S_run, nblk), st));
+      HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F32, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
     } else if (ls.has_tiles32 && cfg.mfma32) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk), st));
+      HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, capg(grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk)), st));
     } else if (l == 0) {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, grid_for(e, e->bpc_simt1632, S_run, nblk), st));
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, capg(grid_for(e, e->bpc_simt1632, S_run, nblk)), st));
     } else {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, grid_for(e, e->bpc_simt32, S_run, nblk), st));
+      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, capg(grid_for(e, e->bpc_simt32, S_run, nblk)), st));
     }
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
   }
@@ -325,8 +345,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
 
 // Input rows ids[0..*count) of the network into dst ([N][ldd] FP16), stream-ordered.
 int gather_input_rows(cospdnn_net* net, const int32_t* ids, const int32_t* count, int64_t max_rows, _Float16* dst,
-                      int64_t ldd) {
-  hipStream_t st = net->eng->stream;
+                      int64_t ldd, hipStream_t st) {
   if (net->packed.p)
     HIP_TRY(launch_gather_packed(net->packed.as<uint32_t>(), net->words, ids, count, max_rows, dst, ldd, net->N, st));
   else
@@ -341,24 +360,76 @@ struct PassOut {
   std::vector<int32_t> flags;   // ids (pass-local) flagged at the checkpoint
 };
 
-int read_pass(const cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out) {
-  const int L = net->L;
-  out.counts.assign(L + 2, 0);
-  HIP_TRY(hipMemcpy(out.counts.data(), rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost));
-  int32_t nret = 0;
-  HIP_TRY(hipMemcpy(&nret, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-  const int32_t nlive = out.counts[L + 1];
-  out.cats.assign(static_cast<size_t>(nlive) + nret, 0);
-  if (nlive > 0) HIP_TRY(hipMemcpy(out.cats.data(), rb.id(cat_side), sizeof(int32_t) * nlive, hipMemcpyDeviceToHost));
-  if (nret > 0)
-    HIP_TRY(hipMemcpy(out.cats.data() + nlive, rb.ret_ids.p, sizeof(int32_t) * nret, hipMemcpyDeviceToHost));
-  std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
-  int32_t nf = 0;
-  HIP_TRY(hipMemcpy(&nf, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-  out.flags.assign(nf, 0);
-  if (nf > 0) HIP_TRY(hipMemcpy(out.flags.data(), rb.flag_ids.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost));
-  std::sort(out.flags.begin(), out.flags.end());
+// Pinned host staging of at least n int32 (contents not kept when it grows).
+int host_stage(cospdnn_net* net, size_t n) {
+  if (net->hpin_n >= n) return COSPDNN_OK;
+  if (net->hpin) HIP_TRY(hipHostFree(net->hpin));
+  net->hpin = nullptr;
+  net->hpin_n = 0;
+  const size_t want = std::max<size_t>(n, 4096);
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&net->hpin), sizeof(int32_t) * want, hipHostMallocDefault));
+  net->hpin_n = want;
   return COSPDNN_OK;
+}
+
+// Two round trips: the counters, then the id lists (copies into pinned memory).
+int read_pass(cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out, bool sort_flags = true) {
+  const int L = net->L;
+  hipStream_t st = net->eng->stream;
+  const size_t nc = static_cast<size_t>(L) + 4;  // counts[L+2], ret_count, flag_count
+  CALL_TRY(host_stage(net, nc));
+  int32_t* h = net->hpin;
+  HIP_TRY(hipMemcpyAsync(h, rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(h + L + 2, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(h + L + 3, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  out.counts.assign(h, h + L + 2);
+  const int32_t nlive = out.counts[L + 1], nret = h[L + 2], nf = h[L + 3];
+  const size_t nl = static_cast<size_t>(nlive) + nret + nf;
+  CALL_TRY(host_stage(net, nl));
+  h = net->hpin;
+  if (nlive > 0) HIP_TRY(hipMemcpyAsync(h, rb.id(cat_side), sizeof(int32_t) * nlive, hipMemcpyDeviceToHost, st));
+  if (nret > 0) HIP_TRY(hipMemcpyAsync(h + nlive, rb.ret_ids.p, sizeof(int32_t) * nret, hipMemcpyDeviceToHost, st));
+  if (nf > 0) HIP_TRY(hipMemcpyAsync(h + nlive + nret, rb.flag_ids.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, st));
+  if (nl > 0) HIP_TRY(hipStreamSynchronize(st));
+  out.cats.assign(h, h + nlive + nret);
+  std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
+  out.flags.assign(h + nlive + nret, h + nl);
+  if (sort_flags) std::sort(out.flags.begin(), out.flags.end());
+  return COSPDNN_OK;
+}
+
+// The FP32 tier: the rows undecided at the checkpoint, layer-major in FP32 (MFMA FP32 on
+// tiled layers); rows still undecided at guard_layer32 leave it for the FP64 tier.
+RunCfg tier32_cfg(const cospdnn_opts& o, int L) {
+  RunCfg c;
+  c.precision = COSPDNN_FP32;
+  c.guard_layer = o.guard_layer32;
+  c.band = 0.0f;
+  c.band_from = 1 << 30;
+  c.use_mfma = 0;
+  c.mfma32 = 1;
+  c.retire_sat = 1;
+  c.drop_flagged = 1;  // the rows handed to the FP64 tier are not carried further here
+  c.check_from = o.guard_layer32 >= 0 ? std::min(o.guard_layer32, L) : 16;
+  c.check_every = 16;
+  c.tail = o.guard_tail;
+  return c;
+}
+
+// The FP64 tier (bit-identical to the FP64 oracle).
+RunCfg tier64_cfg() {
+  RunCfg c;
+  c.precision = COSPDNN_FP64;
+  c.guard_layer = -1;
+  c.band = 0.0f;
+  c.band_from = 1 << 30;
+  c.use_mfma = 0;
+  c.mfma32 = 0;
+  c.retire_sat = 1;
+  c.check_from = 16;
+  c.check_every = 16;
+  return c;
 }
 
 // Layer-major FP64 pass over the given original sample ids (bit-identical to the FP64
@@ -373,17 +444,8 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   HIP_TRY(hipMemcpyAsync(rb.flag_ids.p, ids.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice, st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), nrows, st));
   CALL_TRY(gather_input_rows(net, rb.flag_ids.as<int32_t>(), rb.flag_count.as<int32_t>(), nrows, rb.in16.as<_Float16>(),
-                             rb.ld));
-  RunCfg cfg;
-  cfg.precision = COSPDNN_FP64;
-  cfg.guard_layer = -1;
-  cfg.band = 0.0f;
-  cfg.band_from = 1 << 30;
-  cfg.use_mfma = 0;
-  cfg.mfma32 = 0;
-  cfg.retire_sat = 1;
-  cfg.check_from = 16;
-  cfg.check_every = 16;
+                             rb.ld, st));
+  const RunCfg cfg = tier64_cfg();
   int side = 0;
   CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, st, &side));
   HIP_TRY(hipStreamSynchronize(st));
@@ -442,13 +504,28 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   return COSPDNN_OK;
 }
 
-bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version) {
+// What a captured graph depends on besides the pass options: the device-side guard
+// pipeline (if captured) and the allocations it references.
+struct GraphExtra {
+  int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;
+  int64_t gen_main = -1, gen_rec = -1, gen_rep = -1;
+};
+
+bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version, const GraphExtra& x = GraphExtra()) {
   return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
-         gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version;
+         gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
+         gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
+         gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
 
-void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version) {
+void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, const GraphExtra& x = GraphExtra()) {
+  gc.dev_guard = x.dev_guard;
+  gc.guard_layer32 = x.guard_layer32;
+  gc.guard_tail = x.guard_tail;
+  gc.gen_main = x.gen_main;
+  gc.gen_rec = x.gen_rec;
+  gc.gen_rep = x.gen_rep;
   gc.precision = cfg.precision;
   gc.guard_layer = cfg.guard_layer;
   gc.band = cfg.band;
@@ -460,22 +537,99 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version) {
   gc.plan_version = plan_version;
 }
 
-// The main pass of one network on stream st: eagerly, or (use_graph, no per-layer events)
-// replayed from the network's cached hipGraph.
-int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side) {
+// Device-side guard tiers, enqueued on s1 once the checkpoint scan of the main pass has
+// listed its undecided rows (main.flag_ids[0, gs[0])): the FP32 tier over those rows, then
+// the FP64 tier over the rows the FP32 tier leaves undecided (their sample ids composed
+// into gids, count gs[1]).  Both passes take their row counts from device memory and
+// never synchronise with the host; more rows than the buffers hold set gs[2] (the host
+// then redoes the tiers one after another).
+int enqueue_device_guard(cospdnn_net* net, const cospdnn_opts& o, hipStream_t s1) {
+  const int L = net->L;
+  int32_t* gs = net->gs.as<int32_t>();
+  RunBufs& rec = net->rec;
+  RunBufs& rep = net->rep;
+  CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), gs, net->gcap32, rec.in16.as<_Float16>(), rec.ld, s1));
+  RunCfg c32 = tier32_cfg(o, L);
+  c32.check_from = -1;  // no host checks: the pass runs on the device only
+  // When all-32 is a fixed point from its checkpoint on, the checkpoint scan retires or
+  // drops every row still alive, so the pass ends there.
+  const int d32 = c32.guard_layer;
+  bool sat_rest = d32 >= 0 && d32 < L;
+  for (int l = std::max(d32, 0); sat_rest && l < L; ++l) sat_rest = net->dl[l]->sat_stable;
+  if (sat_rest) c32.stop_after_scan = d32;
+  // few rows: one work-group per CU leaves the main pass most of the GPU
+  c32.grid_cap = net->eng->cu_count;
+  PassHooks h32;
+  h32.rows_dev = gs;
+  h32.overflow = gs + 2;
+  int side = 0;
+  CALL_TRY(enqueue_forward(net, rec, rec.in16.as<_Float16>(), rec.ld, net->gcap32, c32, false, s1, &side, &h32));
+  HIP_TRY(launch_compose_ids(net->main.flag_ids.as<int32_t>(), rec.flag_ids.as<int32_t>(), rec.flag_count.as<int32_t>(),
+                             static_cast<int32_t>(net->gcap64), net->gids.as<int32_t>(), gs + 1, gs + 2, s1));
+  CALL_TRY(gather_input_rows(net, net->gids.as<int32_t>(), gs + 1, net->gcap64, rep.in16.as<_Float16>(), rep.ld, s1));
+  RunCfg c64 = tier64_cfg();
+  c64.check_from = -1;
+  c64.grid_cap = net->eng->cu_count;
+  PassHooks h64;
+  h64.rows_dev = gs + 1;
+  h64.overflow = gs + 2;
+  CALL_TRY(enqueue_forward(net, rep, rep.in16.as<_Float16>(), rep.ld, net->gcap64, c64, false, s1, &side, &h64));
+  return COSPDNN_OK;
+}
+
+// The main pass with the guard tiers forked off after its checkpoint scan onto the
+// engine's first side stream, and joined back at the end.
+int enqueue_main_guarded(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side) {
+  cospdnn_engine* e = net->eng;
+  hipStream_t s1 = e->side[0];
+  int32_t* gs = net->gs.as<int32_t>();
+  HIP_TRY(launch_set_i32(gs + 2, 0, st));
+  PassHooks h;
+  h.fork_layer = cfg.guard_layer;
+  h.at_fork = [&]() -> int {
+    // the checkpoint's list before later scans append the rows that die in the tail layers
+    HIP_TRY(launch_clamp_count(net->main.flag_count.as<int32_t>(), gs, static_cast<int32_t>(net->gcap32), gs + 2, st));
+    HIP_TRY(hipEventRecord(net->gev[0], st));
+    HIP_TRY(hipStreamWaitEvent(s1, net->gev[0], 0));
+    CALL_TRY(enqueue_device_guard(net, o, s1));
+    HIP_TRY(hipEventRecord(net->gev[1], s1));
+    return COSPDNN_OK;
+  };
+  CALL_TRY(enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, false, st, cat_side, &h));
+  HIP_TRY(hipStreamWaitEvent(st, net->gev[1], 0));
+  return COSPDNN_OK;
+}
+
+// The main pass of one network on stream st (with the device-side guard tiers when
+// dev_guard): eagerly, or (use_graph, no per-layer events) replayed from the network's
+// cached hipGraph.
+int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side,
+                 bool dev_guard = false) {
   const bool timing = o.layer_timing != 0;
   *cat_side = (net->L & 1) ^ 1;
-  if (o.use_graph == 0 || timing)
-    return enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, st, cat_side);
+  auto body = [&](int* side) -> int {
+    if (dev_guard) return enqueue_main_guarded(net, o, cfg, st, side);
+    return enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, st, side);
+  };
+  if (o.use_graph == 0 || timing) return body(cat_side);
   GraphCache& gc = net->graph;
-  if (!graph_matches(gc, cfg, net->plan_version)) {
+  GraphExtra x;
+  if (dev_guard) {
+    x.dev_guard = 1;
+    x.guard_layer32 = o.guard_layer32;
+    x.guard_tail = o.guard_tail;
+    x.gen_main = net->main.gen;
+    x.gen_rec = net->rec.gen;
+    x.gen_rep = net->rep.gen;
+  }
+  if (!graph_matches(gc, cfg, net->plan_version, x)) {
     if (gc.exec) {
       (void)hipGraphExecDestroy(gc.exec);
       gc.exec = nullptr;
     }
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int side = 0;
-    int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, false, st, &side);
+    int rc = body(&side);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(st, &g);
     if (rc != COSPDNN_OK) {
@@ -486,7 +640,7 @@ int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hip
     hipError_t ie = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIP_TRY(ie);
-    graph_remember(gc, cfg, net->plan_version);
+    graph_remember(gc, cfg, net->plan_version, x);
   }
   HIP_TRY(hipGraphLaunch(gc.exec, st));
   return COSPDNN_OK;
@@ -494,6 +648,10 @@ int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hip
 
 // After the main pass has finished (stream synchronised): categories, the FP32 and FP64
 // guard tiers on the engine's stream, and the result fields except the times.
+int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, std::vector<int64_t>& live,
+                  const std::vector<char>& alive, const std::vector<char>& first, int64_t n_flagged,
+                  int64_t n_replayed, cospdnn_result* res);
+
 int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int cat_side, cospdnn_result* res) {
   cospdnn_engine* e = net->eng;
   hipStream_t st = e->stream;
@@ -516,19 +674,8 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
     HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
     CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), net->main.flag_count.as<int32_t>(), nflag,
-                               net->rec.in16.as<_Float16>(), net->rec.ld));
-    RunCfg rcfg;
-    rcfg.precision = COSPDNN_FP32;
-    rcfg.guard_layer = o.guard_layer32;
-    rcfg.band = 0.0f;
-    rcfg.band_from = 1 << 30;
-    rcfg.use_mfma = 0;
-    rcfg.mfma32 = 1;
-    rcfg.retire_sat = 1;
-    rcfg.drop_flagged = 1;  // the rows handed to the FP64 tier are not carried further here
-    rcfg.check_from = o.guard_layer32 >= 0 ? std::min(o.guard_layer32, L) : 16;
-    rcfg.check_every = 16;
-    rcfg.tail = o.guard_tail;
+                               net->rec.in16.as<_Float16>(), net->rec.ld, st));
+    const RunCfg rcfg = tier32_cfg(o, L);
     int rside = 0;
     CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, st, &rside));
     HIP_TRY(hipStreamSynchronize(st));
@@ -548,7 +695,88 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
     n_replayed = static_cast<int64_t>(replay_ids.size());
   }
+  return finish_result(net, o, cfg, live, alive, first, n_flagged, n_replayed, res);
+}
 
+// After a main pass with the device-side guard tiers (stream synchronised): the FP32 tier's
+// and the FP64 tier's decisions over the checkpoint's rows, then the FP64 tier for the rows
+// listed after the checkpoint (deaths in the tail layers).  An overflow of the tiers'
+// buffers falls back to run_finish.
+int run_finish_dev_guard(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int cat_side,
+                         cospdnn_result* res) {
+  const int L = net->L;
+  const int64_t S = net->S;
+  int32_t gsc[3] = {0, 0, 0};
+  HIP_TRY(hipMemcpy(gsc, net->gs.p, sizeof(gsc), hipMemcpyDeviceToHost));
+  if (gsc[2] != 0) return run_finish(net, o, cfg, cat_side, res);
+  PassOut p1;
+  CALL_TRY(read_pass(net, net->main, cat_side, p1, false));  // flags in list order
+  std::vector<int64_t> live(L + 1, -1);
+  for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
+  std::vector<char> alive(S, 0);
+  for (int32_t id : p1.cats) alive[id] = 1;
+  const std::vector<char> first = alive;
+  const int32_t n32 = gsc[0], n64 = gsc[1];
+  const int side = (L & 1) ^ 1;  // the tiers run every layer
+  // FP32 tier: its pass-local ids are positions in the checkpoint's list
+  PassOut p2;
+  CALL_TRY(read_pass(net, net->rec, side, p2));
+  for (int32_t i = 0; i < n32; ++i) alive[p1.flags[i]] = 0;
+  for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
+  // FP64 tier: its pass-local ids are positions in gids
+  std::vector<int32_t> ids64(n64);
+  if (n64 > 0) HIP_TRY(hipMemcpy(ids64.data(), net->gids.p, sizeof(int32_t) * n64, hipMemcpyDeviceToHost));
+  PassOut p3;
+  CALL_TRY(read_pass(net, net->rep, side, p3));
+  for (int32_t id : ids64) alive[id] = 0;
+  for (int32_t local : p3.cats) alive[ids64[local]] = 1;
+  // rows listed after the checkpoint: straight to the FP64 tier
+  std::vector<int32_t> late(p1.flags.begin() + n32, p1.flags.end());
+  if (!late.empty()) {
+    std::sort(late.begin(), late.end());
+    std::vector<uint8_t> a64;
+    CALL_TRY(fp64_rows(net, late, a64));
+    for (size_t i = 0; i < late.size(); ++i) alive[late[i]] = static_cast<char>(a64[i]);
+  }
+  return finish_result(net, o, cfg, live, alive, first, static_cast<int64_t>(p1.flags.size()),
+                       static_cast<int64_t>(n64) + static_cast<int64_t>(late.size()), res);
+}
+
+// The device-side guard tiers apply to a guarded FP16 pass with its checkpoint inside the
+// network, run without per-layer events.
+bool use_device_guard(const cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg) {
+  return o.guard_concurrent != 0 && cfg.precision == COSPDNN_FP16 && cfg.guard_layer > 0 && cfg.guard_layer < net->L &&
+         o.layer_timing == 0;
+}
+
+// Buffers, stream and events of the device-side guard tiers (before anything is enqueued:
+// allocations synchronise the device).  The FP32 tier holds up to a quarter of the
+// samples (at least 4096), the FP64 tier up to kFewRows.
+int prepare_device_guard(cospdnn_net* net) {
+  cospdnn_engine* e = net->eng;
+  CALL_TRY(ensure_side_streams(e, 1));
+  for (auto& ev : net->gev)
+    if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  const int64_t S = net->S;
+  net->gcap32 = std::min<int64_t>(S, std::max<int64_t>(4096, S / 4));
+  net->gcap64 = std::min<int64_t>(net->gcap32, kFewRows);
+  CALL_TRY(ensure_bufs(net->rec, net->gcap32, net->L, net->N, COSPDNN_FP32, true, e->stream));
+  CALL_TRY(ensure_bufs(net->rep, net->gcap64, net->L, net->N, COSPDNN_FP64, true, e->stream));
+  if (!net->gs.p) CALL_TRY(dalloc(net->gs, 4 * sizeof(int32_t)));
+  if (net->gids.bytes < sizeof(int32_t) * static_cast<size_t>(net->gcap64))
+    CALL_TRY(dalloc(net->gids, sizeof(int32_t) * static_cast<size_t>(net->gcap64)));
+  return COSPDNN_OK;
+}
+
+// The result fields (categories, live counts, counters, bytes; the dump of an inner
+// layer) from the final per-sample decisions.  live[l]: rows entering layer l in the
+// main pass; first: the main pass's own decisions.
+int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, std::vector<int64_t>& live,
+                  const std::vector<char>& alive, const std::vector<char>& first, int64_t n_flagged,
+                  int64_t n_replayed, cospdnn_result* res) {
+  hipStream_t st = net->eng->stream;
+  const int N = net->N, L = net->L;
+  const int64_t S = net->S;
   int64_t ncat = 0, changed = 0;
   for (int64_t s = 0; s < S; ++s) {
     if (alive[s]) {
@@ -648,12 +876,17 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   hipStream_t st = net->eng->stream;
   RunCfg cfg;
   CALL_TRY(run_prepare(net, o, cfg));
+  const bool dev_guard = use_device_guard(net, o, cfg);
+  if (dev_guard) CALL_TRY(prepare_device_guard(net));
   HIP_TRY(hipEventRecord(net->ev[0], st));
   int cat_side = 0;
-  CALL_TRY(enqueue_main(net, o, cfg, st, &cat_side));
+  CALL_TRY(enqueue_main(net, o, cfg, st, &cat_side, dev_guard));
   HIP_TRY(hipEventRecord(net->ev[1], st));
   HIP_TRY(hipStreamSynchronize(st));
-  CALL_TRY(run_finish(net, o, cfg, cat_side, res));
+  if (dev_guard)
+    CALL_TRY(run_finish_dev_guard(net, o, cfg, cat_side, res));
+  else
+    CALL_TRY(run_finish(net, o, cfg, cat_side, res));
   HIP_TRY(hipEventRecord(net->ev[2], st));
   HIP_TRY(hipStreamSynchronize(st));
   float ms_first = 0.0f, ms_all = 0.0f;
@@ -805,6 +1038,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->mfma_groups_per_cu = 4;
   o->batch_concurrent = 2;
   o->guard_tail = 8;
+  o->guard_concurrent = 1;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {

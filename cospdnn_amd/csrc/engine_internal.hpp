@@ -75,6 +75,7 @@ struct LayerStore {
 struct RunBufs {
   int64_t cap = 0, ld = 0;
   int L = 0;
+  int64_t gen = 0;  // bumped whenever the buffers are reallocated (captured graphs hold their addresses)
   DevBuf act16[2], act32[2], act64[2];
   DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
   DevBuf ids;    // 2 x ld int32
@@ -100,6 +101,9 @@ struct GraphCache {
   hipGraphExec_t exec = nullptr;
   int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
   int64_t plan_version = -1;
+  // the device-side guard pipeline captured with the main pass (0: main pass only)
+  int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;
+  int64_t gen_main = -1, gen_rec = -1, gen_rep = -1;
   float band = 0.0f;
   ~GraphCache() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -170,6 +174,14 @@ struct cospdnn_net {
   std::vector<cospdnn_layer_plan> plan;  // per-layer launch plan of the FP16 pass (empty: options decide)
   int64_t plan_version = 0;
   RunBufs rep;                       // FP64 pass over the rows the FP32 pass leaves undecided
+  // device-side guard pipeline: gs = {rows of the FP32 pass, rows of the FP64 pass, overflow},
+  // gids = sample ids of the FP64 pass; fork/join events of its side stream
+  DevBuf gs, gids;
+  int64_t gcap32 = 0, gcap64 = 0;
+  // pinned host staging for the results of a pass (hipHostMalloc)
+  int32_t* hpin = nullptr;
+  size_t hpin_n = 0;
+  hipEvent_t gev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> layer_ev;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double est_main_ms = -1.0;  // last measured main-pass time (chains of a concurrent run)
@@ -177,6 +189,9 @@ struct cospdnn_net {
     for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : gev)
+      if (e) (void)hipEventDestroy(e);
+    if (hpin) (void)hipHostFree(hpin);
   }
 };
 

@@ -69,6 +69,29 @@ __global__ void iota_i32_kernel(int32_t* p, int64_t n) {
 }
 __global__ void set_i32_kernel(int32_t* p, int32_t v) { *p = v; }
 
+// *dst = min(*src, cap); *overflow = 1 when *src > cap (one thread)
+__global__ void clamp_count_kernel(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int32_t cap,
+                                   int32_t* __restrict__ overflow) {
+  const int32_t v = *src;
+  *dst = v < cap ? v : cap;
+  if (v > cap) *overflow = 1;
+}
+
+// dst[i] = outer[inner[i]] for i < min(*count, cap), *dst_count = that number;
+// *overflow = 1 when *count > cap.  inner holds positions in the list outer.
+__global__ void compose_ids_kernel(const int32_t* __restrict__ outer, const int32_t* __restrict__ inner,
+                                   const int32_t* __restrict__ count, int32_t cap, int32_t* __restrict__ dst,
+                                   int32_t* __restrict__ dst_count, int32_t* __restrict__ overflow) {
+  const int32_t v = *count;
+  const int32_t n = v < cap ? v : cap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *dst_count = n;
+    if (v > cap) *overflow = 1;
+  }
+  if (i < n) dst[i] = outer[inner[i]];
+}
+
 // --------------------------------------------------------------------------
 // scan: work-group g owns positions [4096g, 4096g+4096); thread t of its 256
 // owns 16 consecutive positions and moves them with 16-byte loads and stores.
@@ -891,6 +914,16 @@ hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
 hipError_t launch_iota_i32(int32_t* p, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(iota_i32_kernel, dim3(fill_grid(n)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+hipError_t launch_clamp_count(const int32_t* src, int32_t* dst, int32_t cap, int32_t* overflow, hipStream_t st) {
+  hipLaunchKernelGGL(clamp_count_kernel, dim3(1), dim3(1), 0, st, src, dst, cap, overflow);
+  return hipGetLastError();
+}
+hipError_t launch_compose_ids(const int32_t* outer, const int32_t* inner, const int32_t* count, int32_t cap,
+                              int32_t* dst, int32_t* dst_count, int32_t* overflow, hipStream_t st) {
+  const unsigned grid = static_cast<unsigned>(std::max<int32_t>(1, (cap + 255) / 256));
+  hipLaunchKernelGGL(compose_ids_kernel, dim3(grid), dim3(256), 0, st, outer, inner, count, cap, dst, dst_count, overflow);
   return hipGetLastError();
 }
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st) {

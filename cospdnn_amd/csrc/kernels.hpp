@@ -101,6 +101,12 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 hipError_t launch_iota_i32(int32_t* p, int64_t n, hipStream_t st);
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st);
+// Device-side row counts of passes enqueued before their size is known:
+// *dst = min(*src, cap), *overflow = 1 when *src > cap.
+hipError_t launch_clamp_count(const int32_t* src, int32_t* dst, int32_t cap, int32_t* overflow, hipStream_t st);
+// dst[i] = outer[inner[i]] for i < min(*count, cap) (grid covers cap), *dst_count = that number.
+hipError_t launch_compose_ids(const int32_t* outer, const int32_t* inner, const int32_t* count, int32_t cap,
+                              int32_t* dst, int32_t* dst_count, int32_t* overflow, hipStream_t st);
 // max_rows: upper bound of counts[layer] (the grid covers it).  counts[layer+1] must be 0
 // beforehand: no group writes it when counts[layer] is 0.
 hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st);

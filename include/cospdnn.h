@@ -74,6 +74,10 @@ typedef struct {
                             use_graph), networks assigned longest first; 0 = one after another */
   int guard_tail;        /* guarded passes: rows that die in the last guard_tail layers are
                             recomputed at the next precision too */
+  int guard_concurrent;  /* FP16 with a checkpoint, one network, no per-layer timing: 1 (default)
+                            = the FP32 and FP64 tiers run on the device on a side stream, forked
+                            off after the checkpoint scan, next to the rest of the main pass;
+                            0 = after it, one tier after the other */
 } cospdnn_opts;
 
 typedef struct {

@@ -216,6 +216,45 @@ def test_graph_replay_matches_eager(engine):
     net.close()
 
 
+@pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
+def test_concurrent_guard_matches_sequential(engine, kind):
+    """The guard tiers forked onto a side stream (device-side row counts, eager and replayed
+    from a hipGraph, several replays with eager runs between them) decide every row as the
+    tiers run one after the other do, and as the FP64 oracle does."""
+    N, L, S, bias, seed = (2048, 40, 6000, -0.4, 41) if kind == cd.GEN_BANDED else (1024, 40, 6000, -0.3, 42)
+    say(f"concurrent guard kind={kind}")
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
+    seq = cd.run([net], cd.default_opts(precision=cd.FP16, guard_concurrent=0))[0]
+    assert np.array_equal(seq.categories, want)
+    assert seq.n_flagged > 0
+    for graph in (0, 1, 1, 1):
+        out = cd.run([net], cd.default_opts(precision=cd.FP16, guard_concurrent=1, use_graph=graph))[0]
+        assert np.array_equal(out.categories, want), graph
+        assert np.array_equal(out.live, seq.live)
+        assert out.n_flagged == seq.n_flagged
+        e = cd.run([net], cd.default_opts(precision=cd.FP16, guard_concurrent=0))[0]
+        assert np.array_equal(e.categories, want)
+    net.close()
+
+
+def test_concurrent_guard_overflow_falls_back(engine):
+    """Checkpoints at layer 1 of both guarded passes leave more rows for the FP64 tier than
+    its device buffers hold (512): the run redoes the tiers one after the other and still
+    matches the oracle."""
+    N, L, S, bias, seed = 1024, 12, 3000, -0.3, 43
+    say("concurrent guard overflow")
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, cd.GEN_RANDOM, seed), 0, S)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed, 64.0)
+    for graph in (0, 1):
+        o = cd.default_opts(precision=cd.FP16, guard_layer=1, guard_layer32=1, use_graph=graph)
+        out = cd.run([net], o)[0]
+        assert out.n_replayed > 512
+        assert np.array_equal(out.categories, np.flatnonzero(alive))
+    net.close()
+
+
 def test_layer_timing_and_counters(engine):
     N, L, S, bias, seed = 4096, 20, 4000, -0.35, 3
     net = gen_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
