@@ -139,26 +139,44 @@ __device__ __forceinline__ uint4 kept_flags(const ScanArgs& a, int pos) {
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   __shared__ int wsum[kScanThreads / 64];
   __shared__ int wpre[kScanThreads / 64];
-  const int n = a.counts[a.layer];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (blockIdx.x == 0 && tid == 0 && a.work != nullptr) *a.work = 0;
   const int g0 = blockIdx.x * kScanTile;
   const int pc = g0 + tid * kScanPer;  // multiple of 16: aligned, inside the ld-sized arrays
-  if (g0 >= n) return;                 // uniform per group
-  // kept rows before this group: positions [0, g0) are all < n
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  // Every load is issued before the row count is known (positions below max_rows are
+  // inside the arrays), so the kernel waits for memory about once.
+  const bool mine = pc < a.max_rows;
+  uint4 v0 = zero, vu = zero, vb = zero;
+  int4 id4[kScanPer / 4];
+  if (mine) {
+    v0 = *reinterpret_cast<const uint4*>(a.alive + pc);
+    vu = *reinterpret_cast<const uint4*>(a.unsat + pc);
+    if (a.band != nullptr) vb = *reinterpret_cast<const uint4*>(a.band + pc);
+#pragma unroll
+    for (int k = 0; k < kScanPer / 4; ++k) id4[k] = *reinterpret_cast<const int4*>(a.ids_in + pc + 4 * k);
+  }
+  // kept rows before this group, [0, g0): four 16-flag words per thread in flight
   int before = 0;
-  for (int pos = tid * kScanPer; pos < g0; pos += kScanTile) before += count_flags16(kept_flags(a, pos));
+  for (int base = 0; base < g0; base += 4 * kScanTile) {
+    uint4 w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int pos = base + u * kScanTile + tid * kScanPer;
+      w[u] = pos < g0 ? kept_flags(a, pos) : zero;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) before += count_flags16(w[u]);
+  }
+  const int n = a.counts[a.layer];
+  if (blockIdx.x == 0 && tid == 0 && a.work != nullptr) *a.work = 0;
+  if (g0 >= n) return;  // uniform per group; positions [0, g0) are all < n
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
   if (lane == 0) wpre[wv] = before;
 
-  const uint4 zero = {0u, 0u, 0u, 0u};
-  uint4 va = zero, vu = zero, vb = zero, vr = zero, vf = zero;
+  uint4 va = zero, vr = zero, vf = zero;
   int cnt = 0;
   if (pc < n) {
-    const uint4 v0 = *reinterpret_cast<const uint4*>(a.alive + pc);
-    vu = *reinterpret_cast<const uint4*>(a.unsat + pc);
-    if (a.band != nullptr) vb = *reinterpret_cast<const uint4*>(a.band + pc);
     // flags are 0/1 bytes.  retired = alive & !unsat; flagged (checkpoint) = alive & unsat
     if (a.retire_sat)
       vr = make_uint4(v0.x & ~vu.x & 0x01010101u, v0.y & ~vu.y & 0x01010101u, v0.z & ~vu.z & 0x01010101u,
@@ -198,21 +216,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 #pragma unroll
   for (int i = 0; i < kScanPer; ++i) {
     const int p = pc + i;
+    const int32_t id = (i & 3) == 0 ? id4[i >> 2].x : ((i & 3) == 1 ? id4[i >> 2].y : ((i & 3) == 2 ? id4[i >> 2].z : id4[i >> 2].w));
     const bool al = p < n && flag_at(va, i);
     qm[i] = -1;
     if (al) {
       qm[i] = q;
-      a.ids_out[q] = a.ids_in[p];
+      a.ids_out[q] = id;
       ++q;
     }
-    if (p < n && flag_at(vr, i)) a.ret_ids[atomicAdd(a.ret_count, 1)] = a.ids_in[p];
+    if (p < n && flag_at(vr, i)) a.ret_ids[atomicAdd(a.ret_count, 1)] = id;
     const bool fl = p < n && (flag_at(vb, i) || flag_at(vf, i));
-    if (fl) {
-      const int32_t id = a.ids_in[p];
-      if (a.flag_mark[id] == 0) {
-        a.flag_mark[id] = 1;
-        a.flag_ids[atomicAdd(a.flag_count, 1)] = id;
-      }
+    if (fl && a.flag_mark[id] == 0) {
+      a.flag_mark[id] = 1;
+      a.flag_ids[atomicAdd(a.flag_count, 1)] = id;
     }
   }
 #pragma unroll
@@ -932,7 +948,9 @@ hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st) {
 }
 hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st) {
   const int64_t groups = std::max<int64_t>(1, (max_rows + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(scan_kernel, dim3(static_cast<unsigned>(groups)), dim3(kScanThreads), 0, st, a);
+  ScanArgs b = a;
+  b.max_rows = static_cast<int>(max_rows);
+  hipLaunchKernelGGL(scan_kernel, dim3(static_cast<unsigned>(groups)), dim3(kScanThreads), 0, st, b);
   return hipGetLastError();
 }
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {

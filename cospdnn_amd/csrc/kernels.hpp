@@ -63,6 +63,7 @@ struct StepArgs {
 struct ScanArgs {
   int32_t* counts = nullptr;
   int layer = 0;                     // reads counts[layer], writes counts[layer+1]
+  int max_rows = 0;                  // upper bound of counts[layer] (set by launch_scan)
   const uint8_t* alive = nullptr;
   const uint8_t* unsat = nullptr;
   const uint8_t* band = nullptr;     // may be null
