@@ -136,6 +136,7 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
   CALL_TRY(dupload(ls->orow, bl.orow));
   CALL_TRY(dupload(ls->oval, bl.oval));
   ls->view.nblk = bl.nblk;
+  ls->view.n_out = net->N;
   ls->view.out_idx = ls->out_idx.as<int32_t>();
   ls->view.optr = ls->optr.as<int32_t>();
   ls->view.orow = ls->orow.as<int32_t>();

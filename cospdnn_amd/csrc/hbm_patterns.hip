@@ -243,7 +243,7 @@ int main() {
     struct { const char* name; const int* in; const int* out; } cases[] = {
         {"rows seq -> seq", d_seq, d_seq}, {"rows seq -> rnd", d_seq, d_rnd},
         {"rows band -> rnd", d_band, d_rnd}, {"rows bitrev(band) -> rnd", d_bandp, d_rnd},
-        {"rows band -> seq", d_band, d_seq}};
+        {"rows band -> seq", d_band, d_seq}, {"rows rnd -> seq", d_rnd, d_seq}, {"rows rnd -> rnd", d_rnd, d_rnd}};
     for (auto& cs : cases) {
       char nm[80]; snprintf(nm, 80, "%s ld=%lld", cs.name, (long long)ld);
       run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_tab, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, cs.in, cs.out); });

@@ -384,6 +384,8 @@ __device__ __forceinline__ void store_stream(_Float16* p, uint4 v) {
 __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
   __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];  // 8.5 KB per wave
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  // no row left the pass in the scan before this layer: every position maps to itself
+  const bool ident = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]) == n;
   const int nblk = w.nblk;
   const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
   const int lane = threadIdx.x & 63;
@@ -431,7 +433,6 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   const int r8 = lane >> 3, c8 = (lane & 7) * 8;
   const int j = lane & 15, g = lane >> 4;
   const int32_t* __restrict__ gidx = w.gidx;
-  const int32_t* __restrict__ oidx = w.out_idx;
 
   // Everything a step consumes is loaded one step ahead: its tile rows (d0..d3), its W
   // fragments, the target position of this lane's sample, the output ids; the row ids
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   }
   uint4 d0, d1, d2, d3;
   half8_t w0, w1;
-  int qv, ov0, ov1;
+  int qv;
   {
     const int ch = cur.ch * kChunk + r8;
     const _Float16* base = X + cur.t * kWaveSamples + c8;
@@ -456,9 +457,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * ldx);
     w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
     w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
-    qv = a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
-    ov0 = oidx[cur.c * 32 + j];
-    ov1 = oidx[cur.c * 32 + 16 + j];
+    qv = ident ? cur.t * kWaveSamples + lane : a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
   }
   // accumulators start at the bias: D' + b comes out of the MFMA chain
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
@@ -478,7 +477,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   // sample position (wave-uniform; lane i's sample is that + i) and the lane's flag bits.
   // The 32 output ids sit in the row padding of the staging area (int at half 64 of a row).
   bool pend = false;
-  int pq0 = 0;
+  int pq0 = 0, pcb = 0;
   uint32_t pfl = 0;
   auto flush = [&]() {
     if (!pend) return;
@@ -487,10 +486,11 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     const uint4 v1 = *reinterpret_cast<const uint4*>(orow + 8 * kOutLd + c8);
     const uint4 v2 = *reinterpret_cast<const uint4*>(orow + 16 * kOutLd + c8);
     const uint4 v3 = *reinterpret_cast<const uint4*>(orow + 24 * kOutLd + c8);
-    const int po0 = *reinterpret_cast<const int*>(orow + kWaveSamples);
-    const int po1 = *reinterpret_cast<const int*>(orow + 8 * kOutLd + kWaveSamples);
-    const int po2 = *reinterpret_cast<const int*>(orow + 16 * kOutLd + kWaveSamples);
-    const int po3 = *reinterpret_cast<const int*>(orow + 24 * kOutLd + kWaveSamples);
+    const int s0 = pcb * 32 + r8;  // output slots of the lane's four rows: s0, +8, +16, +24
+    const int po0 = s0 < w.n_out ? s0 : -1;
+    const int po1 = s0 + 8 < w.n_out ? s0 + 8 : -1;
+    const int po2 = s0 + 16 < w.n_out ? s0 + 16 : -1;
+    const int po3 = s0 + 24 < w.n_out ? s0 + 24 : -1;
     _Float16* ybase = Y + pq0 + c8;
     if (full || po0 >= 0) store_stream(ybase + static_cast<int64_t>(po0) * ldy, v0);
     if (full || po1 >= 0) store_stream(ybase + static_cast<int64_t>(po1) * ldy, v1);
@@ -507,7 +507,9 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     const int p_own = cur.t * kWaveSamples + lane;
     const int q_own = p_own < n ? qv : -1;
     const half8_t cw0 = w0, cw1 = w1;
-    const int cov0 = ov0, cov1 = ov1;
+    // output rows j and 16 + j of the block column (slot = row, -1 for padding)
+    const int cov0 = cur.c * 32 + j < w.n_out ? cur.c * 32 + j : -1;
+    const int cov1 = cur.c * 32 + 16 + j < w.n_out ? cur.c * 32 + 16 + j : -1;
     const int cur_c = cur.c;
     // this step's tile into the wave's LDS slice (waits for the loads issued last step)
     {
@@ -523,7 +525,6 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     // step's row ids (no load may be waited for after a store: the wait would cover it).
     const int pt = nx1.valid ? nx1.t : cur.t;
     const int pch = nx1.valid ? nx1.ch : cur.ch;
-    const int pcn = nx1.valid ? nx1.c : cur.c;
     {
       const _Float16* base = X + pt * kWaveSamples + c8;
       d0 = load_stream(base + static_cast<int64_t>(rn0) * ldx);
@@ -537,9 +538,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     {
       w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
       w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
-      qv = a.qmap[min(pt * kWaveSamples + lane, n - 1)];
-      ov0 = oidx[pcn * 32 + j];
-      ov1 = oidx[pcn * 32 + 16 + j];
+      qv = ident ? pt * kWaveSamples + lane : a.qmap[min(pt * kWaveSamples + lane, n - 1)];
       const int ch2 = (nx2.valid ? nx2.ch : pch) * kChunk + r8;
       rn0 = gidx[ch2];
       rn1 = gidx[ch2 + 8];
@@ -588,9 +587,6 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
           o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
           o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
         }
-        // output ids of rows j and 16 + j (the four lanes of a row write the same value)
-        *reinterpret_cast<int*>(ostage + j * kOutLd + kWaveSamples) = cov0;
-        *reinterpret_cast<int*>(ostage + (16 + j) * kOutLd + kWaveSamples) = cov1;
         // liveness bits of the lane's 16 samples (bit 4t+r), OR-reduced over the 16 lanes of
         // the row group: lane j + 16g owns sample 16g + j
         const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
@@ -600,6 +596,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
         if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
         pend = true;
         pq0 = q0;
+        pcb = cur_c;
         pfl = fl;
       } else {
         // rows died in this tile: stage the pre-activations, then each lane stores the 32

@@ -22,7 +22,10 @@ constexpr int kXcds = 8;     // MI355X: 8 XCDs, work-groups dispatched to them r
 
 struct DevLayer {
   int nblk = 0;
-  const int32_t* out_idx = nullptr;  // [nblk*32]
+  // [nblk*32]: the activation row of each output slot.  Rows are stored in block order, so
+  // out_idx[s] = s for s < n_out and -1 for the padding slots after them.
+  const int32_t* out_idx = nullptr;
+  int n_out = 0;
   const int32_t* optr = nullptr;     // [nblk*32+1]
   const int32_t* orow = nullptr;
   const float* oval = nullptr;
