@@ -2,7 +2,10 @@
 // shaped like the SpMM's activation traffic (32 rows x 64 samples of FP16 per wave step,
 // rows one activation row apart) with two lane mappings: one lane per 64 bytes of a row,
 // and 8 lanes x 16 bytes per row (every instruction moves whole 128-byte segments).
-// Prints GB/s per pattern; used to choose the SpMM's load/store mapping.
+// Also: the steady layer's shape with sparse and dense row strides, copies through row
+// tables (banded gather, permuted and consecutive rows), and sample blocks sized for the
+// memory-side cache.  Prints GB/s per pattern; used to choose the SpMM's mapping, row
+// stride and row order.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdint>
@@ -115,6 +118,7 @@ __global__ void tilecopy_co2(const uint16_t* __restrict__ x, uint16_t* __restric
 }
 // the coalesced nontemporal copy through row tables: block c reads rows rin[32c..32c+31]
 // and writes rows rout[32c..32c+31] (block-column-major items)
+template <bool NT>
 __global__ void tilecopy_tab(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int64_t ld, int ntiles, int nblk,
                              const int* __restrict__ rin, const int* __restrict__ rout) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -124,11 +128,15 @@ __global__ void tilecopy_tab(const uint16_t* __restrict__ x, uint16_t* __restric
     const int c = item / ntiles, t = item - c * ntiles;
     u4v v[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      v[k] = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(x + (int64_t)rin[32 * c + 8 * k + r8] * ld + t * 64 + c8));
+    for (int k = 0; k < 4; ++k) {
+      const u4v* src = reinterpret_cast<const u4v*>(x + (int64_t)rin[32 * c + 8 * k + r8] * ld + t * 64 + c8);
+      v[k] = NT ? __builtin_nontemporal_load(src) : *src;
+    }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      __builtin_nontemporal_store(v[k], reinterpret_cast<u4v*>(y + (int64_t)rout[32 * c + 8 * k + r8] * ld + t * 64 + c8));
+    for (int k = 0; k < 4; ++k) {
+      u4v* dst = reinterpret_cast<u4v*>(y + (int64_t)rout[32 * c + 8 * k + r8] * ld + t * 64 + c8);
+      if (NT) __builtin_nontemporal_store(v[k], dst); else *dst = v[k];
+    }
   }
 }
 // 128 samples per wave-step: 16 rows x 256 B per instruction pair
@@ -246,7 +254,23 @@ int main() {
         {"rows band -> seq", d_band, d_seq}, {"rows rnd -> seq", d_rnd, d_seq}, {"rows rnd -> rnd", d_rnd, d_rnd}};
     for (auto& cs : cases) {
       char nm[80]; snprintf(nm, 80, "%s ld=%lld", cs.name, (long long)ld);
-      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_tab, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, cs.in, cs.out); });
+      run(nm, 2.0 * N * S * 2, [&] { hipLaunchKernelGGL(tilecopy_tab<true>, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ld, ntiles, nblk, cs.in, cs.out); });
+    }
+    // sample blocks small enough for the 256 MB memory-side cache: one block ping-ponged
+    // between two buffers (rnd -> seq rows), plain and nontemporal accesses
+    for (int Sb : {1216, 2496, 4992, 9984}) {
+      const int64_t ldb = 64 * (((Sb + 63) / 64) | 1);
+      const int nt = Sb / 64;
+      for (int ntm = 0; ntm < 2; ++ntm) {
+        char nm[80]; snprintf(nm, 80, "block S=%d %s (x2 buf %.0f MB)", Sb, ntm ? "nt" : "plain", 2.0 * N * ldb * 2 / 1e6);
+        run(nm, 2.0 * N * Sb * 2, [&] {
+          if (ntm) {
+            hipLaunchKernelGGL(tilecopy_tab<true>, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ldb, nt, nblk, d_rnd, d_seq);
+          } else {
+            hipLaunchKernelGGL(tilecopy_tab<false>, dim3(1024), dim3(256), 0, 0, (const uint16_t*)a, (uint16_t*)b, ldb, nt, nblk, d_rnd, d_seq);
+          }
+        });
+      }
     }
   }
   return 0;
