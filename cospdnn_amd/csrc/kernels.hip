@@ -471,11 +471,11 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     for (int m = 0; m < 2; ++m) acc[t][m] = bias4;
 
   // The stores of a fast-path epilogue wait in the wave's output staging area (LDS) and
-  // go out at the start of the next step, before that step's prefetch: a VMEM wait with
+  // go out at the start of the next step, next to that step's prefetch: a VMEM wait with
   // loads and stores pending waits for everything, so stores issued at the end of a step
   // would put a whole store latency in front of every step.  Pending: the tile's first
-  // sample position (wave-uniform; lane i's sample is that + i) and the lane's flag bits.
-  // The 32 output ids sit in the row padding of the staging area (int at half 64 of a row).
+  // sample position (wave-uniform; lane i's sample is that + i), its block column (the
+  // output rows follow from it) and the lane's flag bits.
   bool pend = false;
   int pq0 = 0, pcb = 0;
   uint32_t pfl = 0;
