@@ -578,11 +578,10 @@ int enqueue_device_guard(cospdnn_net* net, const cospdnn_opts& o, hipStream_t s1
   return COSPDNN_OK;
 }
 
-// The main pass with the guard tiers forked off after its checkpoint scan onto the
-// engine's first side stream, and joined back at the end.
-int enqueue_main_guarded(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side) {
-  cospdnn_engine* e = net->eng;
-  hipStream_t s1 = e->side[0];
+// The main pass on st with the guard tiers forked off after its checkpoint scan onto s1,
+// and joined back at the end.
+int enqueue_main_guarded(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hipStream_t st, int* cat_side,
+                         hipStream_t s1) {
   int32_t* gs = net->gs.as<int32_t>();
   HIP_TRY(launch_set_i32(gs + 2, 0, st));
   PassHooks h;
@@ -601,6 +600,17 @@ int enqueue_main_guarded(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& 
   return COSPDNN_OK;
 }
 
+GraphExtra guard_extra(const cospdnn_net* net, const cospdnn_opts& o) {
+  GraphExtra x;
+  x.dev_guard = 1;
+  x.guard_layer32 = o.guard_layer32;
+  x.guard_tail = o.guard_tail;
+  x.gen_main = net->main.gen;
+  x.gen_rec = net->rec.gen;
+  x.gen_rep = net->rep.gen;
+  return x;
+}
+
 // The main pass of one network on stream st (with the device-side guard tiers when
 // dev_guard): eagerly, or (use_graph, no per-layer events) replayed from the network's
 // cached hipGraph.
@@ -609,20 +619,12 @@ int enqueue_main(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, hip
   const bool timing = o.layer_timing != 0;
   *cat_side = (net->L & 1) ^ 1;
   auto body = [&](int* side) -> int {
-    if (dev_guard) return enqueue_main_guarded(net, o, cfg, st, side);
+    if (dev_guard) return enqueue_main_guarded(net, o, cfg, st, side, net->eng->side[0]);
     return enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, net->S, cfg, timing, st, side);
   };
   if (o.use_graph == 0 || timing) return body(cat_side);
   GraphCache& gc = net->graph;
-  GraphExtra x;
-  if (dev_guard) {
-    x.dev_guard = 1;
-    x.guard_layer32 = o.guard_layer32;
-    x.guard_tail = o.guard_tail;
-    x.gen_main = net->main.gen;
-    x.gen_rec = net->rec.gen;
-    x.gen_rep = net->rep.gen;
-  }
+  const GraphExtra x = dev_guard ? guard_extra(net, o) : GraphExtra();
   if (!graph_matches(gc, cfg, net->plan_version, x)) {
     if (gc.exec) {
       (void)hipGraphExecDestroy(gc.exec);
@@ -910,8 +912,19 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
   hipStream_t st = e->stream;
   std::vector<RunCfg> cfg(n);
   for (int i = 0; i < n; ++i) CALL_TRY(run_prepare(nets[i], o, cfg[i]));
-  const int chains = std::max(1, std::min(n, o.batch_concurrent));
-  CALL_TRY(ensure_side_streams(e, chains));
+  const int chains = std::max(1, std::min({n, o.batch_concurrent, kMaxSideStreams / 2}));
+  // chain c runs its networks' main passes on side[c] and their guard tiers on side[chains + c]
+  CALL_TRY(ensure_side_streams(e, 2 * chains));
+  std::vector<char> dev_guard(n, 0);
+  for (int i = 0; i < n; ++i) {
+    dev_guard[i] = use_device_guard(nets[i], o, cfg[i]) ? 1 : 0;
+    if (dev_guard[i]) CALL_TRY(prepare_device_guard(nets[i]));
+  }
+  std::vector<std::array<int64_t, 6>> gkeys(n);
+  for (int i = 0; i < n; ++i) {
+    const GraphExtra x = dev_guard[i] ? guard_extra(nets[i], o) : GraphExtra();
+    gkeys[i] = {x.dev_guard, x.guard_layer32, x.guard_tail, x.gen_main, x.gen_rec, x.gen_rep};
+  }
   std::vector<int> chain(n, 0);
   {
     std::vector<double> est(n);
@@ -944,8 +957,11 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
     for (int i = 0; i < n; ++i) {
       hipStream_t si = e->side[chain[i]];
       side[i] = (nets[i]->L & 1) ^ 1;
-      CALL_TRY(enqueue_forward(nets[i], nets[i]->main, nets[i]->store.as<_Float16>(), nets[i]->ld_store, nets[i]->S,
-                               cfg[i], o.layer_timing != 0, si, &side[i]));
+      if (dev_guard[i])
+        CALL_TRY(enqueue_main_guarded(nets[i], o, cfg[i], si, &side[i], e->side[chains + chain[i]]));
+      else
+        CALL_TRY(enqueue_forward(nets[i], nets[i]->main, nets[i]->store.as<_Float16>(), nets[i]->ld_store, nets[i]->S,
+                                 cfg[i], o.layer_timing != 0, si, &side[i]));
     }
     for (int c = 0; c < chains; ++c) {
       HIP_TRY(hipEventRecord(e->join_ev[c], e->side[c]));
@@ -959,7 +975,7 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
     bool hit = bg.exec != nullptr && bg.nets.size() == static_cast<size_t>(n);
     for (int i = 0; hit && i < n; ++i)
       hit = bg.nets[i] == nets[i] && bg.plan_versions[i] == nets[i]->plan_version && bg.chain[i] == chain[i] &&
-            graph_matches(bg.key, cfg[i], 0);
+            bg.guard_keys[i] == gkeys[i] && graph_matches(bg.key, cfg[i], 0);
     if (!hit) {
       bg.reset();
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -978,6 +994,7 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
       bg.plan_versions.resize(n);
       for (int i = 0; i < n; ++i) bg.plan_versions[i] = nets[i]->plan_version;
       bg.chain = chain;
+      bg.guard_keys = gkeys;
       graph_remember(bg.key, cfg[0], 0);
     }
     HIP_TRY(hipGraphLaunch(bg.exec, st));
@@ -986,7 +1003,12 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
   }
   HIP_TRY(hipEventRecord(e->batch_ev[1], st));
   HIP_TRY(hipStreamSynchronize(st));
-  for (int i = 0; i < n; ++i) CALL_TRY(run_finish(nets[i], o, cfg[i], side[i], &res[i]));
+  for (int i = 0; i < n; ++i) {
+    if (dev_guard[i])
+      CALL_TRY(run_finish_dev_guard(nets[i], o, cfg[i], side[i], &res[i]));
+    else
+      CALL_TRY(run_finish(nets[i], o, cfg[i], side[i], &res[i]));
+  }
   HIP_TRY(hipEventRecord(e->batch_ev[2], st));
   HIP_TRY(hipStreamSynchronize(st));
   float ms_first = 0.0f, ms_all = 0.0f;

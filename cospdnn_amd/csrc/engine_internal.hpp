@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -117,6 +118,8 @@ struct BatchGraph {
   std::vector<cospdnn_net*> nets;
   std::vector<int64_t> plan_versions;
   std::vector<int> chain;  // chain (side stream) of each network
+  // per network: device-side guard captured, its checkpoint and tail, buffer generations
+  std::vector<std::array<int64_t, 6>> guard_keys;
   GraphCache key;  // options of the captured passes (its exec stays null)
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -124,6 +127,7 @@ struct BatchGraph {
     nets.clear();
     plan_versions.clear();
     chain.clear();
+    guard_keys.clear();
   }
   ~BatchGraph() { reset(); }
 };
