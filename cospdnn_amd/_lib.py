@@ -158,7 +158,7 @@ def lib():
         L.cospdnn_net_set_plan.argtypes = [ctypes.c_void_p, _pl]
         L.cospdnn_num_candidates.argtypes = []
         L.cospdnn_candidate.argtypes = [ctypes.c_int, _pl]
-        L.cospdnn_net_layer_features.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _pl, _c_f32p]
+        L.cospdnn_net_layer_features.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _pl, _c_f32p]
         L.cospdnn_tune_exhaustive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), ctypes.c_int, _c_f32p, _c_i64p, _pl]
         L.cospdnn_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_void_p)]
@@ -281,10 +281,12 @@ class Net:
             arr[l] = LayerPlan(*t)
         _check(lib().cospdnn_net_set_plan(self.h, arr), "cospdnn_net_set_plan")
 
-    def layer_features(self, layer, live_rows, cand):
+    def layer_features(self, layer, live_rows, cand, rows_prev=None):
+        """rows_prev: rows entering the layer before (default: live_rows, nothing removed)."""
         f = np.zeros(TUNE_FEATURES, dtype=np.float32)
         c = cand if isinstance(cand, LayerPlan) else LayerPlan(*cand)
-        _check(lib().cospdnn_net_layer_features(self.h, layer, int(live_rows), ctypes.byref(c), _p(f, _c_f32p)),
+        prev = int(live_rows if rows_prev is None else rows_prev)
+        _check(lib().cospdnn_net_layer_features(self.h, layer, int(live_rows), prev, ctypes.byref(c), _p(f, _c_f32p)),
                "cospdnn_net_layer_features")
         return f
 

@@ -49,7 +49,11 @@ int check_plan(const cospdnn_layer_plan& p) {
   return COSPDNN_OK;
 }
 
-void layer_features(const cospdnn_net* net, int l, int64_t live, const cospdnn_layer_plan& c, float* f) {
+// rows_prev: rows entering the layer before (== live for layer 0).  The fraction of rows
+// the scan before this layer removed decides how many of its MFMA tiles leave the fast
+// epilogue (their output positions are no longer 64 consecutive ones).
+void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_prev, const cospdnn_layer_plan& c,
+                    float* f) {
   const LayerStore& ls = *net->dl[l];
   const double rows = static_cast<double>(std::max<int64_t>(live, 0));
   const bool mfma = c.kernel == COSPDNN_PLAN_MFMA && ls.has_tiles;
@@ -64,7 +68,7 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, const cospdnn_l
   f[8] = mfma ? static_cast<float>(c.order) / 3.0f : 0.0f;
   f[9] = mfma ? static_cast<float>(c.groups_per_cu) / 4.0f : 0.0f;
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
-  f[11] = 1.0f;
+  f[11] = rows_prev > 0 ? static_cast<float>(static_cast<double>(std::max<int64_t>(rows_prev - live, 0)) / rows_prev) : 0.0f;
 }
 
 int mlp_reserve(cospdnn_mlp* m, int n) {
@@ -127,13 +131,13 @@ int cospdnn_candidate(int k, cospdnn_layer_plan* out) {
   return COSPDNN_OK;
 }
 
-int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, const cospdnn_layer_plan* cand,
-                               float* feat) {
+int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, int64_t rows_prev,
+                               const cospdnn_layer_plan* cand, float* feat) {
   if (!net || !cand || !feat) return fail(COSPDNN_ERR_ARG, "null argument");
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (layer < 0 || layer >= net->L) return fail(COSPDNN_ERR_ARG, "layer index");
   CALL_TRY(check_plan(*cand));
-  layer_features(net, layer, live_rows, *cand, feat);
+  layer_features(net, layer, live_rows, rows_prev, *cand, feat);
   return COSPDNN_OK;
 }
 
@@ -198,7 +202,7 @@ int cospdnn_tune_samples(cospdnn_net* net, const float* ms_table, const int64_t*
     for (int l = 0; l < L; ++l) {
       if (kCandidates[k].kernel == COSPDNN_PLAN_MFMA && !net->dl[l]->has_tiles) continue;
       if (n >= cap) return fail(COSPDNN_ERR_ARG, "sample capacity too small");
-      layer_features(net, l, live[l], kCandidates[k], X + static_cast<size_t>(n) * COSPDNN_TUNE_FEATURES);
+      layer_features(net, l, live[l], live[l > 0 ? l - 1 : 0], kCandidates[k], X + static_cast<size_t>(n) * COSPDNN_TUNE_FEATURES);
       y[n] = static_cast<float>(std::log(1000.0 * ms_table[static_cast<size_t>(k) * L + l] + 1.0));
       ++n;
     }
@@ -321,7 +325,8 @@ int cospdnn_tune_mlp(cospdnn_net* net, cospdnn_mlp* mlp, const int64_t* live, co
   std::vector<float> X(static_cast<size_t>(kNumCandidates) * L * COSPDNN_TUNE_FEATURES);
   for (int k = 0; k < kNumCandidates; ++k)
     for (int l = 0; l < L; ++l)
-      layer_features(net, l, live[l], kCandidates[k], X.data() + (static_cast<size_t>(k) * L + l) * COSPDNN_TUNE_FEATURES);
+      layer_features(net, l, live[l], live[l > 0 ? l - 1 : 0], kCandidates[k],
+                     X.data() + (static_cast<size_t>(k) * L + l) * COSPDNN_TUNE_FEATURES);
   std::vector<float> pred(static_cast<size_t>(kNumCandidates) * L);
   CALL_TRY(cospdnn_mlp_predict(mlp, X.data(), kNumCandidates * L, pred.data()));
   for (int l = 0; l < L; ++l) {

@@ -170,10 +170,11 @@ int cospdnn_net_set_plan(cospdnn_net* net, const cospdnn_layer_plan* plan);
 /* the candidate launch shapes the tuner compares for each layer */
 int cospdnn_num_candidates(void);
 int cospdnn_candidate(int k, cospdnn_layer_plan* out);
-/* feature vector of (layer, candidate, rows entering the layer) for the cost model */
+/* feature vector of (layer, candidate, rows entering the layer, rows entering the layer
+   before it; pass live_rows again for layer 0) for the cost model */
 #define COSPDNN_TUNE_FEATURES 12
-int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, const cospdnn_layer_plan* cand,
-                               float* feat);
+int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, int64_t rows_prev,
+                               const cospdnn_layer_plan* cand, float* feat);
 /* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer
    (median of reps runs, per-layer HIP events) and keeps, per layer, the fastest.
    ms_table [ncand][L] (may be NULL), live [L+1] rows entering each layer (may be NULL),
