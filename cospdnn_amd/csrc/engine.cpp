@@ -373,31 +373,92 @@ int host_stage(cospdnn_net* net, size_t n) {
   return COSPDNN_OK;
 }
 
-// Two round trips: the counters, then the id lists (copies into pinned memory).
-int read_pass(cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out, bool sort_flags = true) {
+// One pass to read back: its buffers, the ids side holding its categories, the output.
+struct PassRead {
+  const RunBufs* rb;
+  int cat_side;
+  bool sort_flags;
+  PassOut* out;
+};
+
+// A fixed-size int32 array read with the counters.
+struct ExtraRead {
+  const void* dev;
+  int n;
+  int32_t* host;
+};
+
+// Reads several passes (and the extra arrays, in the first round) in two round trips:
+// every counter, then every id list (async copies into pinned memory, one sync each).
+int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::vector<ExtraRead>& extras = {}) {
   const int L = net->L;
   hipStream_t st = net->eng->stream;
-  const size_t nc = static_cast<size_t>(L) + 4;  // counts[L+2], ret_count, flag_count
-  CALL_TRY(host_stage(net, nc));
+  const size_t per = static_cast<size_t>(L) + 4;  // counts[L+2], ret_count, flag_count
+  size_t n_extra = 0;
+  for (const ExtraRead& x : extras) n_extra += static_cast<size_t>(x.n);
+  CALL_TRY(host_stage(net, per * reqs.size() + n_extra));
   int32_t* h = net->hpin;
-  HIP_TRY(hipMemcpyAsync(h, rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(h + L + 2, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(h + L + 3, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  for (size_t r = 0; r < reqs.size(); ++r) {
+    const RunBufs& rb = *reqs[r].rb;
+    int32_t* hr = h + r * per;
+    HIP_TRY(hipMemcpyAsync(hr, rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hr + L + 2, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hr + L + 3, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  }
+  {
+    size_t o = per * reqs.size();
+    for (const ExtraRead& x : extras) {
+      if (x.n > 0) HIP_TRY(hipMemcpyAsync(h + o, x.dev, sizeof(int32_t) * x.n, hipMemcpyDeviceToHost, st));
+      o += static_cast<size_t>(x.n);
+    }
+  }
   HIP_TRY(hipStreamSynchronize(st));
-  out.counts.assign(h, h + L + 2);
-  const int32_t nlive = out.counts[L + 1], nret = h[L + 2], nf = h[L + 3];
-  const size_t nl = static_cast<size_t>(nlive) + nret + nf;
-  CALL_TRY(host_stage(net, nl));
+  std::vector<int32_t> nlive(reqs.size()), nret(reqs.size()), nf(reqs.size());
+  size_t total = 0;
+  for (size_t r = 0; r < reqs.size(); ++r) {
+    const int32_t* hr = h + r * per;
+    reqs[r].out->counts.assign(hr, hr + L + 2);
+    nlive[r] = hr[L + 1];
+    nret[r] = hr[L + 2];
+    nf[r] = hr[L + 3];
+    total += static_cast<size_t>(nlive[r]) + nret[r] + nf[r];
+  }
+  {
+    size_t o = per * reqs.size();
+    for (const ExtraRead& x : extras) {
+      std::copy(h + o, h + o + x.n, x.host);
+      o += static_cast<size_t>(x.n);
+    }
+  }
+  CALL_TRY(host_stage(net, total));
   h = net->hpin;
-  if (nlive > 0) HIP_TRY(hipMemcpyAsync(h, rb.id(cat_side), sizeof(int32_t) * nlive, hipMemcpyDeviceToHost, st));
-  if (nret > 0) HIP_TRY(hipMemcpyAsync(h + nlive, rb.ret_ids.p, sizeof(int32_t) * nret, hipMemcpyDeviceToHost, st));
-  if (nf > 0) HIP_TRY(hipMemcpyAsync(h + nlive + nret, rb.flag_ids.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, st));
-  if (nl > 0) HIP_TRY(hipStreamSynchronize(st));
-  out.cats.assign(h, h + nlive + nret);
-  std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
-  out.flags.assign(h + nlive + nret, h + nl);
-  if (sort_flags) std::sort(out.flags.begin(), out.flags.end());
+  size_t off = 0;
+  for (size_t r = 0; r < reqs.size(); ++r) {
+    const RunBufs& rb = *reqs[r].rb;
+    if (nlive[r] > 0)
+      HIP_TRY(hipMemcpyAsync(h + off, rb.id(reqs[r].cat_side), sizeof(int32_t) * nlive[r], hipMemcpyDeviceToHost, st));
+    off += nlive[r];
+    if (nret[r] > 0) HIP_TRY(hipMemcpyAsync(h + off, rb.ret_ids.p, sizeof(int32_t) * nret[r], hipMemcpyDeviceToHost, st));
+    off += nret[r];
+    if (nf[r] > 0) HIP_TRY(hipMemcpyAsync(h + off, rb.flag_ids.p, sizeof(int32_t) * nf[r], hipMemcpyDeviceToHost, st));
+    off += nf[r];
+  }
+  if (total > 0) HIP_TRY(hipStreamSynchronize(st));
+  off = 0;
+  for (size_t r = 0; r < reqs.size(); ++r) {
+    PassOut& out = *reqs[r].out;
+    out.cats.assign(h + off, h + off + nlive[r] + nret[r]);
+    std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
+    off += static_cast<size_t>(nlive[r]) + nret[r];
+    out.flags.assign(h + off, h + off + nf[r]);
+    if (reqs[r].sort_flags) std::sort(out.flags.begin(), out.flags.end());
+    off += nf[r];
+  }
   return COSPDNN_OK;
+}
+
+int read_pass(cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out, bool sort_flags = true) {
+  return read_passes(net, {PassRead{&rb, cat_side, sort_flags, &out}});
 }
 
 // The FP32 tier: the rows undecided at the checkpoint, layer-major in FP32 (MFMA FP32 on
@@ -709,30 +770,28 @@ int run_finish_dev_guard(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& 
                          cospdnn_result* res) {
   const int L = net->L;
   const int64_t S = net->S;
+  // the three passes and the guard counters in two round trips
+  const int side = (L & 1) ^ 1;  // the tiers run every layer
+  PassOut p1, p2, p3;
   int32_t gsc[3] = {0, 0, 0};
-  HIP_TRY(hipMemcpy(gsc, net->gs.p, sizeof(gsc), hipMemcpyDeviceToHost));
+  std::vector<int32_t> gids(net->gcap64);  // sample ids of the FP64 tier (the first gsc[1])
+  CALL_TRY(read_passes(net,
+                       {PassRead{&net->main, cat_side, false, &p1},  // flags in list order
+                        PassRead{&net->rec, side, true, &p2}, PassRead{&net->rep, side, true, &p3}},
+                       {ExtraRead{net->gs.p, 3, gsc}, ExtraRead{net->gids.p, static_cast<int>(net->gcap64), gids.data()}}));
   if (gsc[2] != 0) return run_finish(net, o, cfg, cat_side, res);
-  PassOut p1;
-  CALL_TRY(read_pass(net, net->main, cat_side, p1, false));  // flags in list order
   std::vector<int64_t> live(L + 1, -1);
   for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
   std::vector<char> alive(S, 0);
   for (int32_t id : p1.cats) alive[id] = 1;
   const std::vector<char> first = alive;
   const int32_t n32 = gsc[0], n64 = gsc[1];
-  const int side = (L & 1) ^ 1;  // the tiers run every layer
   // FP32 tier: its pass-local ids are positions in the checkpoint's list
-  PassOut p2;
-  CALL_TRY(read_pass(net, net->rec, side, p2));
   for (int32_t i = 0; i < n32; ++i) alive[p1.flags[i]] = 0;
   for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
   // FP64 tier: its pass-local ids are positions in gids
-  std::vector<int32_t> ids64(n64);
-  if (n64 > 0) HIP_TRY(hipMemcpy(ids64.data(), net->gids.p, sizeof(int32_t) * n64, hipMemcpyDeviceToHost));
-  PassOut p3;
-  CALL_TRY(read_pass(net, net->rep, side, p3));
-  for (int32_t id : ids64) alive[id] = 0;
-  for (int32_t local : p3.cats) alive[ids64[local]] = 1;
+  for (int32_t i = 0; i < n64; ++i) alive[gids[i]] = 0;
+  for (int32_t local : p3.cats) alive[gids[local]] = 1;
   // rows listed after the checkpoint: straight to the FP64 tier
   std::vector<int32_t> late(p1.flags.begin() + n32, p1.flags.end());
   if (!late.empty()) {
