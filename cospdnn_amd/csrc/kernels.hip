@@ -412,7 +412,6 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   }
   const bool k1 = w.one_chunk != 0;
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
-  const cint_ptr_t out_idx = (cint_ptr_t)w.out_idx;
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
   const int64_t ldx = x_ld(a), ldy = y_ld(a);
@@ -612,7 +611,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
           const int32_t cbase = cur_c * 32;
 #pragma unroll 8
           for (int o = 0; o < 32; ++o) {
-            const int out = out_idx[cbase + o];
+            const int out = cbase + o < w.n_out ? cbase + o : -1;  // slot = row
             const float v = stage[o * kStageLd + lane];
             if (out >= 0) {
               fl |= v > 0.0f ? 1u : 0u;
