@@ -42,3 +42,16 @@ def test_host_helpers_reject_bad_arguments():
 
 def test_product_library_is_in_tree():
     assert os.path.dirname(cd.LIB_PATH) == os.path.join(ROOT, "cospdnn_amd")
+
+
+def test_options_struct_matches_the_header():
+    """The ctypes mirror of cospdnn_opts lines up with the C struct: the defaults the library
+    writes land in the right fields, the last one included."""
+    o = cd.default_opts()
+    assert o.precision == cd.FP16
+    assert o.guard_layer == 12 and o.guard_layer32 == 20
+    assert o.use_mfma == 1 and abs(o.max_tile_waste - 4.0) < 1e-6
+    assert o.dump_layer == -1
+    assert o.spmm_order == 1 and o.mfma_groups_per_cu == 4
+    assert o.batch_concurrent == 2 and o.guard_tail == 8
+    assert o.guard_concurrent == 1
