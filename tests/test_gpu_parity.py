@@ -255,6 +255,25 @@ def test_concurrent_guard_overflow_falls_back(engine):
     net.close()
 
 
+def test_graph_recaptured_after_tier_buffers_grow(engine):
+    """A captured graph holds the addresses of the guard tiers' buffers: a run that grows
+    them (one-after-the-other tiers over nearly every row) must make the next graph replay
+    recapture, not replay stale addresses."""
+    N, L, S, bias, seed = 1024, 24, 20000, -0.3, 44
+    say("graph after tier buffers grow")
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, cd.GEN_RANDOM, seed), 0, S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed, 64.0)
+    g = cd.default_opts(precision=cd.FP16, use_graph=1)
+    assert np.array_equal(cd.run([net], g)[0].categories, want)
+    big = cd.run([net], cd.default_opts(precision=cd.FP16, guard_layer=1, guard_concurrent=0))[0]
+    assert big.n_flagged > S // 4  # more rows than the concurrent tiers' buffers hold
+    assert np.array_equal(big.categories, want)
+    for _ in range(2):
+        assert np.array_equal(cd.run([net], g)[0].categories, want)
+    net.close()
+
+
 def test_layer_timing_and_counters(engine):
     N, L, S, bias, seed = 4096, 20, 4000, -0.35, 3
     net = gen_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
