@@ -3,11 +3,13 @@
 // One network = per-layer weight images + an FP16 copy of the input (the
 // "store") in HBM.  A forward pass enqueues, per layer l,
 //   scan_kernel(l)   live flags of layer l's input -> qmap, ids, counts[l+1]
-//   spmm(l)          MFMA (FP16) or SIMT (FP16/FP32) SpMM with fused epilogue
+//   spmm(l)          MFMA (FP16) or SIMT (FP16/FP32/FP64) SpMM with fused epilogue
 // and a final scan that yields the categories.  Everything stays on the
 // device: no host round trip per layer (the reference syncs and scans on the
-// host each layer, cospdnn.cu:240-276).  The FP16 path then recomputes the
-// rows that were still undecided at the guard layer with the FP32 path.
+// host each layer, cospdnn.cu:240-276).  The FP16 path recomputes the rows
+// still undecided at its checkpoint in FP32, and those the FP32 tier leaves
+// undecided in FP64: on a side stream forked after the checkpoint scan (one
+// network, or each chain of a batch), or after the main pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
