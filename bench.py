@@ -128,10 +128,9 @@ def main():
     net = cd.Net(eng, spec.N, spec.L, spec.S, spec.bias)
     net.generate_layers(spec.kind, spec.seed, K=spec.K, wval=spec.wval)
     net.generate_input(spec.seed)
-    # tile images up to waste 64 (random layers up to N=16384 may then take the MFMA kernel
-    # when the tuner finds it faster); beyond that the SIMT image only (a random N=65536
-    # layer's tile image would be 128 MB)
-    net.upload(cd.default_opts(max_tile_waste=64.0 if spec.N <= 16384 else 4.0))
+    # tile images up to waste 64 (the default): random layers may then take the MFMA kernel
+    # when the tuner finds it faster (c4: 128 MB of tile image per layer, 61 GB in all)
+    net.upload(cd.default_opts())
     log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
 
     # Autotuning (untimed): exhaustive per-layer search over the launch candidates, then

@@ -191,6 +191,7 @@ struct RunCfg {
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
+constexpr double kTileBudget = 0.4;  // MFMA tile images: at most this share of the HBM per engine
 
 int grid_rows(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   const int64_t items = (S_run + kFewRowsTile - 1) / kFewRowsTile * static_cast<int64_t>(nblk);
@@ -1114,7 +1115,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->guard_band = 0.0f;
   o->guard_band_from = 1 << 30;
   o->use_mfma = 1;
-  o->max_tile_waste = 4.0f;
+  o->max_tile_waste = 64.0f;
   o->layer_timing = 0;
   o->use_graph = 0;
   o->dump_layer = -1;
@@ -1199,6 +1200,7 @@ void cospdnn_net_destroy(cospdnn_net* net) {
   // a batch graph that captured this network's buffers must not be replayed again
   BatchGraph& bg = net->eng->batch_graph;
   if (std::find(bg.nets.begin(), bg.nets.end(), net) != bg.nets.end()) bg.reset();
+  net->eng->tile_bytes -= net->tile_bytes;
   delete net;
 }
 
@@ -1342,8 +1344,26 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
           stable[l - l0] = all32_fixed_point(*csc, net->N, net->bias) ? 1 : 0;
         });
       for (auto& t : th) t.join();
-      for (int l = l0; l < l1; ++l)
+      for (int l = l0; l < l1; ++l) {
         if (!ok[l - l0]) return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
+        // tile images of random layers are ~32x their SIMT image: past kTileBudget of the
+        // HBM (all of this engine's networks together) layers keep the SIMT image only
+        BlockedLayer& b = bl[l - l0];
+        if (b.has_tiles) {
+          const int64_t tb = static_cast<int64_t>(b.wt.size()) * 2 + static_cast<int64_t>(b.wt32.size()) * 4 +
+                             static_cast<int64_t>(b.gidx.size()) * 4;
+          if (static_cast<double>(net->eng->tile_bytes + tb) > kTileBudget * static_cast<double>(net->eng->hbm_bytes)) {
+            b.has_tiles = b.has_tiles32 = false;
+            b.wt.clear();
+            b.wt32.clear();
+            b.gidx.clear();
+            b.kptr.clear();
+          } else {
+            net->eng->tile_bytes += tb;
+            net->tile_bytes += tb;
+          }
+        }
+      }
       // inputs of layer l through the row order of layer l-1 (rows of the input store: neurons)
       th.clear();
       for (int l = std::max(l0, 1); l < l1; ++l)

@@ -137,6 +137,7 @@ struct cospdnn_engine {
   hipStream_t stream = nullptr;
   int cu_count = 0;
   int64_t hbm_bytes = 0;
+  int64_t tile_bytes = 0;  // MFMA tile images of this engine's networks (bounded at upload)
   int bpc_mfma = 1, bpc_mfma_resident = 1, bpc_simt16 = 1, bpc_simt1632 = 1, bpc_simt32 = 1, bpc_m32h = 1, bpc_m32f = 1,
       bpc_simt1664 = 1, bpc_simt64 = 1, bpc_rows = 1;
   // concurrent runs of several networks
@@ -173,6 +174,7 @@ struct cospdnn_net {
   int words = 0;
   RunBufs main, rec;
   int64_t nnz_total = 0;
+  int64_t tile_bytes = 0;  // this network's MFMA tile images (counted in the engine's too)
   bool weights_fp16_exact = true;
   GraphCache graph;
   std::vector<cospdnn_layer_plan> plan;  // per-layer launch plan of the FP16 pass (empty: options decide)

@@ -61,7 +61,9 @@ typedef struct {
   int guard_band_from;   /* ... at any layer >= guard_band_from (default: 1 << 30 = never) */
   int use_mfma;          /* FP16: 1 = MFMA kernel where a tile image exists, 0 = SIMT only */
   float max_tile_waste;  /* build the MFMA tile image of a layer only when its MFMA work is at
-                            most this multiple of the useful work (default 4.0) */
+                            most this multiple of the useful work (default 64: random layers
+                            too, ~32x; the images of an engine's networks take at most 40%
+                            of the HBM, later layers keep the SIMT image only) */
   int layer_timing;      /* 1: time every layer's SpMM with events (reference-style sum) */
   int use_graph;         /* 1: capture the layer loop of each network in a hipGraph and replay */
   int dump_layer;        /* >= 0: copy the activations after this layer into the result */

@@ -50,7 +50,7 @@ def test_options_struct_matches_the_header():
     o = cd.default_opts()
     assert o.precision == cd.FP16
     assert o.guard_layer == 12 and o.guard_layer32 == 20
-    assert o.use_mfma == 1 and abs(o.max_tile_waste - 4.0) < 1e-6
+    assert o.use_mfma == 1 and abs(o.max_tile_waste - 64.0) < 1e-6
     assert o.dump_layer == -1
     assert o.spmm_order == 1 and o.mfma_groups_per_cu == 4
     assert o.batch_concurrent == 2 and o.guard_tail == 8
