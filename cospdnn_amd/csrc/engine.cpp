@@ -391,72 +391,53 @@ struct ExtraRead {
   int32_t* host;
 };
 
-// Reads several passes (and the extra arrays, in the first round) in two round trips:
-// every counter, then every id list (async copies into pinned memory, one sync each).
+// Reads several passes and the extra arrays with one copy kernel into pinned host memory
+// and one synchronisation: per pass its counts, then its live ids, retired ids and flagged
+// ids, each list as long as the device-side count says (up to the buffer's capacity).
 int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::vector<ExtraRead>& extras = {}) {
   const int L = net->L;
   hipStream_t st = net->eng->stream;
-  const size_t per = static_cast<size_t>(L) + 4;  // counts[L+2], ret_count, flag_count
-  size_t n_extra = 0;
-  for (const ExtraRead& x : extras) n_extra += static_cast<size_t>(x.n);
-  CALL_TRY(host_stage(net, per * reqs.size() + n_extra));
+  CopyJobs jobs;
+  int64_t off = 0;
+  auto add = [&](const int32_t* src, const int32_t* len, int64_t cap, bool counted) -> int64_t {
+    if (jobs.n >= kMaxCopyJobs) return -1;
+    const int64_t at = off;
+    // counted lists: [count][items...]
+    jobs.job[jobs.n++] = CopyJob{src, len, static_cast<int32_t>(cap), counted ? at + 1 : at, counted ? at : -1};
+    off += cap + (counted ? 1 : 0);
+    return at;
+  };
+  struct Where { int64_t counts, cats, rets, flags; };
+  std::vector<Where> where(reqs.size());
+  for (size_t r = 0; r < reqs.size(); ++r) {
+    const RunBufs& rb = *reqs[r].rb;
+    const int32_t* counts = rb.counts.as<int32_t>();
+    where[r].counts = add(counts, nullptr, L + 2, false);
+    where[r].cats = add(rb.id(reqs[r].cat_side), counts + L + 1, rb.ld, true);
+    where[r].rets = add(rb.ret_ids.as<int32_t>(), rb.ret_count.as<int32_t>(), rb.ld, true);
+    where[r].flags = add(rb.flag_ids.as<int32_t>(), rb.flag_count.as<int32_t>(), rb.ld, true);
+  }
+  std::vector<int64_t> xat(extras.size());
+  for (size_t k = 0; k < extras.size(); ++k) xat[k] = add(static_cast<const int32_t*>(extras[k].dev), nullptr, extras[k].n, false);
+  for (const Where& w : where)
+    if (w.counts < 0 || w.cats < 0 || w.rets < 0 || w.flags < 0) return fail(COSPDNN_ERR_ARG, "too many result arrays");
+  for (int64_t x : xat)
+    if (x < 0) return fail(COSPDNN_ERR_ARG, "too many result arrays");
+  CALL_TRY(host_stage(net, static_cast<size_t>(off)));
   int32_t* h = net->hpin;
-  for (size_t r = 0; r < reqs.size(); ++r) {
-    const RunBufs& rb = *reqs[r].rb;
-    int32_t* hr = h + r * per;
-    HIP_TRY(hipMemcpyAsync(hr, rb.counts.p, sizeof(int32_t) * (L + 2), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hr + L + 2, rb.ret_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hr + L + 3, rb.flag_count.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  }
-  {
-    size_t o = per * reqs.size();
-    for (const ExtraRead& x : extras) {
-      if (x.n > 0) HIP_TRY(hipMemcpyAsync(h + o, x.dev, sizeof(int32_t) * x.n, hipMemcpyDeviceToHost, st));
-      o += static_cast<size_t>(x.n);
-    }
-  }
+  HIP_TRY(launch_copy_jobs(jobs, h, st));
   HIP_TRY(hipStreamSynchronize(st));
-  std::vector<int32_t> nlive(reqs.size()), nret(reqs.size()), nf(reqs.size());
-  size_t total = 0;
-  for (size_t r = 0; r < reqs.size(); ++r) {
-    const int32_t* hr = h + r * per;
-    reqs[r].out->counts.assign(hr, hr + L + 2);
-    nlive[r] = hr[L + 1];
-    nret[r] = hr[L + 2];
-    nf[r] = hr[L + 3];
-    total += static_cast<size_t>(nlive[r]) + nret[r] + nf[r];
-  }
-  {
-    size_t o = per * reqs.size();
-    for (const ExtraRead& x : extras) {
-      std::copy(h + o, h + o + x.n, x.host);
-      o += static_cast<size_t>(x.n);
-    }
-  }
-  CALL_TRY(host_stage(net, total));
-  h = net->hpin;
-  size_t off = 0;
-  for (size_t r = 0; r < reqs.size(); ++r) {
-    const RunBufs& rb = *reqs[r].rb;
-    if (nlive[r] > 0)
-      HIP_TRY(hipMemcpyAsync(h + off, rb.id(reqs[r].cat_side), sizeof(int32_t) * nlive[r], hipMemcpyDeviceToHost, st));
-    off += nlive[r];
-    if (nret[r] > 0) HIP_TRY(hipMemcpyAsync(h + off, rb.ret_ids.p, sizeof(int32_t) * nret[r], hipMemcpyDeviceToHost, st));
-    off += nret[r];
-    if (nf[r] > 0) HIP_TRY(hipMemcpyAsync(h + off, rb.flag_ids.p, sizeof(int32_t) * nf[r], hipMemcpyDeviceToHost, st));
-    off += nf[r];
-  }
-  if (total > 0) HIP_TRY(hipStreamSynchronize(st));
-  off = 0;
   for (size_t r = 0; r < reqs.size(); ++r) {
     PassOut& out = *reqs[r].out;
-    out.cats.assign(h + off, h + off + nlive[r] + nret[r]);
+    const Where& w = where[r];
+    out.counts.assign(h + w.counts, h + w.counts + L + 2);
+    out.cats.assign(h + w.cats + 1, h + w.cats + 1 + h[w.cats]);
+    out.cats.insert(out.cats.end(), h + w.rets + 1, h + w.rets + 1 + h[w.rets]);
     std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
-    off += static_cast<size_t>(nlive[r]) + nret[r];
-    out.flags.assign(h + off, h + off + nf[r]);
+    out.flags.assign(h + w.flags + 1, h + w.flags + 1 + h[w.flags]);
     if (reqs[r].sort_flags) std::sort(out.flags.begin(), out.flags.end());
-    off += nf[r];
   }
+  for (size_t k = 0; k < extras.size(); ++k) std::copy(h + xat[k], h + xat[k] + extras[k].n, extras[k].host);
   return COSPDNN_OK;
 }
 

@@ -69,6 +69,14 @@ __global__ void iota_i32_kernel(int32_t* p, int64_t n) {
 }
 __global__ void set_i32_kernel(int32_t* p, int32_t v) { *p = v; }
 
+// One work-group per copy job (see launch_copy_jobs).
+__global__ __launch_bounds__(256) void copy_jobs_kernel(CopyJobs jobs, int32_t* __restrict__ dst) {
+  const CopyJob& jb = jobs.job[blockIdx.x];
+  const int32_t n = jb.len ? min(*jb.len, jb.cap) : jb.cap;
+  if (threadIdx.x == 0 && jb.count_at >= 0) dst[jb.count_at] = n;
+  for (int32_t i = threadIdx.x; i < n; i += blockDim.x) dst[jb.off + i] = jb.src[i];
+}
+
 // *dst = min(*src, cap); *overflow = 1 when *src > cap (one thread)
 __global__ void clamp_count_kernel(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int32_t cap,
                                    int32_t* __restrict__ overflow) {
@@ -936,6 +944,11 @@ hipError_t launch_compose_ids(const int32_t* outer, const int32_t* inner, const 
                               int32_t* dst, int32_t* dst_count, int32_t* overflow, hipStream_t st) {
   const unsigned grid = static_cast<unsigned>(std::max<int32_t>(1, (cap + 255) / 256));
   hipLaunchKernelGGL(compose_ids_kernel, dim3(grid), dim3(256), 0, st, outer, inner, count, cap, dst, dst_count, overflow);
+  return hipGetLastError();
+}
+hipError_t launch_copy_jobs(const CopyJobs& jobs, int32_t* dst, hipStream_t st) {
+  if (jobs.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_jobs_kernel, dim3(jobs.n), dim3(256), 0, st, jobs, dst);
   return hipGetLastError();
 }
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t st) {

@@ -130,6 +130,23 @@ hipError_t launch_pack_rows(const _Float16* store, int64_t ld, int N, int64_t S,
 hipError_t launch_gather_packed(const uint32_t* packed, int words, const int32_t* ids, const int32_t* count,
                                 int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
 
+// Result read-back in one launch: job j copies min(*len, cap) (len == nullptr: cap) int32
+// from src to dst + off and stores that count at dst[count_at] (count_at < 0: nowhere).
+// dst is pinned host memory; the host reads it after one synchronisation.
+constexpr int kMaxCopyJobs = 16;
+struct CopyJob {
+  const int32_t* src;
+  const int32_t* len;
+  int32_t cap;
+  int64_t off;
+  int64_t count_at;
+};
+struct CopyJobs {
+  CopyJob job[kMaxCopyJobs];
+  int n = 0;
+};
+hipError_t launch_copy_jobs(const CopyJobs& jobs, int32_t* dst, hipStream_t st);
+
 // Kernel attributes set once per process (dynamic LDS limits).
 hipError_t configure_kernels();
 
