@@ -49,9 +49,11 @@ int check_plan(const cospdnn_layer_plan& p) {
   return COSPDNN_OK;
 }
 
-// rows_prev: rows entering the layer before (== live for layer 0).  The fraction of rows
-// the scan before this layer removed decides how many of its MFMA tiles leave the fast
-// epilogue (their output positions are no longer 64 consecutive ones).
+// rows_prev: rows entering the layer before (== live for layer 0).  Every MFMA tile after
+// the first row the scan before this layer removed leaves the fast epilogue (its output
+// positions are shifted off 64 consecutive ones).  With k removals at uniform positions
+// the expected fraction of positions after the first is k / (k + 1), so that is f[11]:
+// one removal already moves about half the tiles, a handful moves nearly all of them.
 void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_prev, const cospdnn_layer_plan& c,
                     float* f) {
   const LayerStore& ls = *net->dl[l];
@@ -68,7 +70,8 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[8] = mfma ? static_cast<float>(c.order) / 3.0f : 0.0f;
   f[9] = mfma ? static_cast<float>(c.groups_per_cu) / 4.0f : 0.0f;
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
-  f[11] = rows_prev > 0 ? static_cast<float>(static_cast<double>(std::max<int64_t>(rows_prev - live, 0)) / rows_prev) : 0.0f;
+  const double removed = static_cast<double>(std::max<int64_t>(rows_prev - live, 0));
+  f[11] = static_cast<float>(removed / (removed + 1.0));
 }
 
 int mlp_reserve(cospdnn_mlp* m, int n) {
