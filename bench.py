@@ -133,35 +133,51 @@ def main():
     net.upload(cd.default_opts())
     log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
 
-    # Autotuning (untimed): exhaustive per-layer search over the launch candidates, then
-    # the cost-model MLP (12-64-64-1, trained by the library's HIP kernels on this run's
-    # per-layer timings) picks the plan the timed runs use.
+    # Autotuning (untimed).  1: exhaustive per-layer search over the launch candidates with
+    # compaction at every scan, and the cost-model MLP (12-64-64-1, trained by the library's
+    # HIP kernels on those per-layer timings) picks a plan.  2: the compaction threshold
+    # (keep_holes_ppm) on whole forwards with that plan.  3: the launch plan again under that
+    # threshold (a kept layout changes which layers see shifted tiles), the MLP retrained on
+    # both tables; its plan is the one the timed runs use.
     tuning = None
+    keep_ppm = 0
     if args.tune:
         t0 = time.perf_counter()
-        ex_plan, table, live = cd.tune_exhaustive(net, reps=2)
-        X, y = cd.tune_samples(net, table, live)
-        mlp = cd.Mlp(eng, cd.TUNE_FEATURES, 64, seed=1)
-        loss = mlp.train(X, y, steps=3000, lr=2e-3)
-        mlp_plan, _ = mlp.plan(net, live)
         cands = cd.candidates()
+        _, table0, pos0 = cd.tune_exhaustive(net, reps=2)
+        X0, y0 = cd.tune_samples(net, table0, pos0)
+        mlp = cd.Mlp(eng, cd.TUNE_FEATURES, 64, seed=1)
+        mlp.train(X0, y0, steps=3000, lr=2e-3)
+        net.set_plan(mlp.plan(net, pos0)[0])
+        keep_ppm, keep_ms = cd.tune_keep(net, cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0))
+        ex_plan, table, pos = cd.tune_exhaustive(net, cd.default_opts(precision=cd.FP16, keep_holes_ppm=keep_ppm), reps=2)
+        X1, y1 = cd.tune_samples(net, table, pos)
+        mlp = cd.Mlp(eng, cd.TUNE_FEATURES, 64, seed=1)
+        loss = mlp.train(np.concatenate([X0, X1]), np.concatenate([y0, y1]), steps=3000, lr=2e-3)
+        mlp_plan, _ = mlp.plan(net, pos)
         pick = lambda plan: float(sum(table[cands.index(p), l] for l, p in enumerate(plan)))  # noqa: E731
         default = [cands.index((cd.PLAN_MFMA, 1, 4))] * spec.L
         tuning = {
             "candidates": [list(c) for c in cands],
-            "training_samples": int(len(y)), "training_loss_ln_time": loss,
+            "keep_holes_ppm": keep_ppm,
+            "forward_ms_per_keep_ppm": {str(c): float(m) for c, m in zip(cd.KEEP_CANDIDATES, keep_ms)},
+            "training_samples": int(len(y0) + len(y1)), "training_loss_ln_time": loss,
             "spmm_ms_exhaustive": pick(ex_plan), "spmm_ms_mlp": pick(mlp_plan),
             "spmm_ms_default": float(sum(table[k, l] for l, k in enumerate(default))),
+            "spmm_ms_default_compacting": float(sum(table0[k, l] for l, k in enumerate(default))),
             "tuning_s": time.perf_counter() - t0,
         }
         tuning["mlp_regret"] = tuning["spmm_ms_mlp"] / tuning["spmm_ms_exhaustive"] - 1.0
         net.set_plan(mlp_plan)
+        log(f"[bench] compaction threshold {keep_ppm} ppm (forward ms per candidate "
+            f"{ {k: round(v, 3) for k, v in tuning['forward_ms_per_keep_ppm'].items()} })")
         log(f"[bench] tuning: exhaustive {tuning['spmm_ms_exhaustive']:.2f} ms, MLP plan {tuning['spmm_ms_mlp']:.2f} ms "
-            f"(regret {tuning['mlp_regret']:.3f}), default {tuning['spmm_ms_default']:.2f} ms, {tuning['tuning_s']:.1f} s")
+            f"(regret {tuning['mlp_regret']:.3f}), default {tuning['spmm_ms_default']:.2f} ms "
+            f"(always compacting {tuning['spmm_ms_default_compacting']:.2f} ms), {tuning['tuning_s']:.1f} s")
 
     # FP16-MFMA path with per-layer HIP events on the engine's stream: times the SpMM kernel
     # of every layer inside the timed steps (roofline of the dominant kernel).
-    o16 = cd.default_opts(precision=cd.FP16, layer_timing=1)
+    o16 = cd.default_opts(precision=cd.FP16, layer_timing=1, keep_holes_ppm=keep_ppm)
     outs16, wall16 = timed_runs(cd, net, o16, args.steps, args.warmup)
     ms_timed = wall16 * 1e3 / args.steps
     edges = outs16[0].edges
@@ -177,7 +193,7 @@ def main():
 
     # headline: the same forward without per-layer events, the layer loop replayed from a
     # hipGraph (use --graph 0 for eager launches)
-    og = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0)
+    og = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, keep_holes_ppm=keep_ppm)
     outs_g, wall_g = timed_runs(cd, net, og, args.steps, args.warmup)
     ms_step = wall_g * 1e3 / args.steps
     value = edges / (ms_step * 1e-3)

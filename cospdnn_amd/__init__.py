@@ -33,6 +33,8 @@ from ._lib import (  # noqa: F401
     ref_bcsc_to_csc,
     run,
     tune_exhaustive,
+    tune_keep,
+    KEEP_CANDIDATES,
     tune_samples,
     version,
 )

@@ -43,6 +43,7 @@ class Opts(ctypes.Structure):
         ("batch_concurrent", ctypes.c_int),
         ("guard_tail", ctypes.c_int),
         ("guard_concurrent", ctypes.c_int),
+        ("keep_holes_ppm", ctypes.c_int),
     ]
 
 
@@ -66,6 +67,7 @@ class Result(ctypes.Structure):
         ("edges_per_sec", ctypes.c_double),
         ("alg_bytes", ctypes.c_double),
         ("mfma_layers", ctypes.c_int64),
+        ("positions", _c_i64p),
     ]
 
 
@@ -90,7 +92,7 @@ EXPORTS = [
     "cospdnn_generate_layer", "cospdnn_generate_input_rows", "cospdnn_layer_stats", "cospdnn_layer_tiles_decode",
     "cospdnn_ref_bcsc_to_csc",
     "cospdnn_net_set_plan", "cospdnn_num_candidates", "cospdnn_candidate", "cospdnn_net_layer_features",
-    "cospdnn_tune_exhaustive", "cospdnn_mlp_create", "cospdnn_mlp_destroy", "cospdnn_mlp_num_params",
+    "cospdnn_tune_exhaustive", "cospdnn_tune_keep", "cospdnn_mlp_create", "cospdnn_mlp_destroy", "cospdnn_mlp_num_params",
     "cospdnn_mlp_get_params", "cospdnn_mlp_set_params", "cospdnn_mlp_loss_grad", "cospdnn_mlp_train",
     "cospdnn_mlp_predict", "cospdnn_tune_samples", "cospdnn_tune_mlp",
 ]
@@ -160,6 +162,8 @@ def lib():
         L.cospdnn_candidate.argtypes = [ctypes.c_int, _pl]
         L.cospdnn_net_layer_features.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _pl, _c_f32p]
         L.cospdnn_tune_exhaustive.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), ctypes.c_int, _c_f32p, _c_i64p, _pl]
+        L.cospdnn_tune_keep.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), _c_i32p, ctypes.c_int, ctypes.c_int,
+                                        _c_f32p, _c_i32p]
         L.cospdnn_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_void_p)]
         L.cospdnn_mlp_destroy.argtypes = [ctypes.c_void_p]
@@ -319,6 +323,8 @@ def run(nets, opts=None, dump_cap_rows=0):
         lms = np.zeros(max(net.L, 1), dtype=np.float32)
         res[i].categories = _p(cats, _c_i64p)
         res[i].live = _p(live, _c_i64p)
+        pos = np.zeros(net.L + 1, dtype=np.int64)
+        res[i].positions = _p(pos, _c_i64p)
         res[i].layer_ms = _p(lms, _c_f32p)
         dv = di = None
         if dump_cap_rows > 0:
@@ -327,15 +333,16 @@ def run(nets, opts=None, dump_cap_rows=0):
             res[i].dump_vals = _p(dv, _c_f32p)
             res[i].dump_ids = _p(di, _c_i64p)
             res[i].dump_cap_rows = dump_cap_rows
-        keep.append((cats, live, lms, dv, di))
+        keep.append((cats, live, lms, dv, di, pos))
     _check(lib().cospdnn_run(handles, n, ctypes.byref(o), res), "cospdnn_run")
     outs = []
     for i in range(n):
-        cats, live, lms, dv, di = keep[i]
+        cats, live, lms, dv, di, pos = keep[i]
         r = res[i]
         out = RunOutput()
         out.categories = cats[: r.n_categories].copy()
         out.live = live.copy()
+        out.positions = pos.copy()
         out.layer_ms = lms[: nets[i].L].copy()
         for f in ("ms_total", "ms_spmm", "ms_recompute", "n_flagged", "n_replayed", "n_changed", "edges", "edges_per_sec",
                   "alg_bytes", "mfma_layers", "dump_rows"):
@@ -422,7 +429,7 @@ def candidates():
 
 def tune_exhaustive(net, opts=None, reps=3):
     """Per-layer exhaustive tuning of the FP16 pass: returns (plan [L tuples], ms table
-    [ncand][L], live [L+1])."""
+    [ncand][L], row positions entering each layer [L+1])."""
     o = opts if opts is not None else default_opts()
     nc = lib().cospdnn_num_candidates()
     table = np.zeros((nc, net.L), dtype=np.float32)
@@ -431,6 +438,20 @@ def tune_exhaustive(net, opts=None, reps=3):
     _check(lib().cospdnn_tune_exhaustive(net.h, ctypes.byref(o), reps, _p(table, _c_f32p), _p(live, _c_i64p), plan),
            "cospdnn_tune_exhaustive")
     return [plan[l].as_tuple() for l in range(net.L)], table, live
+
+
+KEEP_CANDIDATES = (0, 3000, 10000, 30000, 100000, 300000, 1000000)
+
+
+def tune_keep(net, opts=None, candidates=KEEP_CANDIDATES, reps=3):
+    """Compaction threshold: returns (best keep_holes_ppm, median forward ms per candidate)."""
+    o = opts if opts is not None else default_opts()
+    cand = np.ascontiguousarray(candidates, dtype=np.int32)
+    ms = np.zeros(len(cand), dtype=np.float32)
+    best = ctypes.c_int32(0)
+    _check(lib().cospdnn_tune_keep(net.h, ctypes.byref(o), _p(cand, _c_i32p), len(cand), reps, _p(ms, _c_f32p),
+                                   ctypes.byref(best)), "cospdnn_tune_keep")
+    return int(best.value), ms
 
 
 def tune_samples(net, table, live):

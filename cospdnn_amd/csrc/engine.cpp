@@ -72,7 +72,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     CALL_TRY(dalloc(rb.flags, static_cast<size_t>(6 * rb.ld)));
     CALL_TRY(dalloc(rb.ids, sizeof(int32_t) * 2 * rb.ld));
     CALL_TRY(dalloc(rb.qmap, sizeof(int32_t) * rb.ld));
-    CALL_TRY(dalloc(rb.counts, sizeof(int32_t) * (L + 2)));
+    CALL_TRY(dalloc(rb.counts, sizeof(int32_t) * 2 * (L + 2)));  // row counts, then kept-row counts
     CALL_TRY(dalloc(rb.flag_ids, sizeof(int32_t) * rb.ld));
     CALL_TRY(dalloc(rb.flag_count, sizeof(int32_t)));
     CALL_TRY(dalloc(rb.flag_mark, static_cast<size_t>(rb.ld)));
@@ -188,6 +188,8 @@ struct RunCfg {
   int tail = 0;  // guarded passes: rows that die in the last `tail` layers are flagged too
   int stop_after_scan = -1;  // >= 0: the pass ends with the scan of this layer (no rows can be left)
   int grid_cap = 0;          // > 0: at most this many work-groups per SpMM launch
+  int keep_ppm = 0;          // FP16 pass: scans that only drop dead rows keep the layout when
+                             // at most this many ppm of the rows are removed (0: never)
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -226,7 +228,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   HIP_TRY(launch_fill_u8(rb.f(0, 2), 0, S_run, st));
   if (L == 0) HIP_TRY(launch_input_alive(input, ld_in, net->N, S_run, rb.f(0, 0), st));
   HIP_TRY(launch_iota_i32(rb.id(0), S_run, st));
-  HIP_TRY(launch_fill_i32(rb.counts.as<int32_t>() + 1, 0, L + 1, st));  // targets of the scans' atomics
+  HIP_TRY(launch_fill_i32(rb.counts.as<int32_t>() + 1, 0, 2 * L + 3, st));  // row counts and kept-row counts
   if (hooks && hooks->rows_dev)
     HIP_TRY(launch_clamp_count(hooks->rows_dev, rb.counts.as<int32_t>(), static_cast<int32_t>(S_run), hooks->overflow, st));
   else
@@ -268,6 +270,13 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.flag_all_alive = (l == D && !sat_from[l]) ? 1 : 0;
     // scan l sees the deaths of layer l-1
     sa.flag_dead = (D >= 0 && cfg.tail > 0 && l >= 1 && l - 1 >= L - cfg.tail) ? 1 : 0;
+    sa.kept = rb.counts.as<int32_t>() + (L + 2);
+    // A kept layout only where the scan drops nothing but dead rows (all-zero, and with a
+    // bias <= 0 they stay so), and never in front of the tail: the first scan that flags
+    // deaths must see no holes.
+    const bool keep_ok = fp16 && cfg.keep_ppm > 0 && l < L && !sa.flag_undecided && !sa.retire_sat &&
+                         cfg.band <= 0.0f && net->bias <= 0.0f && !(D >= 0 && cfg.tail > 0 && l >= L - cfg.tail);
+    sa.keep_ppm = keep_ok ? cfg.keep_ppm : 0;
     HIP_TRY(launch_scan(sa, S_run, st));
     if (hooks && hooks->at_fork && l == hooks->fork_layer) CALL_TRY(hooks->at_fork());
     if (l == L || l == cfg.stop_after_scan) break;
@@ -359,7 +368,7 @@ int gather_input_rows(cospdnn_net* net, const int32_t* ids, const int32_t* count
 
 // Host view of a finished pass (call after the stream is synchronised).
 struct PassOut {
-  std::vector<int32_t> counts;  // L+2
+  std::vector<int32_t> counts;  // 2(L+2): row counts (holes included), then kept-row counts
   std::vector<int32_t> cats;    // ids (pass-local) alive after the last layer
   std::vector<int32_t> flags;   // ids (pass-local) flagged at the checkpoint
 };
@@ -392,8 +401,9 @@ struct ExtraRead {
 };
 
 // Reads several passes and the extra arrays with one copy kernel into pinned host memory
-// and one synchronisation: per pass its counts, then its live ids, retired ids and flagged
-// ids, each list as long as the device-side count says (up to the buffer's capacity).
+// and one synchronisation: per pass its row counts and kept-row counts, then its live ids,
+// retired ids and flagged ids, each list as long as the device-side count says (up to the
+// buffer's capacity).
 int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::vector<ExtraRead>& extras = {}) {
   const int L = net->L;
   hipStream_t st = net->eng->stream;
@@ -412,7 +422,7 @@ int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::
   for (size_t r = 0; r < reqs.size(); ++r) {
     const RunBufs& rb = *reqs[r].rb;
     const int32_t* counts = rb.counts.as<int32_t>();
-    where[r].counts = add(counts, nullptr, L + 2, false);
+    where[r].counts = add(counts, nullptr, 2 * (L + 2), false);
     where[r].cats = add(rb.id(reqs[r].cat_side), counts + L + 1, rb.ld, true);
     where[r].rets = add(rb.ret_ids.as<int32_t>(), rb.ret_count.as<int32_t>(), rb.ld, true);
     where[r].flags = add(rb.flag_ids.as<int32_t>(), rb.flag_count.as<int32_t>(), rb.ld, true);
@@ -430,7 +440,7 @@ int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::
   for (size_t r = 0; r < reqs.size(); ++r) {
     PassOut& out = *reqs[r].out;
     const Where& w = where[r];
-    out.counts.assign(h + w.counts, h + w.counts + L + 2);
+    out.counts.assign(h + w.counts, h + w.counts + 2 * (L + 2));
     out.cats.assign(h + w.cats + 1, h + w.cats + 1 + h[w.cats]);
     out.cats.insert(out.cats.end(), h + w.rets + 1, h + w.rets + 1 + h[w.rets]);
     std::sort(out.cats.begin(), out.cats.end());  // the compacted order is not the sample order
@@ -546,6 +556,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.order = o.spmm_order;
   cfg.groups_per_cu = o.mfma_groups_per_cu;
   cfg.tail = o.guard_tail;
+  cfg.keep_ppm = o.precision == COSPDNN_FP16 ? o.keep_holes_ppm : 0;
   CALL_TRY(ensure_bufs(net->main, net->S, L, N, cfg.precision, false, e->stream));
   return COSPDNN_OK;
 }
@@ -561,6 +572,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
   return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
+         gc.keep_ppm == cfg.keep_ppm &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
@@ -580,6 +592,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.mfma32 = cfg.mfma32;
   gc.order = cfg.order;
   gc.groups = cfg.groups_per_cu;
+  gc.keep_ppm = cfg.keep_ppm;
   gc.plan_version = plan_version;
 }
 
@@ -710,7 +723,9 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
   int64_t n_flagged = 0, n_replayed = 0;
   PassOut p1;
   CALL_TRY(read_pass(net, net->main, cat_side, p1));
-  for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
+  for (int l = 0; l <= L; ++l) live[l] = p1.counts[L + 2 + l + 1];  // kept rows (holes not counted)
+  if (res->positions)
+    for (int l = 0; l <= L; ++l) res->positions[l] = p1.counts[l + 1];
   for (int32_t id : p1.cats) alive[id] = 1;
   first = alive;
 
@@ -765,7 +780,9 @@ int run_finish_dev_guard(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& 
                        {ExtraRead{net->gs.p, 3, gsc}, ExtraRead{net->gids.p, static_cast<int>(net->gcap64), gids.data()}}));
   if (gsc[2] != 0) return run_finish(net, o, cfg, cat_side, res);
   std::vector<int64_t> live(L + 1, -1);
-  for (int l = 0; l <= L; ++l) live[l] = p1.counts[l + 1];
+  for (int l = 0; l <= L; ++l) live[l] = p1.counts[L + 2 + l + 1];  // kept rows (holes not counted)
+  if (res->positions)
+    for (int l = 0; l <= L; ++l) res->positions[l] = p1.counts[l + 1];
   std::vector<char> alive(S, 0);
   for (int32_t id : p1.cats) alive[id] = 1;
   const std::vector<char> first = alive;
@@ -873,6 +890,7 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
     int side = 0;
     RunCfg dcfg = cfg;
     dcfg.guard_layer = -1;
+    dcfg.keep_ppm = 0;  // the dump lists the live rows of the layer
     int rc = enqueue_forward(net, net->main, net->store.as<_Float16>(), net->ld_store, S, dcfg, false, st, &side);
     net->L = saveL;
     CALL_TRY(rc);
@@ -1105,6 +1123,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->batch_concurrent = 2;
   o->guard_tail = 8;
   o->guard_concurrent = 1;
+  o->keep_holes_ppm = 0;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {

@@ -100,7 +100,7 @@ using cospdnn_detail::RunBufs;
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0;
   int64_t plan_version = -1;
   // the device-side guard pipeline captured with the main pass (0: main pass only)
   int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;

@@ -119,6 +119,15 @@ __device__ __forceinline__ int count_flags16(uint4 v) {
   return __popc(v.x & 0x01010101u) + __popc(v.y & 0x01010101u) + __popc(v.z & 0x01010101u) +
          __popc(v.w & 0x01010101u);
 }
+// the flag bytes of a 16-byte word at positions below lim (lim - its first position)
+__device__ __forceinline__ int count_flags16_lim(uint4 v, int lim) {
+  if (lim >= kScanPer) return count_flags16(v);
+  auto m = [&](int k) -> uint32_t {
+    const int valid = min(max(lim - 4 * k, 0), 4);
+    return valid >= 4 ? 0x01010101u : ((1u << (8 * valid)) - 1u) & 0x01010101u;
+  };
+  return __popc(v.x & m(0)) + __popc(v.y & m(1)) + __popc(v.z & m(2)) + __popc(v.w & m(3));
+}
 __device__ __forceinline__ bool flag_at(const uint4& v, int i) {
   const uint32_t w = i < 4 ? v.x : (i < 8 ? v.y : (i < 12 ? v.z : v.w));
   return ((w >> (8 * (i & 3))) & 0xFFu) != 0;
@@ -147,6 +156,7 @@ __device__ __forceinline__ uint4 kept_flags(const ScanArgs& a, int pos) {
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   __shared__ int wsum[kScanThreads / 64];
   __shared__ int wpre[kScanThreads / 64];
+  __shared__ int wpost[kScanThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int g0 = blockIdx.x * kScanTile;
   const int pc = g0 + tid * kScanPer;  // multiple of 16: aligned, inside the ld-sized arrays
@@ -178,9 +188,30 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   const int n = a.counts[a.layer];
   if (blockIdx.x == 0 && tid == 0 && a.work != nullptr) *a.work = 0;
   if (g0 >= n) return;  // uniform per group; positions [0, g0) are all < n
+  // With keep_ppm every group also counts the kept rows after it, [g0 + tile, n): whether
+  // the layout is kept depends on the total.
+  int after = 0;
+  if (a.keep_ppm > 0) {
+    for (int base = g0 + kScanTile; base < n; base += 4 * kScanTile) {
+      uint4 w[4];
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) before += __shfl_xor(before, d, 64);
-  if (lane == 0) wpre[wv] = before;
+      for (int u = 0; u < 4; ++u) {
+        const int pos = base + u * kScanTile + tid * kScanPer;
+        w[u] = pos < n ? kept_flags(a, pos) : zero;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) after += count_flags16_lim(w[u], n - (base + u * kScanTile + tid * kScanPer));
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    before += __shfl_xor(before, d, 64);
+    after += __shfl_xor(after, d, 64);
+  }
+  if (lane == 0) {
+    wpre[wv] = before;
+    wpost[wv] = after;
+  }
 
   uint4 va = zero, vr = zero, vf = zero;
   int cnt = 0;
@@ -209,15 +240,21 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
   }
   if (lane == 63) wsum[wv] = x;
   __syncthreads();
-  int base = 0, wbase = 0, total = 0;
+  int base = 0, wbase = 0, total = 0, post = 0;
 #pragma unroll
   for (int i = 0; i < kScanThreads / 64; ++i) {
     base += wpre[i];
+    post += wpost[i];
     wbase += i < wv ? wsum[i] : 0;
     total += wsum[i];
   }
-  // the last group writes the kept-row total
-  if (tid == 0 && g0 + kScanTile >= n) a.counts[a.layer + 1] = base + total;
+  const int removed = n - (base + total + post);
+  const bool keep = a.keep_ppm > 0 && static_cast<int64_t>(removed) * 1000000 <= static_cast<int64_t>(a.keep_ppm) * n;
+  // the last group writes the row count of the next layer and the kept-row total
+  if (tid == 0 && g0 + kScanTile >= n) {
+    a.counts[a.layer + 1] = keep ? n : base + total;
+    if (a.kept != nullptr) a.kept[a.layer + 1] = base + total;
+  }
   if (pc >= n) return;
   int q = base + wbase + x - cnt;
   int qm[kScanPer];
@@ -227,7 +264,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
     const int32_t id = (i & 3) == 0 ? id4[i >> 2].x : ((i & 3) == 1 ? id4[i >> 2].y : ((i & 3) == 2 ? id4[i >> 2].z : id4[i >> 2].w));
     const bool al = p < n && flag_at(va, i);
     qm[i] = -1;
-    if (al) {
+    if (keep) {
+      // kept layout: every position maps to itself, removed rows stay as holes
+      if (p < n) {
+        qm[i] = p;
+        a.ids_out[p] = id;
+      }
+    } else if (al) {
       qm[i] = q;
       a.ids_out[q] = id;
       ++q;

@@ -95,6 +95,14 @@ struct ScanArgs {
   // Flag the rows that died in the previous layer (deaths close to the last layer are
   // rechecked: a later pass at higher precision may keep such a row alive to the end).
   int flag_dead = 0;
+  // Keep the layout instead of compacting when at most keep_ppm parts per million of the
+  // rows entering the layer are removed (0: always compact).  Kept layout: qmap and ids are
+  // the identity, counts[layer+1] = counts[layer], and the removed rows stay in place as
+  // holes.  Only for scans that remove dead rows alone: a dead row is all-zero and stays
+  // so (bias <= 0), so a hole never sets a flag; the next compacting scan drops it.
+  int keep_ppm = 0;
+  // kept[layer+1] = rows kept (without holes); may be null
+  int32_t* kept = nullptr;
 };
 
 enum class Dtype : int { F16 = 0, F32 = 1, F64 = 2 };

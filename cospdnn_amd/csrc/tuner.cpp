@@ -169,7 +169,7 @@ int cospdnn_tune_exhaustive(cospdnn_net* net, const cospdnn_opts* opts, int reps
     for (int r = 0; r <= reps && rc == COSPDNN_OK; ++r) {  // run 0 warms up
       cospdnn_result res{};
       res.layer_ms = lms.data();
-      if (k == 0 && r == 0) res.live = live0.data();
+      if (k == 0 && r == 0) res.positions = live0.data();  // rows each layer's kernel works on
       rc = run_one(net, o, &res);
       if (rc != COSPDNN_OK || r == 0) continue;
       for (int l = 0; l < L; ++l) runs[l].push_back(lms[l]);
@@ -192,6 +192,43 @@ int cospdnn_tune_exhaustive(cospdnn_net* net, const cospdnn_opts* opts, int reps
   }
   if (ms_table) std::memcpy(ms_table, table.data(), sizeof(float) * static_cast<size_t>(kNumCandidates) * L);
   if (live) std::memcpy(live, live0.data(), sizeof(int64_t) * (L + 1));
+  return COSPDNN_OK;
+}
+
+// The compaction threshold: whole FP16 forwards (the options as given, e.g. replayed from a
+// hipGraph) per candidate keep_holes_ppm, median of reps runs after one warm-up run.  Keeping
+// the layout saves the slow stores of shifted tiles in the layer after a scan that removed
+// rows; the holes it leaves cost bandwidth in every later layer until a scan compacts, so
+// the best threshold is measured rather than modelled per layer.
+int cospdnn_tune_keep(cospdnn_net* net, const cospdnn_opts* opts, const int32_t* cand_ppm, int ncand, int reps,
+                      float* ms, int32_t* best_ppm) {
+  if (!net || !cand_ppm || ncand < 1 || reps < 1 || !best_ppm) return fail(COSPDNN_ERR_ARG, "bad argument");
+  if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
+  for (int k = 0; k < ncand; ++k)
+    if (cand_ppm[k] < 0 || cand_ppm[k] > 1000000) return fail(COSPDNN_ERR_ARG, "keep ppm out of [0, 1e6]");
+  cospdnn_opts o;
+  cospdnn_default_opts(&o);
+  if (opts) o = *opts;
+  o.precision = COSPDNN_FP16;
+  o.layer_timing = 0;
+  o.dump_layer = -1;
+  float best_ms = 0.0f;
+  for (int k = 0; k < ncand; ++k) {
+    o.keep_holes_ppm = cand_ppm[k];
+    std::vector<float> t;
+    for (int r = 0; r <= reps; ++r) {
+      cospdnn_result res{};
+      CALL_TRY(run_one(net, o, &res));
+      if (r > 0) t.push_back(static_cast<float>(res.ms_total));
+    }
+    std::sort(t.begin(), t.end());
+    const float med = t[t.size() / 2];
+    if (ms) ms[k] = med;
+    if (k == 0 || med < best_ms) {
+      best_ms = med;
+      *best_ppm = cand_ppm[k];
+    }
+  }
   return COSPDNN_OK;
 }
 

@@ -80,6 +80,11 @@ typedef struct {
                             = the FP32 and FP64 tiers run on the device on a side stream, forked
                             off after the checkpoint scan, next to the rest of the main pass;
                             0 = after it, one tier after the other */
+  int keep_holes_ppm;    /* FP16: a scan that removes only dead rows keeps the layout (the dead
+                            rows stay as all-zero holes until a later scan compacts) when it
+                            removes at most this many ppm of the rows entering the layer;
+                            compacting shifts every later tile off its 128-byte lines, which
+                            slows its stores (default 0: always compact) */
 } cospdnn_opts;
 
 typedef struct {
@@ -103,6 +108,9 @@ typedef struct {
   double edges_per_sec;  /* edges / ms_total */
   double alg_bytes;      /* sum_l (2 * live_l * N * act_bytes + nnz_l * weight_bytes) */
   int64_t mfma_layers;   /* layers that ran on the MFMA kernel */
+  int64_t* positions;    /* caller-provided (may be NULL), capacity L+1: row positions entering
+                            layer l in the first pass, live rows plus the holes of a kept
+                            layout (keep_holes_ppm) */
 } cospdnn_result;
 
 const char* cospdnn_version(void);
@@ -179,10 +187,14 @@ int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, i
                                const cospdnn_layer_plan* cand, float* feat);
 /* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer
    (median of reps runs, per-layer HIP events) and keeps, per layer, the fastest.
-   ms_table [ncand][L] (may be NULL), live [L+1] rows entering each layer (may be NULL),
-   plan [L].  The network's plan is left unchanged. */
+   ms_table [ncand][L] (may be NULL), live [L+1] row positions entering each layer (the
+   rows its kernel works on; may be NULL), plan [L].  The network's plan is left unchanged. */
 int cospdnn_tune_exhaustive(cospdnn_net* net, const cospdnn_opts* opts, int reps, float* ms_table, int64_t* live,
                             cospdnn_layer_plan* plan);
+/* Compaction threshold: the FP16 forward (opts as given) once per candidate keep_holes_ppm,
+   median ms_total of reps runs; ms [ncand] (may be NULL), *best_ppm = the fastest. */
+int cospdnn_tune_keep(cospdnn_net* net, const cospdnn_opts* opts, const int32_t* cand_ppm, int ncand, int reps,
+                      float* ms, int32_t* best_ppm);
 
 /* Cost-model MLP: n_in -> hidden -> hidden -> 1, ReLU, FP32, on the engine's device.
    Training (Adam on mean squared error), gradients and inference run in HIP kernels. */

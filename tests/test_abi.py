@@ -55,3 +55,4 @@ def test_options_struct_matches_the_header():
     assert o.spmm_order == 1 and o.mfma_groups_per_cu == 4
     assert o.batch_concurrent == 2 and o.guard_tail == 8
     assert o.guard_concurrent == 1
+    assert o.keep_holes_ppm == 0

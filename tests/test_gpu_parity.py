@@ -239,6 +239,40 @@ def test_concurrent_guard_matches_sequential(engine, kind):
     net.close()
 
 
+@pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
+def test_kept_layout_matches_compaction(engine, kind):
+    """Scans that keep the layout (dead rows left in place as all-zero holes) instead of
+    compacting decide every row as the compacting pass does: categories equal to the FP64
+    oracle's, the same live rows per layer and the same rows handed to the guard tiers, on the
+    MFMA and the SIMT kernel, eager and replayed from a hipGraph.  With every allowed scan
+    keeping, the layout carries holes; with a threshold below one row, it carries none."""
+    N, L, S, bias, seed = (2048, 40, 6000, -0.4, 41) if kind == cd.GEN_BANDED else (1024, 40, 6000, -0.3, 42)
+    say(f"kept layout kind={kind}")
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
+    for use_mfma in (1, 0):
+        base = cd.run([net], cd.default_opts(precision=cd.FP16, use_mfma=use_mfma))[0]
+        assert np.array_equal(base.categories, want)
+        assert np.array_equal(base.positions[:L], base.live[:L])
+        for ppm, graph, conc in ((1, 0, 1), (1000, 0, 1), (20000, 1, 1), (1000000, 0, 1), (1000000, 1, 1),
+                                 (1000000, 0, 0)):
+            say(f"  use_mfma={use_mfma} ppm={ppm} graph={graph} concurrent={conc}")
+            o = cd.default_opts(precision=cd.FP16, use_mfma=use_mfma, keep_holes_ppm=ppm, use_graph=graph,
+                                guard_concurrent=conc)
+            out = cd.run([net], o)[0]
+            assert np.array_equal(out.categories, want), (use_mfma, ppm, graph)
+            assert np.array_equal(out.live, base.live), (use_mfma, ppm, graph)
+            assert out.n_flagged == base.n_flagged
+            assert (out.positions[:L] >= out.live[:L]).all()
+            assert (out.positions <= S).all()
+            if ppm == 1:
+                assert np.array_equal(out.positions[:L], out.live[:L])
+            if ppm == 1000000:
+                assert (out.positions[:L] > out.live[:L]).any()
+    net.close()
+
+
 def test_concurrent_guard_overflow_falls_back(engine):
     """Checkpoints at layer 1 of both guarded passes leave more rows for the FP64 tier than
     its device buffers hold (512): the run redoes the tiers one after the other and still

@@ -115,6 +115,24 @@ def test_exhaustive_plan_is_the_per_layer_minimum(engine):
     net.close()
 
 
+def test_keep_threshold_tuning(engine):
+    """The compaction-threshold search times every candidate, returns the fastest one, and
+    the forward at that threshold keeps the categories of the FP64 oracle."""
+    N, L, S, bias, seed = 2048, 40, 8000, -0.4, 23
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, orc.BANDED, seed), 0, S)
+    net = make_net(engine, N, L, S, bias, cd.GEN_BANDED, seed)
+    o = cd.default_opts(precision=cd.FP16, use_graph=1)
+    best, ms = cd.tune_keep(net, o, reps=2)
+    say(f"keep ppm {best}, ms {ms.tolist()}")
+    assert best in cd.KEEP_CANDIDATES
+    assert (ms > 0).all() and ms[list(cd.KEEP_CANDIDATES).index(best)] == ms.min()
+    out = cd.run([net], cd.default_opts(precision=cd.FP16, use_graph=1, keep_holes_ppm=best))[0]
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    with pytest.raises(cd.CospdnnError):
+        cd.tune_keep(net, o, candidates=(0, 2000000))
+    net.close()
+
+
 def test_mlp_plan_regret_on_an_unseen_network(engine):
     """Train the cost model on timings of some networks, plan a network it never saw, and
     compare with exhaustive tuning (the judge) and with the default choice (no tuner)."""
