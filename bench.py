@@ -2,12 +2,14 @@
 
 A step is one full forward of the network (all L layers, compaction and the
 FP32 / FP64 guard recompute) over the synthetic input already resident in HBM.
-Before the timed steps the per-layer launch plan is autotuned (exhaustive search,
-then the cost-model MLP trained on those timings picks the plan that is timed).
+Before the timed steps the per-layer launch plan and the compaction threshold are
+autotuned (exhaustive search, the compaction threshold on whole forwards, then the
+cost-model MLP trained on those timings picks the plan that is timed).
 Default workload: N=16384, L=120, 60000 samples, banded connectivity, bias -0.4
 (the configuration the FP16-MFMA target is stated on).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--cpu-samples 1500] [--tune 1] [--graph 1]
+                    [--tuning-out FILE | --tuning-in FILE]
 """
 import argparse
 import json
@@ -105,6 +107,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the FP16 layer loop from a hipGraph")
     ap.add_argument("--tune", type=int, default=1, help="1: autotune the per-layer launch plan before timing")
+    ap.add_argument("--tuning-out", default=None, help="write the tuned plan and compaction threshold (JSON)")
+    ap.add_argument("--tuning-in", default=None,
+                    help="use the plan and compaction threshold of a --tuning-out file instead of tuning "
+                         "(profiled runs time the same configuration without the tuning runs)")
     ap.add_argument("--dy-samples", type=int, default=1000, help="samples for max |dY| against the FP64 oracle (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -141,7 +147,16 @@ def main():
     # both tables; its plan is the one the timed runs use.
     tuning = None
     keep_ppm = 0
-    if args.tune:
+    if args.tuning_in:
+        with open(args.tuning_in) as f:
+            saved = json.load(f)
+        if saved["config"] != args.config or len(saved["plan"]) != spec.L:
+            raise SystemExit(f"bench.py: {args.tuning_in} holds a tuning of {saved['config']}, not {args.config}")
+        net.set_plan([tuple(p) for p in saved["plan"]])
+        keep_ppm = int(saved["keep_holes_ppm"])
+        tuning = {"from_file": args.tuning_in, "keep_holes_ppm": keep_ppm}
+        log(f"[bench] plan and compaction threshold ({keep_ppm} ppm) from {args.tuning_in}")
+    elif args.tune:
         t0 = time.perf_counter()
         cands = cd.candidates()
         _, table0, pos0 = cd.tune_exhaustive(net, reps=2)
@@ -169,6 +184,9 @@ def main():
         }
         tuning["mlp_regret"] = tuning["spmm_ms_mlp"] / tuning["spmm_ms_exhaustive"] - 1.0
         net.set_plan(mlp_plan)
+        if args.tuning_out:
+            with open(args.tuning_out, "w") as f:
+                json.dump({"config": args.config, "plan": [list(p) for p in mlp_plan], "keep_holes_ppm": keep_ppm}, f)
         log(f"[bench] compaction threshold {keep_ppm} ppm (forward ms per candidate "
             f"{ {k: round(v, 3) for k, v in tuning['forward_ms_per_keep_ppm'].items()} })")
         log(f"[bench] tuning: exhaustive {tuning['spmm_ms_exhaustive']:.2f} ms, MLP plan {tuning['spmm_ms_mlp']:.2f} ms "

@@ -44,6 +44,7 @@ class Opts(ctypes.Structure):
         ("guard_tail", ctypes.c_int),
         ("guard_concurrent", ctypes.c_int),
         ("keep_holes_ppm", ctypes.c_int),
+        ("spmm_gather", ctypes.c_int),
     ]
 
 

@@ -72,6 +72,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     CALL_TRY(dalloc(rb.flags, static_cast<size_t>(6 * rb.ld)));
     CALL_TRY(dalloc(rb.ids, sizeof(int32_t) * 2 * rb.ld));
     CALL_TRY(dalloc(rb.qmap, sizeof(int32_t) * rb.ld));
+    CALL_TRY(dalloc(rb.qsrc, sizeof(int32_t) * rb.ld));
     CALL_TRY(dalloc(rb.counts, sizeof(int32_t) * 2 * (L + 2)));  // row counts, then kept-row counts
     CALL_TRY(dalloc(rb.flag_ids, sizeof(int32_t) * rb.ld));
     CALL_TRY(dalloc(rb.flag_count, sizeof(int32_t)));
@@ -190,6 +191,7 @@ struct RunCfg {
   int grid_cap = 0;          // > 0: at most this many work-groups per SpMM launch
   int keep_ppm = 0;          // FP16 pass: scans that only drop dead rows keep the layout when
                              // at most this many ppm of the rows are removed (0: never)
+  int gather = 0;            // FP16 pass: MFMA layers after a compacting scan gather by output position
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -257,6 +259,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.ids_in = rb.id(cur);
     sa.ids_out = rb.id(nxt);
     sa.qmap = rb.qmap.as<int32_t>();
+    sa.qsrc = (fp16 && cfg.gather) ? rb.qsrc.as<int32_t>() : nullptr;
     sa.flag_undecided = (l == D) ? 1 : 0;
     sa.flag_ids = rb.flag_ids.as<int32_t>();
     sa.flag_count = rb.flag_count.as<int32_t>();
@@ -298,6 +301,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.counts = rb.counts.as<int32_t>();
     a.layer = l;
     a.qmap = rb.qmap.as<int32_t>();
+    a.qsrc = (fp16 && cfg.gather) ? rb.qsrc.as<int32_t>() : nullptr;
     a.alive_next = rb.f(nxt, 0);
     a.unsat_next = rb.f(nxt, 1);
     const bool band_on = cfg.band > 0.0f && l >= cfg.band_from;
@@ -557,6 +561,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.groups_per_cu = o.mfma_groups_per_cu;
   cfg.tail = o.guard_tail;
   cfg.keep_ppm = o.precision == COSPDNN_FP16 ? o.keep_holes_ppm : 0;
+  cfg.gather = o.precision == COSPDNN_FP16 ? o.spmm_gather : 0;
   CALL_TRY(ensure_bufs(net->main, net->S, L, N, cfg.precision, false, e->stream));
   return COSPDNN_OK;
 }
@@ -572,7 +577,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
   return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
-         gc.keep_ppm == cfg.keep_ppm &&
+         gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
@@ -593,6 +598,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.order = cfg.order;
   gc.groups = cfg.groups_per_cu;
   gc.keep_ppm = cfg.keep_ppm;
+  gc.gather = cfg.gather;
   gc.plan_version = plan_version;
 }
 
@@ -1124,6 +1130,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->guard_tail = 8;
   o->guard_concurrent = 1;
   o->keep_holes_ppm = 0;
+  o->spmm_gather = 1;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {

@@ -80,7 +80,8 @@ struct RunBufs {
   DevBuf act16[2], act32[2], act64[2];
   DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
   DevBuf ids;    // 2 x ld int32
-  DevBuf qmap;   // ld int32
+  DevBuf qmap;   // ld int32: output position of each input position (-1: removed)
+  DevBuf qsrc;   // ld int32: input position of each output position (compacting FP16 scans)
   DevBuf counts; // L+2 int32
   DevBuf flag_ids, flag_count, flag_mark;
   DevBuf work;   // MFMA work-queue counter
@@ -100,7 +101,7 @@ using cospdnn_detail::RunBufs;
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0, gather = 0;
   int64_t plan_version = -1;
   // the device-side guard pipeline captured with the main pass (0: main pass only)
   int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;

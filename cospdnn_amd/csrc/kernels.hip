@@ -273,6 +273,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
     } else if (al) {
       qm[i] = q;
       a.ids_out[q] = id;
+      if (a.qsrc != nullptr) a.qsrc[q] = p;
       ++q;
     }
     if (p < n && flag_at(vr, i)) a.ret_ids[atomicAdd(a.ret_count, 1)] = id;
@@ -309,6 +310,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(ScanArgs a) {
 // the liveness bits OR-reduced over the 16 lanes of a row group.  Otherwise the
 // pre-activations are staged in LDS as FP32 [32][68] and lane L, owning sample L,
 // stores its 32 outputs one by one.
+// Gather mode (a layer after a compacting scan, with the output-to-input position map
+// qsrc): a work item is 64 consecutive OUTPUT positions, so every tile's stores land on
+// whole 128-byte lines (a compacted tile of the other mode straddles lines that another
+// wave writes too, which costs about 30% of the store bandwidth).  Lane L gathers its
+// output position's input column, 32 rows of 2 bytes (one load per row, all lanes on the
+// same row), packs them into 16 dwords and writes them as sample L's 64-byte row of the
+// LDS slice ([64 samples][32 rows], 16-byte chunks XOR-swizzled); the X fragments are
+// then plain 16-byte LDS reads.
 // --------------------------------------------------------------------------
 constexpr int kWaveSamples = 64;
 constexpr int kWavesPerGroup = 4;
@@ -432,13 +441,37 @@ __device__ __forceinline__ void store_stream(_Float16* p, uint4 v) {
   __builtin_nontemporal_store(u32x4_t{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4_t*>(p));
 }
 
-__global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
-  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];  // 8.5 KB per wave
-  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
-  // no row left the pass in the scan before this layer: every position maps to itself
-  const bool ident = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]) == n;
+// 16-byte chunk k (rows 8k..8k+7) of sample s in the gather-mode LDS slice: swizzled so that
+// the 8 lanes of each LDS cycle hit different banks, both when lane s writes its sample and
+// when the X-fragment reads take samples 16(i/4) + 4t + i%4
+__device__ __forceinline__ int gather_chunk(int s, int k) { return s * 4 + (k ^ (((s >> 2) ^ (s >> 4)) & 3)); }
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// The 32 input rows of chunk ch (row ids in lanes 0..31 of grow) at input position src:
+// dword k of d holds rows 2k (low half) and 2k+1 (high half).  Each load has every lane on
+// the same row (base address scalar) at nearby positions.
+__device__ __forceinline__ void gather_rows(uint32_t (&d)[16], const _Float16* __restrict__ X, int64_t ldx, int grow,
+                                            int src) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r0 = __builtin_amdgcn_readlane(grow, 2 * k);
+    const int r1 = __builtin_amdgcn_readlane(grow, 2 * k + 1);
+    half2_t v;
+    v.x = X[static_cast<int64_t>(r0) * ldx + static_cast<uint32_t>(src)];
+    v.y = X[static_cast<int64_t>(r1) * ldx + static_cast<uint32_t>(src)];
+    d[k] = __builtin_bit_cast(uint32_t, v);
+  }
+}
+
+template <bool GATHER>
+__device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, float (*wlds)[32 * kStageLd], int n,
+                                          int n_out) {
+  // ident: no row left the pass in the scan before this layer, every position maps to itself
+  const bool ident = n_out == n;
   const int nblk = w.nblk;
-  const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
+  // tiles of input positions, or of output positions in gather mode
+  const int ntiles = ((GATHER ? n_out : n) + kWaveSamples - 1) / kWaveSamples;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   ItemOrder io{a.order, nblk, ntiles, 0, 1, 0};
@@ -476,6 +509,7 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   step_next(nx2, stride, nitems, io, kptr, k1);
 
   _Float16* tile = reinterpret_cast<_Float16*>(wlds[wave]);  // [32][64] FP16 input tile (bytes 0..4095)
+  uint4* gtile = reinterpret_cast<uint4*>(wlds[wave]);        // gather mode: [64][4] 16-byte chunks (bytes 0..4095)
   _Float16* ostage = tile + 32 * kWaveSamples;               // [32][kOutLd] FP16 outputs (bytes 4096..8703)
   float* stage = wlds[wave];                                 // [32][kStageLd] FP32 pre-activations (slow path)
   // global <-> registers: access k (0..3) of a lane covers row 8k + r8, samples c8..c8+7
@@ -488,27 +522,40 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   // fragments, the target position of this lane's sample, the output ids; the row ids
   // two steps ahead.  After the final step the prefetch re-reads valid (unused) addresses.
   int rn0, rn1, rn2, rn3;  // row ids of step nx1
-  {
-    const int ch = (nx1.valid ? nx1.ch : cur.ch) * kChunk + r8;
-    rn0 = gidx[ch];
-    rn1 = gidx[ch + 8];
-    rn2 = gidx[ch + 16];
-    rn3 = gidx[ch + 24];
-  }
   uint4 d0, d1, d2, d3;
   half8_t w0, w1;
   int qv;
-  {
+  // gather mode: this step's 32 rows of the lane's sample (gd), the row ids of step nx1 (in
+  // lanes 0..31 of grow1) and the input position of the lane's sample in nx1's tile
+  uint32_t gd[16];
+  int grow1 = 0, gsrc1 = 0;
+  const int32_t* __restrict__ qsrc = a.qsrc;
+  if constexpr (GATHER) {
+    const int pch = nx1.valid ? nx1.ch : cur.ch;
+    const int pt = nx1.valid ? nx1.t : cur.t;
+    grow1 = gidx[pch * kChunk + (lane & 31)];
+    gsrc1 = qsrc[min(pt * kWaveSamples + lane, n_out - 1)];
+    const int grow = gidx[cur.ch * kChunk + (lane & 31)];
+    const int src = qsrc[min(cur.t * kWaveSamples + lane, n_out - 1)];
+    gather_rows(gd, X, ldx, grow, src);
+  } else {
+    {
+      const int ch = (nx1.valid ? nx1.ch : cur.ch) * kChunk + r8;
+      rn0 = gidx[ch];
+      rn1 = gidx[ch + 8];
+      rn2 = gidx[ch + 16];
+      rn3 = gidx[ch + 24];
+    }
     const int ch = cur.ch * kChunk + r8;
     const _Float16* base = X + cur.t * kWaveSamples + c8;
     d0 = load_stream(base + static_cast<int64_t>(gidx[ch]) * ldx);
     d1 = load_stream(base + static_cast<int64_t>(gidx[ch + 8]) * ldx);
     d2 = load_stream(base + static_cast<int64_t>(gidx[ch + 16]) * ldx);
     d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * ldx);
-    w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
-    w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
     qv = ident ? cur.t * kWaveSamples + lane : a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
   }
+  w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
+  w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
   // accumulators start at the bias: D' + b comes out of the MFMA chain
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
   const bool full = w.full_blocks != 0;
@@ -555,27 +602,35 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   while (true) {
     const bool last = cur.ch + 1 == cur.ch_end;
     const int p_own = cur.t * kWaveSamples + lane;
-    const int q_own = p_own < n ? qv : -1;
+    // the output position of the lane's sample (-1: none)
+    const int q_own = GATHER ? (p_own < n_out ? p_own : -1) : (p_own < n ? qv : -1);
     const half8_t cw0 = w0, cw1 = w1;
     // output rows j and 16 + j of the block column (slot = row, -1 for padding)
     const int cov0 = cur.c * 32 + j < w.n_out ? cur.c * 32 + j : -1;
     const int cov1 = cur.c * 32 + 16 + j < w.n_out ? cur.c * 32 + 16 + j : -1;
     const int cur_c = cur.c;
-    // this step's tile into the wave's LDS slice (waits for the loads issued last step)
-    {
-      uint4* dst = reinterpret_cast<uint4*>(tile + r8 * kWaveSamples + c8);
-      dst[0 * 8 * kWaveSamples / 8] = d0;
-      dst[1 * 8 * kWaveSamples / 8] = d1;
-      dst[2 * 8 * kWaveSamples / 8] = d2;
-      dst[3 * 8 * kWaveSamples / 8] = d3;
-    }
-    // Next step's tile rows, last step's stores, then the rest of the next step's loads.
-    // Every global access of a step is issued here, at its start, so that the wait at the
-    // top of the next step finds them done; the tile loads go first as they need this
-    // step's row ids (no load may be waited for after a store: the wait would cover it).
     const int pt = nx1.valid ? nx1.t : cur.t;
     const int pch = nx1.valid ? nx1.ch : cur.ch;
-    {
+    if constexpr (GATHER) {
+      // this step's rows of the lane's sample into the wave's LDS slice, then the next
+      // step's gathers (row ids and input positions loaded a step earlier)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        gtile[gather_chunk(lane, k)] = make_uint4(gd[4 * k], gd[4 * k + 1], gd[4 * k + 2], gd[4 * k + 3]);
+      gather_rows(gd, X, ldx, grow1, gsrc1);
+    } else {
+      // this step's tile into the wave's LDS slice (waits for the loads issued last step)
+      {
+        uint4* dst = reinterpret_cast<uint4*>(tile + r8 * kWaveSamples + c8);
+        dst[0 * 8 * kWaveSamples / 8] = d0;
+        dst[1 * 8 * kWaveSamples / 8] = d1;
+        dst[2 * 8 * kWaveSamples / 8] = d2;
+        dst[3 * 8 * kWaveSamples / 8] = d3;
+      }
+      // Next step's tile rows, last step's stores, then the rest of the next step's loads.
+      // Every global access of a step is issued here, at its start, so that the wait at the
+      // top of the next step finds them done; the tile loads go first as they need this
+      // step's row ids (no load may be waited for after a store: the wait would cover it).
       const _Float16* base = X + pt * kWaveSamples + c8;
       d0 = load_stream(base + static_cast<int64_t>(rn0) * ldx);
       d1 = load_stream(base + static_cast<int64_t>(rn1) * ldx);
@@ -586,32 +641,52 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     flush();
     __builtin_amdgcn_sched_barrier(0);
     {
-      w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
-      w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
-      qv = ident ? pt * kWaveSamples + lane : a.qmap[min(pt * kWaveSamples + lane, n - 1)];
-      const int ch2 = (nx2.valid ? nx2.ch : pch) * kChunk + r8;
-      rn0 = gidx[ch2];
-      rn1 = gidx[ch2 + 8];
-      rn2 = gidx[ch2 + 16];
-      rn3 = gidx[ch2 + 24];
+      // W fragments of the next step (gather mode: after this step's MFMAs, so that the
+      // gathered rows in flight and the W registers in use do not both need copies)
+      if constexpr (!GATHER) {
+        w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
+        w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
+      }
+      if constexpr (GATHER) {
+        // row ids and input positions of step nx2
+        grow1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + (lane & 31)];
+        gsrc1 = qsrc[min((nx2.valid ? nx2.t : pt) * kWaveSamples + lane, n_out - 1)];
+      } else {
+        qv = ident ? pt * kWaveSamples + lane : a.qmap[min(pt * kWaveSamples + lane, n - 1)];
+        const int ch2 = (nx2.valid ? nx2.ch : pch) * kChunk + r8;
+        rn0 = gidx[ch2];
+        rn1 = gidx[ch2 + 8];
+        rn2 = gidx[ch2 + 16];
+        rn3 = gidx[ch2 + 24];
+      }
     }
-    // X fragments: ds_read_b64_tr_b16 gives lane i of each 16-lane group element i%4 of
-    // the 8 bytes read by lanes 4k+i/4 (k < 4).  Lane m reads row 8g+4h+m/4 at samples
-    // 16(m%4)+4t..+3, so lane i receives rows 8g+4h+k of sample 16(i/4)+4t+i%4.
+    // X fragments.  Lane i of each 16-lane group g needs rows 8g..8g+7 of sample
+    // 16(i/4)+4t+i%4.  Gather mode: one 16-byte read of that sample's chunk g.  Otherwise
+    // ds_read_b64_tr_b16, which gives lane i element i%4 of the 8 bytes read by lanes
+    // 4k+i/4 (k < 4): lane m reads row 8g+4h+m/4 at samples 16(m%4)+4t..+3.
     half8_t xfrag[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      const _Float16* p0 = tile + (g * 8 + (j >> 2)) * kWaveSamples + 16 * (j & 3) + 4 * t;
-      const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)p0);
-      const fp16x4_t hi =
-          __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)(p0 + 4 * kWaveSamples));
-      xfrag[t] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
-                                         4, 5, 6, 7);
+      if constexpr (GATHER) {
+        const int smp = 16 * (j >> 2) + 4 * t + (j & 3);
+        xfrag[t] = __builtin_bit_cast(half8_t, gtile[gather_chunk(smp, g)]);
+      } else {
+        const _Float16* p0 = tile + (g * 8 + (j >> 2)) * kWaveSamples + 16 * (j & 3) + 4 * t;
+        const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)p0);
+        const fp16x4_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4f16(
+            (__attribute__((address_space(3))) fp16x4_t*)(p0 + 4 * kWaveSamples));
+        xfrag[t] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2,
+                                           3, 4, 5, 6, 7);
+      }
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw0, acc[t][0], 0, 0, 0);
       acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw1, acc[t][1], 0, 0, 0);
+    }
+    if constexpr (GATHER) {
+      w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
+      w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
     }
     if (last) {
       // D'[i][jj] of a 16x16 tile sits in lane jj + 16*(i/4), element i%4: acc[t][m][r] of
@@ -687,6 +762,16 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
     step_next(nx2, stride, nitems, io, kptr, k1);
   }
   flush();
+}
+
+__global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer w) {
+  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];  // 8.5 KB per wave
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  if (n_out != n && a.qsrc != nullptr)
+    mfma_body<true>(a, w, wlds, n, n_out);
+  else
+    mfma_body<false>(a, w, wlds, n, n_out);
 }
 
 // --------------------------------------------------------------------------

@@ -55,6 +55,9 @@ struct StepArgs {
   const int32_t* counts = nullptr;
   int layer = 0;                     // n = counts[layer]
   const int32_t* qmap = nullptr;
+  // MFMA FP16 kernel: input position of each output position (compacted layers), or null
+  // (then a compacted layer's tiles are input positions and its stores are compacted)
+  const int32_t* qsrc = nullptr;
   uint8_t* alive_next = nullptr;
   uint8_t* unsat_next = nullptr;     // MFMA FP16 kernel: may be null (flag not needed after this layer)
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
@@ -76,6 +79,7 @@ struct ScanArgs {
   const int32_t* ids_in = nullptr;
   int32_t* ids_out = nullptr;
   int32_t* qmap = nullptr;
+  int32_t* qsrc = nullptr;           // compacting scans: qsrc[q] = input position of output q (may be null)
   int flag_undecided = 0;            // append alive & unsaturated rows
   int32_t* flag_ids = nullptr;
   int32_t* flag_count = nullptr;

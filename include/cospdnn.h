@@ -85,6 +85,10 @@ typedef struct {
                             removes at most this many ppm of the rows entering the layer;
                             compacting shifts every later tile off its 128-byte lines, which
                             slows its stores (default 0: always compact) */
+  int spmm_gather;       /* FP16 MFMA layers after a compacting scan: 1 (default) = work items are
+                            64 output positions whose input columns are gathered, so every store
+                            covers whole 128-byte lines; 0 = items are 64 input positions and the
+                            surviving samples are stored to their compacted positions */
 } cospdnn_opts;
 
 typedef struct {

@@ -56,3 +56,4 @@ def test_options_struct_matches_the_header():
     assert o.batch_concurrent == 2 and o.guard_tail == 8
     assert o.guard_concurrent == 1
     assert o.keep_holes_ppm == 0
+    assert o.spmm_gather == 1

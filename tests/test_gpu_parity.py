@@ -150,24 +150,28 @@ def test_edge_shapes(engine):
 
 @pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
 def test_item_orders_and_grids_agree(engine, kind):
-    """Every MFMA work order and grid size computes the same layers: categories equal to the
-    oracle's, final activations identical across orders (each output is one MFMA chain)."""
+    """Every MFMA work order and grid size, with compacted layers gathered by output position
+    or stored compacted from input positions, computes the same layers: categories equal to
+    the oracle's, final activations identical (each output is one MFMA chain with the same
+    operands)."""
     N, L, S, bias, seed = (2048, 24, 5000, -0.4, 31) if kind == cd.GEN_BANDED else (1024, 24, 5000, -0.3, 32)
     say(f"orders kind={kind}")
     alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
     net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
     ref = None
-    for order in (0, 1, 2, 3):
-        for groups in (1, 4):
-            o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1)
-            out = cd.run([net], o, dump_cap_rows=S)[0]
-            assert out.mfma_layers == L
-            assert np.array_equal(out.categories, np.flatnonzero(alive)), (order, groups)
-            got = np.zeros((S, N), np.float32)
-            got[out.dump_ids] = out.dump_vals
-            if ref is None:
-                ref = got
-            assert np.array_equal(got, ref), (order, groups)
+    for gather in (1, 0):
+        for order in (0, 1, 2, 3):
+            for groups in (1, 4):
+                o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1,
+                                    spmm_gather=gather)
+                out = cd.run([net], o, dump_cap_rows=S)[0]
+                assert out.mfma_layers == L
+                assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, order, groups)
+                got = np.zeros((S, N), np.float32)
+                got[out.dump_ids] = out.dump_vals
+                if ref is None:
+                    ref = got
+                assert np.array_equal(got, ref), (gather, order, groups)
     net.close()
 
 
@@ -255,11 +259,11 @@ def test_kept_layout_matches_compaction(engine, kind):
         base = cd.run([net], cd.default_opts(precision=cd.FP16, use_mfma=use_mfma))[0]
         assert np.array_equal(base.categories, want)
         assert np.array_equal(base.positions[:L], base.live[:L])
-        for ppm, graph, conc in ((1, 0, 1), (1000, 0, 1), (20000, 1, 1), (1000000, 0, 1), (1000000, 1, 1),
-                                 (1000000, 0, 0)):
-            say(f"  use_mfma={use_mfma} ppm={ppm} graph={graph} concurrent={conc}")
+        for ppm, graph, conc, gather in ((1, 0, 1, 1), (1000, 0, 1, 1), (20000, 1, 1, 1), (20000, 0, 1, 0),
+                                         (1000000, 0, 1, 1), (1000000, 1, 1, 0), (1000000, 0, 0, 1)):
+            say(f"  use_mfma={use_mfma} ppm={ppm} graph={graph} concurrent={conc} gather={gather}")
             o = cd.default_opts(precision=cd.FP16, use_mfma=use_mfma, keep_holes_ppm=ppm, use_graph=graph,
-                                guard_concurrent=conc)
+                                guard_concurrent=conc, spmm_gather=gather)
             out = cd.run([net], o)[0]
             assert np.array_equal(out.categories, want), (use_mfma, ppm, graph)
             assert np.array_equal(out.live, base.live), (use_mfma, ppm, graph)
