@@ -16,6 +16,7 @@ from ._lib import (  # noqa: F401
     PLAN_AUTO,
     PLAN_MFMA,
     PLAN_SIMT,
+    PLAN_MFMA_COMPACT,
     TUNE_FEATURES,
     CospdnnError,
     Engine,

@@ -72,8 +72,8 @@ class Result(ctypes.Structure):
     ]
 
 
-PLAN_AUTO, PLAN_MFMA, PLAN_SIMT = 0, 1, 2
-TUNE_FEATURES = 12
+PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT = 0, 1, 2, 3
+TUNE_FEATURES = 13
 
 
 class LayerPlan(ctypes.Structure):

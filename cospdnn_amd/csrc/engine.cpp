@@ -245,6 +245,31 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   for (int l = L - 1; l >= 0; --l) sat_from[l] = sat_from[l + 1] && net->dl[l]->sat_stable;
   const bool few = S_run <= kFewRows;
   auto capg = [&](int g) { return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g; };
+  // Layer l's SpMM: the options' choice, overridden by the network's per-layer plan (FP16)
+  struct LayerLaunch {
+    bool mfma = false, gather = false;
+    int order = 1, groups = 1;
+  };
+  auto layer_launch = [&](int l) {
+    LayerLaunch x;
+    const LayerStore& ls = *net->dl[l];
+    x.mfma = fp16 && ls.has_tiles && cfg.use_mfma;
+    x.order = cfg.order;
+    x.groups = cfg.groups_per_cu;
+    bool compact = false;
+    if (fp16 && !net->plan.empty()) {
+      const cospdnn_layer_plan& pl = net->plan[l];
+      if (pl.kernel == COSPDNN_PLAN_SIMT) x.mfma = false;
+      if (pl.kernel == COSPDNN_PLAN_MFMA || pl.kernel == COSPDNN_PLAN_MFMA_COMPACT) {
+        x.mfma = ls.has_tiles;
+        x.order = pl.order;
+        x.groups = pl.groups_per_cu;
+        compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
+      }
+    }
+    x.gather = x.mfma && cfg.gather && !compact;
+    return x;
+  };
   for (int l = 0; l <= L; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     ScanArgs sa;
@@ -259,7 +284,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.ids_in = rb.id(cur);
     sa.ids_out = rb.id(nxt);
     sa.qmap = rb.qmap.as<int32_t>();
-    sa.qsrc = (fp16 && cfg.gather) ? rb.qsrc.as<int32_t>() : nullptr;
+    // the output-to-input position map, for a layer in gather mode
+    const LayerLaunch lx = l < L ? layer_launch(l) : LayerLaunch();
+    sa.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
     sa.flag_undecided = (l == D) ? 1 : 0;
     sa.flag_ids = rb.flag_ids.as<int32_t>();
     sa.flag_count = rb.flag_count.as<int32_t>();
@@ -301,7 +328,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.counts = rb.counts.as<int32_t>();
     a.layer = l;
     a.qmap = rb.qmap.as<int32_t>();
-    a.qsrc = (fp16 && cfg.gather) ? rb.qsrc.as<int32_t>() : nullptr;
+    a.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
     a.alive_next = rb.f(nxt, 0);
     a.unsat_next = rb.f(nxt, 1);
     const bool band_on = cfg.band > 0.0f && l >= cfg.band_from;
@@ -310,18 +337,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.work = rb.work.as<int32_t>();
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
     const int nblk = ls.view.nblk;
-    // FP16 pass: the network's per-layer plan (if any) overrides the options' choice
-    bool mfma_l = fp16 && ls.has_tiles && cfg.use_mfma;
-    int order_l = cfg.order, groups_l = cfg.groups_per_cu;
-    if (fp16 && !net->plan.empty()) {
-      const cospdnn_layer_plan& pl = net->plan[l];
-      if (pl.kernel == COSPDNN_PLAN_SIMT) mfma_l = false;
-      if (pl.kernel == COSPDNN_PLAN_MFMA) {
-        mfma_l = ls.has_tiles;
-        order_l = pl.order;
-        groups_l = pl.groups_per_cu;
-      }
-    }
+    const bool mfma_l = lx.mfma;
+    const int order_l = lx.order, groups_l = lx.groups;
     if (mfma_l) {
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
@@ -870,7 +887,8 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
     bool m = net->dl[l]->has_tiles && ((o.precision == COSPDNN_FP16 && cfg.use_mfma) || (o.precision == COSPDNN_FP32 && cfg.mfma32));
     if (o.precision == COSPDNN_FP16 && !net->plan.empty() && net->plan[l].kernel != COSPDNN_PLAN_AUTO)
-      m = net->dl[l]->has_tiles && net->plan[l].kernel == COSPDNN_PLAN_MFMA;
+      m = net->dl[l]->has_tiles &&
+          (net->plan[l].kernel == COSPDNN_PLAN_MFMA || net->plan[l].kernel == COSPDNN_PLAN_MFMA_COMPACT);
     mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;

@@ -35,16 +35,18 @@ struct cospdnn_mlp {
 namespace {
 
 const cospdnn_layer_plan kCandidates[] = {
-    {COSPDNN_PLAN_SIMT, 0, 0}, {COSPDNN_PLAN_MFMA, 0, 4}, {COSPDNN_PLAN_MFMA, 1, 2},
-    {COSPDNN_PLAN_MFMA, 1, 4}, {COSPDNN_PLAN_MFMA, 2, 2}, {COSPDNN_PLAN_MFMA, 2, 4},
-    {COSPDNN_PLAN_MFMA, 3, 4},
+    {COSPDNN_PLAN_SIMT, 0, 0},         {COSPDNN_PLAN_MFMA, 0, 4},         {COSPDNN_PLAN_MFMA, 1, 2},
+    {COSPDNN_PLAN_MFMA, 1, 4},         {COSPDNN_PLAN_MFMA, 2, 2},         {COSPDNN_PLAN_MFMA, 2, 4},
+    {COSPDNN_PLAN_MFMA, 3, 4},         {COSPDNN_PLAN_MFMA_COMPACT, 1, 4}, {COSPDNN_PLAN_MFMA_COMPACT, 2, 2},
+    {COSPDNN_PLAN_MFMA_COMPACT, 2, 4}, {COSPDNN_PLAN_MFMA_COMPACT, 3, 4},
 };
 constexpr int kNumCandidates = static_cast<int>(sizeof(kCandidates) / sizeof(kCandidates[0]));
 
 int check_plan(const cospdnn_layer_plan& p) {
-  if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_SIMT) return fail(COSPDNN_ERR_ARG, "plan: kernel");
-  if (p.kernel == COSPDNN_PLAN_MFMA && (p.order < 0 || p.order > 3)) return fail(COSPDNN_ERR_ARG, "plan: order");
-  if (p.kernel == COSPDNN_PLAN_MFMA && (p.groups_per_cu < 1 || p.groups_per_cu > 16))
+  if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_MFMA_COMPACT) return fail(COSPDNN_ERR_ARG, "plan: kernel");
+  const bool mfma = p.kernel == COSPDNN_PLAN_MFMA || p.kernel == COSPDNN_PLAN_MFMA_COMPACT;
+  if (mfma && (p.order < 0 || p.order > 3)) return fail(COSPDNN_ERR_ARG, "plan: order");
+  if (mfma && (p.groups_per_cu < 1 || p.groups_per_cu > 16))
     return fail(COSPDNN_ERR_ARG, "plan: groups_per_cu");
   return COSPDNN_OK;
 }
@@ -58,7 +60,7 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
                     float* f) {
   const LayerStore& ls = *net->dl[l];
   const double rows = static_cast<double>(std::max<int64_t>(live, 0));
-  const bool mfma = c.kernel == COSPDNN_PLAN_MFMA && ls.has_tiles;
+  const bool mfma = (c.kernel == COSPDNN_PLAN_MFMA || c.kernel == COSPDNN_PLAN_MFMA_COMPACT) && ls.has_tiles;
   f[0] = static_cast<float>(std::log2(rows + 1.0) / 16.0);
   f[1] = static_cast<float>(std::log2(static_cast<double>(net->N)) / 16.0);
   f[2] = static_cast<float>(static_cast<double>(ls.nnz) / net->N / 64.0);
@@ -72,6 +74,8 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
   const double removed = static_cast<double>(std::max<int64_t>(rows_prev - live, 0));
   f[11] = static_cast<float>(removed / (removed + 1.0));
+  // gather mode allowed after a compacting scan (plain MFMA) or compacted stores
+  f[12] = mfma && c.kernel == COSPDNN_PLAN_MFMA ? 1.0f : 0.0f;
 }
 
 int mlp_reserve(cospdnn_mlp* m, int n) {
@@ -185,7 +189,7 @@ int cospdnn_tune_exhaustive(cospdnn_net* net, const cospdnn_opts* opts, int reps
   for (int l = 0; l < L; ++l) {
     int best = 0;
     for (int k = 1; k < kNumCandidates; ++k) {
-      if (kCandidates[k].kernel == COSPDNN_PLAN_MFMA && !net->dl[l]->has_tiles) continue;  // ran as SIMT
+      if (kCandidates[k].kernel != COSPDNN_PLAN_SIMT && !net->dl[l]->has_tiles) continue;  // ran as SIMT
       if (table[static_cast<size_t>(k) * L + l] < table[static_cast<size_t>(best) * L + l]) best = k;
     }
     plan[l] = kCandidates[best];
@@ -240,7 +244,7 @@ int cospdnn_tune_samples(cospdnn_net* net, const float* ms_table, const int64_t*
   int n = 0;
   for (int k = 0; k < kNumCandidates; ++k)
     for (int l = 0; l < L; ++l) {
-      if (kCandidates[k].kernel == COSPDNN_PLAN_MFMA && !net->dl[l]->has_tiles) continue;
+      if (kCandidates[k].kernel != COSPDNN_PLAN_SIMT && !net->dl[l]->has_tiles) continue;
       if (n >= cap) return fail(COSPDNN_ERR_ARG, "sample capacity too small");
       layer_features(net, l, live[l], live[l > 0 ? l - 1 : 0], kCandidates[k], X + static_cast<size_t>(n) * COSPDNN_TUNE_FEATURES);
       y[n] = static_cast<float>(std::log(1000.0 * ms_table[static_cast<size_t>(k) * L + l] + 1.0));
@@ -372,7 +376,7 @@ int cospdnn_tune_mlp(cospdnn_net* net, cospdnn_mlp* mlp, const int64_t* live, co
   for (int l = 0; l < L; ++l) {
     int best = 0;
     for (int k = 1; k < kNumCandidates; ++k) {
-      if (kCandidates[k].kernel == COSPDNN_PLAN_MFMA && !net->dl[l]->has_tiles) continue;
+      if (kCandidates[k].kernel != COSPDNN_PLAN_SIMT && !net->dl[l]->has_tiles) continue;
       if (pred[static_cast<size_t>(k) * L + l] < pred[static_cast<size_t>(best) * L + l]) best = k;
     }
     plan[l] = kCandidates[best];

@@ -172,8 +172,10 @@ int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const 
    No counterpart in the reference, which hard-codes its choices (16x16 blocks,
    cospdnn.cpp:94-95; compaction up to layer 21, cospdnn.cu:240; evaluateConfig's
    fixed split, cospdnn_double.cu:3-30).  A plan fixes, for each layer of the FP16
-   pass, the SpMM kernel and its launch shape. */
-enum { COSPDNN_PLAN_AUTO = 0, COSPDNN_PLAN_MFMA = 1, COSPDNN_PLAN_SIMT = 2 };
+   pass, the SpMM kernel and its launch shape.  COSPDNN_PLAN_MFMA_COMPACT is the MFMA
+   kernel with gather mode off for that layer (after a compacting scan its tiles are input
+   positions and the surviving samples are stored compacted, see spmm_gather). */
+enum { COSPDNN_PLAN_AUTO = 0, COSPDNN_PLAN_MFMA = 1, COSPDNN_PLAN_SIMT = 2, COSPDNN_PLAN_MFMA_COMPACT = 3 };
 typedef struct {
   int kernel;        /* COSPDNN_PLAN_*; MFMA falls back to SIMT on a layer without a tile image */
   int order;         /* MFMA work order (as cospdnn_opts.spmm_order) */
@@ -186,7 +188,7 @@ int cospdnn_num_candidates(void);
 int cospdnn_candidate(int k, cospdnn_layer_plan* out);
 /* feature vector of (layer, candidate, rows entering the layer, rows entering the layer
    before it; pass live_rows again for layer 0) for the cost model */
-#define COSPDNN_TUNE_FEATURES 12
+#define COSPDNN_TUNE_FEATURES 13
 int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, int64_t rows_prev,
                                const cospdnn_layer_plan* cand, float* feat);
 /* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer

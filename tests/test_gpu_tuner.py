@@ -92,11 +92,13 @@ def test_plans_keep_categories_exact(engine):
     for plan in [[c] * L for c in cands] + [mixed]:
         net.set_plan(plan)
         out = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
-        want_mfma = sum(1 for p in plan if p[0] == cd.PLAN_MFMA)
+        want_mfma = sum(1 for p in plan if p[0] in (cd.PLAN_MFMA, cd.PLAN_MFMA_COMPACT))
         assert out.mfma_layers == want_mfma
         assert np.array_equal(out.categories, np.flatnonzero(alive)), plan[0]
     with pytest.raises(cd.CospdnnError):
         net.set_plan([(cd.PLAN_MFMA, 4, 4)] * L)  # bad order
+    with pytest.raises(cd.CospdnnError):
+        net.set_plan([(4, 1, 4)] * L)  # bad kernel
     net.set_plan(None)
     net.close()
 
