@@ -692,8 +692,11 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
       // D'[i][jj] of a 16x16 tile sits in lane jj + 16*(i/4), element i%4: acc[t][m][r] of
       // lane j + 16g is output 16m + j of sample 16g + 4t + r.
       const int q0 = __builtin_amdgcn_readfirstlane(q_own);
+      // Gather mode: the tile's outputs are the 64 positions from 64t, always aligned; in the
+      // last tile the positions past the row count get stores (inside the buffer's leading
+      // dimension, used by no later layer) but no flags.
       const bool fast =
-          __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
+          GATHER || __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
       if (fast) {
         // FP16 outputs clamped with med3, transposed through LDS ([32][kOutLd]) so that the
         // stores run 8 lanes x 16 B along each output row
@@ -719,6 +722,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
         uint32_t fl = (ba >> j) & 1u;
         if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
         if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
+        if (GATHER && q_own < 0) fl = 0;
         pend = true;
         pq0 = q0;
         pcb = cur_c;
