@@ -45,6 +45,7 @@ class Opts(ctypes.Structure):
         ("guard_concurrent", ctypes.c_int),
         ("keep_holes_ppm", ctypes.c_int),
         ("spmm_gather", ctypes.c_int),
+        ("input_bits", ctypes.c_int),
     ]
 
 

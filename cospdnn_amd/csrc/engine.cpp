@@ -192,6 +192,7 @@ struct RunCfg {
   int keep_ppm = 0;          // FP16 pass: scans that only drop dead rows keep the layout when
                              // at most this many ppm of the rows are removed (0: never)
   int gather = 0;            // FP16 pass: MFMA layers after a compacting scan gather by output position
+  int bits = 0;              // FP16 pass: an MFMA layer 0 reads the bit-packed copy of a binary input
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -329,6 +330,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.layer = l;
     a.qmap = rb.qmap.as<int32_t>();
     a.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
+    if (l == 0 && lx.mfma && cfg.bits && net->bits.p && input == net->store.as<_Float16>()) {
+      a.xbits = net->bits.as<uint32_t>();
+      a.xbits_ld = net->ld_store / 32;
+    }
     a.alive_next = rb.f(nxt, 0);
     a.unsat_next = rb.f(nxt, 1);
     const bool band_on = cfg.band > 0.0f && l >= cfg.band_from;
@@ -579,6 +584,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.tail = o.guard_tail;
   cfg.keep_ppm = o.precision == COSPDNN_FP16 ? o.keep_holes_ppm : 0;
   cfg.gather = o.precision == COSPDNN_FP16 ? o.spmm_gather : 0;
+  cfg.bits = o.precision == COSPDNN_FP16 ? o.input_bits : 0;
   CALL_TRY(ensure_bufs(net->main, net->S, L, N, cfg.precision, false, e->stream));
   return COSPDNN_OK;
 }
@@ -594,7 +600,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
   return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
-         gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather &&
+         gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather && gc.bits == cfg.bits &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
@@ -616,6 +622,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.groups = cfg.groups_per_cu;
   gc.keep_ppm = cfg.keep_ppm;
   gc.gather = cfg.gather;
+  gc.bits = cfg.bits;
   gc.plan_version = plan_version;
 }
 
@@ -1149,6 +1156,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->guard_concurrent = 1;
   o->keep_holes_ppm = 0;
   o->spmm_gather = 1;
+  o->input_bits = 1;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {
@@ -1412,8 +1420,11 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
     HIP_TRY(hipStreamSynchronize(net->eng->stream));
   }
   net->packed.reset();
+  net->bits.reset();
   net->words = (net->N + 31) / 32;
   if (net->input_binary) {
+    CALL_TRY(dalloc(net->bits, sizeof(uint32_t) * static_cast<size_t>(net->N) * static_cast<size_t>(net->ld_store / 32)));
+    HIP_TRY(launch_pack_cols(net->store.as<_Float16>(), net->ld_store, net->N, net->bits.as<uint32_t>(), net->eng->stream));
     CALL_TRY(dalloc(net->packed, sizeof(uint32_t) * static_cast<size_t>(net->words) * static_cast<size_t>(net->S)));
     HIP_TRY(launch_pack_rows(net->store.as<_Float16>(), net->ld_store, net->N, net->S, net->packed.as<uint32_t>(),
                              net->words, net->eng->stream));

@@ -464,7 +464,23 @@ __device__ __forceinline__ void gather_rows(uint32_t (&d)[16], const _Float16* _
   }
 }
 
-template <bool GATHER>
+// 32 input bits (samples 32h..32h+31 of one row, bit i = sample 32h+i) as 32 FP16 0/1
+// values: dword k holds samples 2k (low half) and 2k+1.
+__device__ __forceinline__ void expand_bits(uint32_t wbits, uint4 (&o)[4]) {
+  uint32_t d[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t b = (wbits >> (2 * k)) & 3u;
+    d[k] = (b & 1u) * 0x3C00u | (b >> 1) * 0x3C000000u;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = make_uint4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+}
+
+// GATHER: tiles of output positions after a compacting scan.  BITS: layer 0 of an identity
+// layout with a binary input, read from its bit-packed neuron-major copy: a step's 32 rows x
+// 64 samples are 256 bytes instead of 4 KB, expanded to FP16 in registers.
+template <bool GATHER, bool BITS>
 __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, float (*wlds)[32 * kStageLd], int n,
                                           int n_out) {
   // ident: no row left the pass in the scan before this layer, every position maps to itself
@@ -530,7 +546,16 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
   uint32_t gd[16];
   int grow1 = 0, gsrc1 = 0;
   const int32_t* __restrict__ qsrc = a.qsrc;
-  if constexpr (GATHER) {
+  // bits mode: lane L covers row L/2 of the chunk, samples 32(L%2)..+31 (one word); the word
+  // of this step and the row id of step nx1
+  uint32_t bw = 0;
+  int brow1 = 0;
+  const int bh = lane & 1;
+  if constexpr (BITS) {
+    brow1 = gidx[(nx1.valid ? nx1.ch : cur.ch) * kChunk + (lane >> 1)];
+    const int brow = gidx[cur.ch * kChunk + (lane >> 1)];
+    bw = a.xbits[static_cast<int64_t>(brow) * a.xbits_ld + cur.t * 2 + bh];
+  } else if constexpr (GATHER) {
     const int pch = nx1.valid ? nx1.ch : cur.ch;
     const int pt = nx1.valid ? nx1.t : cur.t;
     grow1 = gidx[pch * kChunk + (lane & 31)];
@@ -603,7 +628,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     const bool last = cur.ch + 1 == cur.ch_end;
     const int p_own = cur.t * kWaveSamples + lane;
     // the output position of the lane's sample (-1: none)
-    const int q_own = GATHER ? (p_own < n_out ? p_own : -1) : (p_own < n ? qv : -1);
+    const int q_own = GATHER ? (p_own < n_out ? p_own : -1) : (p_own < n ? (BITS ? p_own : qv) : -1);
     const half8_t cw0 = w0, cw1 = w1;
     // output rows j and 16 + j of the block column (slot = row, -1 for padding)
     const int cov0 = cur.c * 32 + j < w.n_out ? cur.c * 32 + j : -1;
@@ -611,7 +636,18 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     const int cur_c = cur.c;
     const int pt = nx1.valid ? nx1.t : cur.t;
     const int pch = nx1.valid ? nx1.ch : cur.ch;
-    if constexpr (GATHER) {
+    if constexpr (BITS) {
+      // this step's bits as FP16 into the tile (row L/2, samples 32(L%2)..+31), then the
+      // next step's word
+      uint4 o[4];
+      expand_bits(bw, o);
+      uint4* dst = reinterpret_cast<uint4*>(tile + (lane >> 1) * kWaveSamples + 32 * bh);
+      dst[0] = o[0];
+      dst[1] = o[1];
+      dst[2] = o[2];
+      dst[3] = o[3];
+      bw = a.xbits[static_cast<int64_t>(brow1) * a.xbits_ld + pt * 2 + bh];
+    } else if constexpr (GATHER) {
       // this step's rows of the lane's sample into the wave's LDS slice, then the next
       // step's gathers (row ids and input positions loaded a step earlier)
 #pragma unroll
@@ -647,7 +683,9 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
         w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
         w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
       }
-      if constexpr (GATHER) {
+      if constexpr (BITS) {
+        brow1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + (lane >> 1)];
+      } else if constexpr (GATHER) {
         // row ids and input positions of step nx2
         grow1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + (lane & 31)];
         gsrc1 = qsrc[min((nx2.valid ? nx2.t : pt) * kWaveSamples + lane, n_out - 1)];
@@ -773,9 +811,11 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
   const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
   if (n_out != n && a.qsrc != nullptr)
-    mfma_body<true>(a, w, wlds, n, n_out);
+    mfma_body<true, false>(a, w, wlds, n, n_out);
+  else if (n_out == n && a.xbits != nullptr)
+    mfma_body<false, true>(a, w, wlds, n, n_out);
   else
-    mfma_body<false>(a, w, wlds, n, n_out);
+    mfma_body<false, false>(a, w, wlds, n, n_out);
 }
 
 // --------------------------------------------------------------------------
@@ -996,6 +1036,28 @@ __global__ void input_alive_kernel(const _Float16* __restrict__ store, int64_t l
   alive[s] = static_cast<uint8_t>(any);
 }
 
+// thread = (neuron row, word): bit i of word w of row n = sample 32w + i of the store is nonzero
+__global__ __launch_bounds__(256) void pack_cols_kernel(const _Float16* __restrict__ store, int64_t ld, int N,
+                                                        uint32_t* __restrict__ bits) {
+  const int64_t wpr = ld / 32;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= wpr * N) return;
+  const int64_t nrow = i / wpr, wd = i - nrow * wpr;
+  const uint4* src = reinterpret_cast<const uint4*>(store + nrow * ld + 32 * wd);
+  uint32_t out = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint4 v = src[q];
+    const uint32_t h[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if ((h[e] & 0x7FFFu) != 0u) out |= 1u << (8 * q + 2 * e);
+      if ((h[e] & 0x7FFF0000u) != 0u) out |= 1u << (8 * q + 2 * e + 1);
+    }
+  }
+  bits[i] = out;
+}
+
 __global__ __launch_bounds__(256) void pack_rows_kernel(const _Float16* __restrict__ store, int64_t ld, int N,
                                                         int64_t S, uint32_t* __restrict__ packed, int words) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1141,6 +1203,13 @@ hipError_t launch_pack_rows(const _Float16* store, int64_t ld, int N, int64_t S,
   if (S <= 0 || words <= 0) return hipSuccess;
   dim3 grid(static_cast<unsigned>((S + 255) / 256), static_cast<unsigned>(words < 256 ? words : 256));
   hipLaunchKernelGGL(pack_rows_kernel, grid, dim3(256), 0, st, store, ld, N, S, packed, words);
+  return hipGetLastError();
+}
+hipError_t launch_pack_cols(const _Float16* store, int64_t ld, int N, uint32_t* bits, hipStream_t st) {
+  const int64_t total = ld / 32 * N;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_cols_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, store, ld, N,
+                     bits);
   return hipGetLastError();
 }
 hipError_t launch_gather_packed(const uint32_t* packed, int words, const int32_t* ids, const int32_t* count,

@@ -89,6 +89,9 @@ typedef struct {
                             64 output positions whose input columns are gathered, so every store
                             covers whole 128-byte lines; 0 = items are 64 input positions and the
                             surviving samples are stored to their compacted positions */
+  int input_bits;        /* FP16 MFMA layer 0 with a binary input: 1 (default) = read the input's
+                            bit-packed copy (1 bit per value instead of 16), expanded to
+                            FP16 in registers; 0 = read the FP16 input */
 } cospdnn_opts;
 
 typedef struct {

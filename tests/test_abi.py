@@ -57,3 +57,4 @@ def test_options_struct_matches_the_header():
     assert o.guard_concurrent == 1
     assert o.keep_holes_ppm == 0
     assert o.spmm_gather == 1
+    assert o.input_bits == 1

@@ -151,27 +151,27 @@ def test_edge_shapes(engine):
 @pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
 def test_item_orders_and_grids_agree(engine, kind):
     """Every MFMA work order and grid size, with compacted layers gathered by output position
-    or stored compacted from input positions, computes the same layers: categories equal to
-    the oracle's, final activations identical (each output is one MFMA chain with the same
-    operands)."""
+    or stored compacted from input positions, and with layer 0 reading the bit-packed or the
+    FP16 input, computes the same layers: categories equal to the oracle's, final activations
+    identical (each output is one MFMA chain with the same operands)."""
     N, L, S, bias, seed = (2048, 24, 5000, -0.4, 31) if kind == cd.GEN_BANDED else (1024, 24, 5000, -0.3, 32)
     say(f"orders kind={kind}")
     alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
     net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
     ref = None
-    for gather in (1, 0):
+    for gather, bits in ((1, 1), (0, 1), (1, 0)):
         for order in (0, 1, 2, 3):
             for groups in (1, 4):
                 o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1,
-                                    spmm_gather=gather)
+                                    spmm_gather=gather, input_bits=bits)
                 out = cd.run([net], o, dump_cap_rows=S)[0]
                 assert out.mfma_layers == L
-                assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, order, groups)
+                assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, bits, order, groups)
                 got = np.zeros((S, N), np.float32)
                 got[out.dump_ids] = out.dump_vals
                 if ref is None:
                     ref = got
-                assert np.array_equal(got, ref), (gather, order, groups)
+                assert np.array_equal(got, ref), (gather, bits, order, groups)
     net.close()
 
 
