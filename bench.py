@@ -29,18 +29,30 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(config, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (profiles/hbm_traffic_<config>_spmm_mfma.md), or None."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}_spmm_mfma.md")
+def profiled_value(name, key, kernel):
+    """The figure `key` of `kernel` from a committed rocprofv3 summary (profiles/<name>),
+    which ends with a machine-readable line `<key> <kernel> <value>`, or (None, None)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         for line in f:
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
                 return float(parts[2]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def profiled_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (profiles/hbm_traffic_<config>_spmm_mfma.md), or None."""
+    return profiled_value(f"hbm_traffic_{config}_spmm_mfma.md", "traffic_bytes_per_launch", kernel)
+
+
+def profiled_mfma_util(config, kernel):
+    """MfmaUtil (%) of `kernel` from the rocprofv3 counter summary of this workload
+    (profiles/mfma_util_<config>_spmm_mfma.md), or None."""
+    return profiled_value(f"mfma_util_{config}_spmm_mfma.md", "mfma_util_percent", kernel)
 
 
 def timed_runs(cd, net, opts, steps, warmup):
@@ -255,6 +267,10 @@ def main():
             + ", ".join(f"{l}: {v:.3e}" for l, v in dy.items()) + f" ({time.perf_counter() - t0:.1f}s)")
 
     traffic, traffic_src = profiled_traffic(args.config, "spmm_mfma_kernel")
+    mfma_util, mfma_src = profiled_mfma_util(args.config, "spmm_mfma_kernel")
+    # 2 flops per live edge: live rows entering each layer x that layer's nonzeros
+    alg_flops = 2.0 * float(sum(int(outs16[0].live[l]) for l in range(spec.L))) * spec.K * spec.N
+    layer_us = np.median(np.stack([o.layer_ms for o in outs16]), axis=0) * 1e3
     line = {
         "metric": METRIC,
         "value": value,
@@ -286,6 +302,10 @@ def main():
                       ("spmm_simt_kernel<half,half>" if outs16[0].mfma_layers == 0 else "spmm_mfma_kernel + spmm_simt_kernel"),
             "avg_launch_ms": spmm_ms / launches,
             "alg_bytes_per_launch": alg_bytes / launches,
+            "alg_flops_per_launch": alg_flops / launches,
+            "mfma_tflops": alg_flops / (spmm_ms * 1e-3) / 1e12,
+            "mfma_util_percent": mfma_util,
+            "mfma_util_source": mfma_src,
         },
         "cpu_baseline": cpu,
         "fp32_gpu": fp32,
@@ -299,6 +319,7 @@ def main():
         "replayed_rows_fp64": int(outs16[0].n_replayed),
         "guard_tiers_ms": float(outs16[0].ms_recompute),
         "live_per_layer": [int(x) for x in outs16[0].live],
+        "spmm_us_by_layer": [round(float(x), 1) for x in layer_us],
         "max_abs_dy_fp16_vs_fp64": max(dy.values()) if dy else None,
         "max_abs_dy_by_layer": {str(k): v for k, v in dy.items()} if dy else None,
         "dy_samples": min(args.dy_samples, spec.S) if dy is not None else 0,
