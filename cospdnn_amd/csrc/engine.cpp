@@ -557,7 +557,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
-  if (o.spmm_order < 0 || o.spmm_order > 3) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..3");
+  if (o.spmm_order < 0 || o.spmm_order > 4) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..4");
   if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
     return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
   if (o.guard_tail < 0) return fail(COSPDNN_ERR_ARG, "guard_tail must be >= 0");

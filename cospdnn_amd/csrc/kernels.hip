@@ -347,7 +347,10 @@ struct StepPos {
 // re-read each input row about K times).  3 XCD-partitioned block-column-major: XCD x takes
 // the block columns [x nblk / 8, (x + 1) nblk / 8) in block-column-major order, so the
 // rows that neighbouring block columns of a banded layer share are gathered through the
-// same L2.  Orders 2 and 3 need a grid of at least 8 work-groups.
+// same L2.  4 XCD-partitioned tile-major: XCD x takes the block columns of order 3, tile after
+// tile, so its share of the tile image stays in its L2 while the tiles stream through (random
+// layers whose tile image is larger than one L2).  Orders 2 to 4 need a grid of at least 8
+// work-groups.
 struct ItemOrder {
   int mode, nblk, ntiles, part, nparts, c0;
 };
@@ -367,7 +370,7 @@ __device__ __forceinline__ void step_first(StepPos& s, int item, int nitems, con
     c = io.c0 + cc;
   } else {
     const int it = item / io.nblk;
-    c = item - it * io.nblk;
+    c = io.c0 + item - it * io.nblk;
     t = io.mode == 2 ? io.part + io.nparts * it : it;
   }
   s.c = c;
@@ -507,6 +510,17 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     const int groups = (static_cast<int>(gridDim.x) - io.part + kXcds - 1) / kXcds;
     io.c0 = io.part * nblk / kXcds;
     nitems = ((io.part + 1) * nblk / kXcds - io.c0) * ntiles;
+    first = (blockIdx.x / kXcds) * kWavesPerGroup + wave;
+    stride = groups * kWavesPerGroup;
+  } else if (a.order == 4) {
+    // this XCD's block columns, tile-major: io.nblk becomes the XCD's share
+    io.nparts = kXcds;
+    io.part = blockIdx.x % kXcds;
+    const int groups = (static_cast<int>(gridDim.x) - io.part + kXcds - 1) / kXcds;
+    io.c0 = io.part * nblk / kXcds;
+    io.nblk = (io.part + 1) * nblk / kXcds - io.c0;
+    if (io.nblk == 0) return;  // fewer block columns than XCDs (wave-uniform)
+    nitems = io.nblk * ntiles;
     first = (blockIdx.x / kXcds) * kWavesPerGroup + wave;
     stride = groups * kWavesPerGroup;
   }

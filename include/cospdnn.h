@@ -69,7 +69,9 @@ typedef struct {
   int dump_layer;        /* >= 0: copy the activations after this layer into the result */
   int spmm_order;        /* MFMA SpMM work order: 0 sample-tile-major, 1 block-column-major
                             (default), 2 sample tiles partitioned over the XCDs (L2 reuse on
-                            random layers), 3 block columns partitioned over the XCDs */
+                            random layers), 3 block columns partitioned over the XCDs,
+                            4 block columns partitioned over the XCDs, sample tiles in turn
+                            (each XCD's share of the tile image stays in its L2) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
   int batch_concurrent;  /* several networks in one cospdnn_run: k > 0 = their main passes
                             run in k concurrent chains on side streams (one hipGraph with
