@@ -299,7 +299,8 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "kernel": "spmm_mfma_kernel" if outs16[0].mfma_layers == spec.L else
-                      ("spmm_simt_kernel<half,half>" if outs16[0].mfma_layers == 0 else "spmm_mfma_kernel + spmm_simt_kernel"),
+                      (f"spmm_mfma_kernel on {outs16[0].mfma_layers} of {spec.L} layers, "
+                       "the few-row or SIMT kernel on the others"),
             "avg_launch_ms": spmm_ms / launches,
             "alg_bytes_per_launch": alg_bytes / launches,
             "alg_flops_per_launch": alg_flops / launches,

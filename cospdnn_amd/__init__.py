@@ -17,6 +17,7 @@ from ._lib import (  # noqa: F401
     PLAN_MFMA,
     PLAN_SIMT,
     PLAN_MFMA_COMPACT,
+    PLAN_ROWS,
     TUNE_FEATURES,
     CospdnnError,
     Engine,

@@ -73,8 +73,8 @@ class Result(ctypes.Structure):
     ]
 
 
-PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT = 0, 1, 2, 3
-TUNE_FEATURES = 13
+PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT, PLAN_ROWS = 0, 1, 2, 3, 4
+TUNE_FEATURES = 14
 
 
 class LayerPlan(ctypes.Structure):

@@ -248,7 +248,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   auto capg = [&](int g) { return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g; };
   // Layer l's SpMM: the options' choice, overridden by the network's per-layer plan (FP16)
   struct LayerLaunch {
-    bool mfma = false, gather = false;
+    bool mfma = false, gather = false, rows = false;
     int order = 1, groups = 1;
   };
   auto layer_launch = [&](int l) {
@@ -261,6 +261,12 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     if (fp16 && !net->plan.empty()) {
       const cospdnn_layer_plan& pl = net->plan[l];
       if (pl.kernel == COSPDNN_PLAN_SIMT) x.mfma = false;
+      if (pl.kernel == COSPDNN_PLAN_ROWS) {
+        x.mfma = false;
+        x.rows = true;
+        x.order = pl.order;
+        x.groups = pl.groups_per_cu;
+      }
       if (pl.kernel == COSPDNN_PLAN_MFMA || pl.kernel == COSPDNN_PLAN_MFMA_COMPACT) {
         x.mfma = ls.has_tiles;
         x.order = pl.order;
@@ -354,6 +360,13 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
       if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
       HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
+    } else if (fp16 && lx.rows) {
+      // groups per CU, but never below what the work split needs (split 0: one wave per
+      // 64-sample group; split 1: one work-group per XCD), whatever the options' grid cap
+      const int64_t need = order_l == 1 ? kXcds : ((S_run + 63) / 64 + 3) / 4;
+      const int grid = static_cast<int>(std::max<int64_t>(need, capg(groups_l * e->cu_count)));
+      a.order = order_l;
+      HIP_TRY(launch_spmm_wave_rows(a, ls.view, grid, st));
     } else if (fp16) {
       if (few)
         HIP_TRY(launch_spmm_simt_rows(a, ls.view, Dtype::F16, Dtype::F16, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));

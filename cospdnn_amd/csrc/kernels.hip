@@ -941,6 +941,111 @@ __global__ __launch_bounds__(256) void spmm_simt_rows_kernel(StepArgs a, DevLaye
   }
 }
 
+// Row-gather FP16 kernel for the FP16 pass (plan kind ROWS), first meant for layers whose
+// live rows fill a handful of 64-sample groups.  A wave takes one 64-sample group at a time
+// (lane = sample) and a strided set of its output slots, two at a time, so every
+// gather is one 128-byte row segment with all lanes on one row, the CSC lists are read
+// once per sample group and no tile image is read at all.  The MFMA kernel instead re-reads its padded tile image (up to 64 times the
+// nonzeros) once per 64-sample tile.  Up to 32 row ids and values of each of the two slots
+// come in with one vector load (lanes 0..31 and 32..63), and all 64 gathers are issued
+// before any is summed (scalar row bases from readlane); the sums run in CSC order with
+// FP32 accumulation, like spmm_simt_kernel<_Float16, _Float16>.  The flags of the wave's
+// samples are OR-ed over its slots and written once per group.  Work split 0 needs a grid of
+// at least ceil(n / 64) waves, split 1 one of at least 8 work-groups.
+constexpr int kRowsBatch = 32;  // rows of one slot loaded before they are summed
+
+// Sample group g of the few-row kernel: the wave computes output slots s0, s0 + stride, ...
+// (two at a time) for the 64 samples of the group and writes their flags.
+__device__ __forceinline__ void rows_group(const StepArgs& a, const DevLayer& w, const _Float16* __restrict__ X,
+                                           _Float16* __restrict__ Y, int64_t ldx, int64_t ldy, int n, int g, int lane,
+                                           int s0, int stride) {
+  const int p = g * 64 + lane;
+  const int q = p < n ? a.qmap[p] : -1;
+  const _Float16* __restrict__ xcol = X + (p < n ? p : 0);
+  const cint_ptr_t out_idx = (cint_ptr_t)w.out_idx;
+  const cint_ptr_t optr = (cint_ptr_t)w.optr;
+  const int nslots = w.nblk * 32;
+  int fl = 0;
+  for (int sa = s0; sa < nslots; sa += 2 * stride) {
+    const int sb = sa + stride;  // the pair's second slot (none past the last)
+    const int out_a = out_idx[sa];
+    const int out_b = sb < nslots ? out_idx[sb] : -1;
+    int ka = out_a >= 0 ? optr[sa] : 0, ka1 = out_a >= 0 ? optr[sa + 1] : 0;
+    int kb = out_b >= 0 ? optr[sb] : 0, kb1 = out_b >= 0 ? optr[sb + 1] : 0;
+    float acc_a = 0.0f, acc_b = 0.0f;
+    while (ka < ka1 || kb < kb1) {  // wave-uniform
+      const int ma = min(kRowsBatch, ka1 - ka), mb = min(kRowsBatch, kb1 - kb);
+      // this batch's row ids and values in lanes 0..31 (slot a) and 32..63 (slot b); row 0
+      // where a list has ended, so that every gather below is issued unconditionally
+      const int lk = lane & 31, lm = lane < 32 ? ma : mb;
+      const int rowv = lk < lm ? w.orow[(lane < 32 ? ka : kb) + lk] : 0;
+      const float valv = lk < lm ? w.oval[(lane < 32 ? ka : kb) + lk] : 0.0f;
+      _Float16 xa[kRowsBatch], xb[kRowsBatch];
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u)
+        xa[u] = xcol[static_cast<int64_t>(__builtin_amdgcn_readlane(rowv, u)) * ldx];
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u)
+        xb[u] = xcol[static_cast<int64_t>(__builtin_amdgcn_readlane(rowv, 32 + u)) * ldx];
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u)
+        if (u < ma)
+          acc_a = madd<float>(acc_a, static_cast<float>(xa[u]),
+                              __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, valv), u)));
+#pragma unroll
+      for (int u = 0; u < kRowsBatch; ++u)
+        if (u < mb)
+          acc_b = madd<float>(acc_b, static_cast<float>(xb[u]),
+                              __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, valv), 32 + u)));
+      ka += ma > 0 ? ma : 0;
+      kb += mb > 0 ? mb : 0;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int out = h == 0 ? out_a : out_b;
+      if (out < 0) continue;
+      const float vv = (h == 0 ? acc_a : acc_b) + a.bias;
+      fl |= (vv > 0.0f) ? 1 : 0;
+      fl |= (vv < 32.0f) ? 2 : 0;
+      fl |= (fabsf(vv) < a.band || fabsf(vv - 32.0f) < a.band) ? 4 : 0;
+      const float y = vv < 0.0f ? 0.0f : (vv > 32.0f ? 32.0f : vv);
+      if (q >= 0) Y[static_cast<int64_t>(out) * ldy + q] = static_cast<_Float16>(y);
+    }
+  }
+  if (q < 0) return;
+  if (fl & 1) a.alive_next[q] = 1;
+  if ((fl & 2) && a.unsat_next != nullptr) a.unsat_next[q] = 1;
+  if ((fl & 4) && a.band_next != nullptr) a.band_next[q] = 1;
+}
+
+// Work split (a.order): 0, wave v takes group v % ng and the slots v / ng + k * (waves per
+// group), every group at once (few groups); 1, the groups are partitioned over the XCDs
+// (blockIdx % 8) and all waves of an XCD stream through its groups one after another, the
+// slots of a group strided over them, so the rows that the group's 32-fold gathers re-read
+// stay in that XCD's L2 (a 64-sample group is N x 128 bytes).
+__global__ __launch_bounds__(256) void spmm_wave_rows_kernel(StepArgs a, DevLayer w) {
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int ng = (n + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+  if (ng == 0) return;
+  const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+  _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
+  if (a.order == 1) {
+    // the launch has at least kXcds work-groups
+    const int x = static_cast<int>(blockIdx.x) % kXcds;
+    const int nwx = (static_cast<int>(gridDim.x) - x + kXcds - 1) / kXcds * 4;  // waves on XCD x
+    const int wv = static_cast<int>(blockIdx.x) / kXcds * 4 + wave;
+    for (int g = x; g < ng; g += kXcds) rows_group(a, w, X, Y, ldx, ldy, n, g, lane, wv, nwx);
+    return;
+  }
+  const int v = static_cast<int>(blockIdx.x) * 4 + wave;
+  const int per = static_cast<int>(gridDim.x) * 4 / ng;  // >= 1: the launch covers ceil(S_run / 64) >= ng waves
+  if (v >= per * ng) return;
+  rows_group(a, w, X, Y, ldx, ldy, n, v % ng, lane, v / ng, per);
+}
+
 // --------------------------------------------------------------------------
 // FP32 MFMA SpMM (v_mfma_f32_16x16x4f32), same work decomposition as the FP16
 // kernel; used for the FP32 guard recompute.  TIn = _Float16 reads the input
@@ -1187,6 +1292,10 @@ hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin
     hipLaunchKernelGGL((spmm_simt_rows_kernel<double, double>), dim3(grid), dim3(256), 0, st, a, w);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(spmm_wave_rows_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
 }
 hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st) {

@@ -135,6 +135,9 @@ hipError_t launch_spmm_simt(const StepArgs& a, const DevLayer& w, Dtype tin, Dty
 // Few-row variant (work item = 8 samples x 32 outputs, one thread per output and sample).
 constexpr int kFewRowsTile = 8;
 hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st);
+// FP16 pass, few live rows: wave = (64-sample group, strided output slots); grid * 4 must be at
+// least ceil(counts[layer] / 64).
+hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,

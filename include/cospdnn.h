@@ -179,12 +179,21 @@ int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const 
    fixed split, cospdnn_double.cu:3-30).  A plan fixes, for each layer of the FP16
    pass, the SpMM kernel and its launch shape.  COSPDNN_PLAN_MFMA_COMPACT is the MFMA
    kernel with gather mode off for that layer (after a compacting scan its tiles are input
-   positions and the surviving samples are stored compacted, see spmm_gather). */
-enum { COSPDNN_PLAN_AUTO = 0, COSPDNN_PLAN_MFMA = 1, COSPDNN_PLAN_SIMT = 2, COSPDNN_PLAN_MFMA_COMPACT = 3 };
+   positions and the surviving samples are stored compacted, see spmm_gather).
+   COSPDNN_PLAN_ROWS is the row-gather kernel (a wave per 64-sample group and output slots,
+   CSC lists, no tile image); its order is the work split: 0 all sample groups at once
+   (few live rows), 1 sample groups partitioned over the XCDs and streamed (L2 reuse). */
+enum {
+  COSPDNN_PLAN_AUTO = 0,
+  COSPDNN_PLAN_MFMA = 1,
+  COSPDNN_PLAN_SIMT = 2,
+  COSPDNN_PLAN_MFMA_COMPACT = 3,
+  COSPDNN_PLAN_ROWS = 4
+};
 typedef struct {
   int kernel;        /* COSPDNN_PLAN_*; MFMA falls back to SIMT on a layer without a tile image */
-  int order;         /* MFMA work order (as cospdnn_opts.spmm_order) */
-  int groups_per_cu; /* MFMA grid (as cospdnn_opts.mfma_groups_per_cu) */
+  int order;         /* MFMA work order (as cospdnn_opts.spmm_order); ROWS work split (0, 1) */
+  int groups_per_cu; /* MFMA and ROWS grid (as cospdnn_opts.mfma_groups_per_cu) */
 } cospdnn_layer_plan;
 /* L entries (copied), or NULL to go back to the options' choice for every layer */
 int cospdnn_net_set_plan(cospdnn_net* net, const cospdnn_layer_plan* plan);
@@ -193,7 +202,7 @@ int cospdnn_num_candidates(void);
 int cospdnn_candidate(int k, cospdnn_layer_plan* out);
 /* feature vector of (layer, candidate, rows entering the layer, rows entering the layer
    before it; pass live_rows again for layer 0) for the cost model */
-#define COSPDNN_TUNE_FEATURES 13
+#define COSPDNN_TUNE_FEATURES 14
 int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, int64_t rows_prev,
                                const cospdnn_layer_plan* cand, float* feat);
 /* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer

@@ -98,7 +98,40 @@ def test_plans_keep_categories_exact(engine):
     with pytest.raises(cd.CospdnnError):
         net.set_plan([(cd.PLAN_MFMA, 5, 4)] * L)  # bad order
     with pytest.raises(cd.CospdnnError):
-        net.set_plan([(4, 1, 4)] * L)  # bad kernel
+        net.set_plan([(5, 1, 4)] * L)  # bad kernel
+    net.set_plan(None)
+    net.close()
+
+
+@pytest.mark.parametrize("N,L,S,bias,kind,seed", [
+    (1024, 12, 1, -0.3, cd.GEN_RANDOM, 41),     # one sample: one partial 64-sample group
+    (2048, 10, 70, -0.4, cd.GEN_BANDED, 42),    # two groups, the second partial
+    (4096, 8, 300, -0.35, cd.GEN_RANDOM, 43),
+    (1024, 20, 3000, -0.3, cd.GEN_RANDOM, 44),  # rows die: compacted positions
+], ids=["S1", "S70-banded", "S300", "S3000"])
+def test_rows_kernel_matches_oracle(engine, N, L, S, bias, kind, seed):
+    """The row-gather kernel (plan kind ROWS, both work splits) on every layer, and
+    alternating with the MFMA kernel: categories equal to the FP64 oracle's, and the
+    activations after layer 3 within the FP16 pass's error bound of the oracle's."""
+    okind = orc.BANDED if kind == cd.GEN_BANDED else orc.RANDOM
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, okind, seed), 0, S)
+    net = make_net(engine, N, L, S, bias, kind, seed)
+    dl = 3
+    _, _, ref = orc.forward_fp64(orc.OracleNet.generated(N, dl + 1, bias, okind, seed), 0, S, final_ids=np.arange(S))
+    for name, plan in [("rows", [(cd.PLAN_ROWS, 0, 4)] * L), ("rows-xcd", [(cd.PLAN_ROWS, 1, 5)] * L),
+                       ("mixed", [(cd.PLAN_ROWS, l % 4 // 2, 1) if l % 2 else (cd.PLAN_MFMA, 1, 4) for l in range(L)])]:
+        net.set_plan(plan)
+        out = cd.run([net], cd.default_opts(precision=cd.FP16, dump_layer=dl), dump_cap_rows=S)[0]
+        d = np.abs(out.dump_vals - ref[out.dump_ids]).max() if len(out.dump_ids) else 0.0
+        say(f"rows kernel N={N} S={S} plan={name} categories={len(out.categories)} flagged={out.n_flagged} "
+            f"max|dY| layer {dl} {d:.3e}")
+        assert out.mfma_layers == sum(1 for p in plan if p[0] == cd.PLAN_MFMA)
+        assert np.array_equal(out.categories, np.flatnonzero(alive)), name
+        assert d < 2e-2, name
+    with pytest.raises(cd.CospdnnError):
+        net.set_plan([(cd.PLAN_ROWS, 0, 0)] * L)  # bad grid
+    with pytest.raises(cd.CospdnnError):
+        net.set_plan([(cd.PLAN_ROWS, 2, 4)] * L)  # bad work split
     net.set_plan(None)
     net.close()
 
