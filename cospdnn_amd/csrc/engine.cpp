@@ -353,7 +353,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     if (mfma_l) {
       // items are strided over the waves of a grid that is resident at once
       const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
-      int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, int64_t{groups_l} * e->cu_count)));
+      // at most the work-groups that are resident at once (items are strided over the grid)
+      const int64_t resident = std::min(groups_l, e->bpc_mfma_resident);
+      int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, resident * e->cu_count)));
       grid = capg(grid);
       if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
       a.order = order_l;
