@@ -153,7 +153,7 @@ def main():
     log(f"[bench] upload {time.perf_counter() - t0:.1f}s, {cu} CUs, {hbm / 2**30:.0f} GiB HBM")
 
     # Autotuning (untimed).  1: exhaustive per-layer search over the launch candidates with
-    # compaction at every scan, and the cost-model MLP (13-64-64-1, trained by the library's
+    # compaction at every scan, and the cost-model MLP (14-64-64-1, trained by the library's
     # HIP kernels on those per-layer timings) picks a plan.  2: the compaction threshold
     # (keep_holes_ppm) on whole forwards with that plan.  3: the launch plan again under that
     # threshold (a kept layout changes which layers see shifted tiles), the MLP retrained on
