@@ -127,8 +127,8 @@ def main():
     ap.add_argument("--dy-samples", type=int, default=1000, help="samples for max |dY| against the FP64 oracle (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(sample sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by samples: each GPU would run this engine on its share)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -7,7 +7,7 @@
 // the other layers, so the exhaustive optimum is the per-layer argmin over runs that apply
 // one candidate to every layer.  The cost model predicts those per-layer times from
 // features; its plan is the per-layer argmin of the predictions, and the exhaustive plan
-// is both its fallback and the judge of its regret.
+// is both its fallback and the yardstick of its regret.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

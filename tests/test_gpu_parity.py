@@ -104,7 +104,7 @@ FULL = ["c1", "c2", "c3"]
 
 @pytest.mark.parametrize("name", FULL)
 def test_full_config_categories(engine, name):
-    """The BASELINE-size networks (60000 samples) against the FP64 oracle."""
+    """The full-size networks (60000 samples) against the FP64 oracle."""
     (spec,) = cd.config_nets(name)
     say(f"full config {name}: N={spec.N} L={spec.L} S={spec.S}")
     alive, live = orc.forward_fp64(orc.OracleNet.generated(spec.N, spec.L, spec.bias, spec.kind, spec.seed), 0, spec.S)

@@ -170,7 +170,7 @@ def test_keep_threshold_tuning(engine):
 
 def test_mlp_plan_regret_on_an_unseen_network(engine):
     """Train the cost model on timings of some networks, plan a network it never saw, and
-    compare with exhaustive tuning (the judge) and with the default choice (no tuner)."""
+    compare with exhaustive tuning (the yardstick) and with the default choice (no tuner)."""
     train_nets = [(1024, 24, 12000, -0.3, cd.GEN_RANDOM, 31), (4096, 24, 12000, -0.35, cd.GEN_RANDOM, 32),
                   (4096, 24, 12000, -0.35, cd.GEN_BANDED, 33), (2048, 24, 20000, -0.3, cd.GEN_RANDOM, 34),
                   (8192, 16, 8000, -0.4, cd.GEN_BANDED, 35)]
