@@ -274,7 +274,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
       }
     }
-    x.gather = x.mfma && cfg.gather && !compact;
+    // the LDS-resident kernel (order 5) takes tiles of input positions and stores compacted
+    x.gather = x.mfma && cfg.gather && !compact && x.order != 5;
     return x;
   };
   for (int l = 0; l <= L; ++l) {
@@ -358,10 +359,21 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, resident * e->cu_count)));
       grid = capg(grid);
       if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
-      a.order = order_l;
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
       if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
-      HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
+      // order 5: the whole input tile in LDS, where the input has at most 1024 rows (the
+      // input store's neurons for layer 0, the previous layer's output slots after it);
+      // otherwise order 1
+      const int x_rows = l == 0 ? net->N : net->dl[l - 1]->view.n_out;
+      if (order_l == 5 && x_rows <= kLdsRowsMax) {
+        a.order = 1;
+        a.x_rows = x_rows;
+        const int lds_grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((S_run + 63) / 64, e->cu_count)));
+        HIP_TRY(launch_spmm_mfma_lds(a, ls.view, lds_grid, st));
+      } else {
+        a.order = order_l == 5 ? 1 : order_l;
+        HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
+      }
     } else if (fp16 && lx.rows) {
       // groups per CU, but never below what the work split needs (split 0: one wave per
       // 64-sample group; split 1: one work-group per XCD), whatever the options' grid cap
@@ -572,7 +584,7 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
-  if (o.spmm_order < 0 || o.spmm_order > 4) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..4");
+  if (o.spmm_order < 0 || o.spmm_order > 5) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..5");
   if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
     return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
   if (o.guard_tail < 0) return fail(COSPDNN_ERR_ARG, "guard_tail must be >= 0");

@@ -833,6 +833,187 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
 }
 
 // --------------------------------------------------------------------------
+// LDS-resident MFMA SpMM (plan order 5) for layers with at most kLdsRows input rows (N <= 1024):
+// a work-group holds a whole 64-sample tile of the input, every row, in LDS (128 KB), loaded
+// once with 16-byte loads, and its 8 waves run all block columns of that tile from it: the X
+// fragments are ds_read_b64_tr_b16 at the gathered rows' LDS addresses, so a random layer's
+// 20-32-fold re-reads of each input row are LDS reads instead of L2 gathers.  Same operand
+// layout, MFMA chain (bias-initialised, K chunks in order) and epilogue arithmetic as the
+// plain mode of spmm_mfma_kernel; tiles of input positions, stores compacted through qmap.
+// Epilogue: the fast path (64 consecutive aligned positions: layers where no row died) goes
+// through a per-wave FP16 staging area as whole 128-byte row segments; otherwise each lane
+// stores its 32 values per sample to that sample's position.
+// --------------------------------------------------------------------------
+constexpr int kLdsRows = kLdsRowsMax;
+constexpr int kLdsWaves = 8;  // 2 per SIMD; one work-group per CU (the tile takes up to 128 KB)
+
+// W fragments and fragment-read rows of one K chunk
+struct LdsChunk {
+  half8_t w0, w1;
+  int rlo, rhi;
+};
+__device__ __forceinline__ void lds_chunk_load(LdsChunk& k, const DevLayer& w, int ch, int lane, int j, int g) {
+  k.rlo = w.gidx[ch * kChunk + g * 8 + (j >> 2)];
+  k.rhi = w.gidx[ch * kChunk + g * 8 + 4 + (j >> 2)];
+  k.w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(ch) * 2 + 0) * 512 + lane * 8);
+  k.w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(ch) * 2 + 1) * 512 + lane * 8);
+}
+// the chunk's X fragments from the LDS tile, then its 8 MFMAs
+__device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsChunk& k, const _Float16* lds_x, int j) {
+  half8_t xfrag[4];
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    const _Float16* plo = lds_x + k.rlo * kWaveSamples + 16 * (j & 3) + 4 * tt;
+    const _Float16* phi = lds_x + k.rhi * kWaveSamples + 16 * (j & 3) + 4 * tt;
+    const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)plo);
+    const fp16x4_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)phi);
+    xfrag[tt] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
+                                        4, 5, 6, 7);
+  }
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    acc[tt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], k.w0, acc[tt][0], 0, 0, 0);
+    acc[tt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], k.w1, acc[tt][1], 0, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepArgs a, DevLayer w) {
+  // [x_rows][64] input tile, then per wave [16][kOutLd] FP16 output staging
+  extern __shared__ __attribute__((aligned(16))) _Float16 lds_x[];
+  constexpr int kThreads = kLdsWaves * 64;
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  const bool ident = n_out == n;
+  const int x_rows = a.x_rows;  // <= kLdsRows, checked by the launcher
+  const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int j = lane & 15, g = lane >> 4;
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
+  const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+  _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
+  const int nblk = w.nblk;
+  const bool full = w.full_blocks != 0;
+  const bool want_unsat = a.unsat_next != nullptr;
+  const bool want_band = a.band_next != nullptr;
+  const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
+  const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
+  _Float16* ostage = lds_x + static_cast<int64_t>(x_rows) * kWaveSamples + wave * 16 * kOutLd;
+
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // the tile: x_rows rows x 64 samples; thread tid moves the 16-byte pieces tid, tid + 512,
+    // ... (piece i: row i / 8, samples 8 (i % 8)..+7); columns past n lie inside the buffer's
+    // leading dimension (a multiple of 64 at least as large as n)
+    __syncthreads();  // the previous tile's fragment reads are done
+    {
+      const _Float16* src = X + static_cast<int64_t>(t) * kWaveSamples;
+      const int pieces = x_rows * 8;
+      for (int i0 = 0; i0 < pieces; i0 += kThreads * 8) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + u * kThreads + tid;
+          if (i < pieces) v[u] = load_stream(src + static_cast<int64_t>(i >> 3) * ldx + (i & 7) * 8);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + u * kThreads + tid;
+          if (i < pieces) *reinterpret_cast<uint4*>(lds_x + (i >> 3) * kWaveSamples + (i & 7) * 8) = v[u];
+        }
+      }
+    }
+    __syncthreads();
+    // this lane's sample (lane L owns sample 64t + L) and its output position
+    const int p_own = t * kWaveSamples + lane;
+    const int q_own = p_own < n ? (ident ? p_own : a.qmap[p_own]) : -1;
+    const int q0 = __builtin_amdgcn_readfirstlane(q_own);
+    const bool fast = __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
+    for (int c = wave; c < nblk; c += kLdsWaves) {
+      float4_t acc[4][2];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) acc[tt][m] = bias4;
+      const int ch0 = kptr[c], ch1 = kptr[c + 1];
+      // K chunks in order, two register sets loaded two chunks ahead (no register that a
+      // load in flight writes is copied)
+      // load them only when the block column has any (all-empty inputs: no chunk, acc = bias)
+      LdsChunk kp, kq;
+      if (ch1 > ch0) {
+        lds_chunk_load(kp, w, ch0, lane, j, g);
+        lds_chunk_load(kq, w, min(ch0 + 1, ch1 - 1), lane, j, g);
+      }
+      for (int ch = ch0; ch < ch1; ch += 2) {
+        lds_chunk_mfma(acc, kp, lds_x, j);
+        lds_chunk_load(kp, w, min(ch + 2, ch1 - 1), lane, j, g);
+        if (ch + 1 < ch1) {
+          lds_chunk_mfma(acc, kq, lds_x, j);
+          lds_chunk_load(kq, w, min(ch + 3, ch1 - 1), lane, j, g);
+        }
+      }
+      // epilogue: acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r
+      const int cov0 = c * 32 + j < w.n_out ? c * 32 + j : -1;
+      const int cov1 = c * 32 + 16 + j < w.n_out ? c * 32 + 16 + j : -1;
+      const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
+      const uint32_t ba = full ? flag_bits<0, true>(acc, true, true, 0.0f) : flag_bits<0, false>(acc, ok0, ok1, 0.0f);
+      uint32_t fl = (ba >> j) & 1u;
+      if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
+      if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
+      if (fast) {
+        // FP16 outputs through the wave's staging area, 16 rows at a time, stored as 8 lanes x
+        // 16 B per row
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          uint32_t pk[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int tt = e / 2, r = 2 * (e % 2);
+            const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r], 0.0f, 32.0f));
+            const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r + 1], 0.0f, 32.0f));
+            pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
+                    (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
+          }
+          uint4* o = reinterpret_cast<uint4*>(ostage + j * kOutLd + 16 * g);
+          o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+          o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          _Float16* ybase = Y + q0 + c8;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const int slot = c * 32 + 16 * m + r8 + 8 * k;
+            const uint4 v = *reinterpret_cast<const uint4*>(ostage + (r8 + 8 * k) * kOutLd + c8);
+            if (full || slot < w.n_out) store_stream(ybase + static_cast<int64_t>(slot) * ldy, v);
+          }
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+      } else {
+        // each value to its sample's position: sample 16g + 4tt + r is owned by that lane
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int qs = __shfl(q_own, 16 * g + 4 * tt + r, 64);
+            if (qs >= 0) {
+              if (cov0 >= 0)
+                Y[static_cast<int64_t>(cov0) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][0][r], 0.0f, 32.0f));
+              if (cov1 >= 0)
+                Y[static_cast<int64_t>(cov1) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][1][r], 0.0f, 32.0f));
+            }
+          }
+      }
+      if (q_own >= 0) {
+        if (fl & 1u) a.alive_next[q_own] = 1;
+        if (fl & 2u) a.unsat_next[q_own] = 1;
+        if (fl & 4u) a.band_next[q_own] = 1;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // SIMT gather SpMM: thread = sample, work item = (tile of 256 samples, block of
 // 32 outputs); each output sums its CSC inputs read straight from global memory.
 // --------------------------------------------------------------------------
@@ -1294,6 +1475,14 @@ hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
+int spmm_mfma_lds_bytes(int x_rows) {
+  return (x_rows * kWaveSamples + kLdsWaves * 16 * kOutLd) * static_cast<int>(sizeof(_Float16));
+}
+hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
+  if (a.x_rows < 1 || a.x_rows > kLdsRows || w.kptr == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(spmm_mfma_lds_kernel, dim3(grid), dim3(kLdsWaves * 64), spmm_mfma_lds_bytes(a.x_rows), st, a, w);
+  return hipGetLastError();
+}
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   hipLaunchKernelGGL(spmm_wave_rows_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
@@ -1364,7 +1553,10 @@ int spmm_mfma32_blocks_per_cu(Dtype tin) {
   return nb;
 }
 
-hipError_t configure_kernels() { return hipSuccess; }
+hipError_t configure_kernels() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_lds_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, spmm_mfma_lds_bytes(kLdsRows));
+}
 
 int spmm_mfma_resident_per_cu() {
   hipFuncAttributes attr;

@@ -67,7 +67,8 @@ struct StepArgs {
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
   float band = 0.0f;
   int32_t* work = nullptr;           // work-item counter of the MFMA kernel (zeroed by the scan)
-  int order = 1;                     // MFMA kernel item order (0..3)
+  int order = 1;                     // MFMA kernel item order (0..4); row-gather kernel work split
+  int x_rows = 0;                    // LDS-resident MFMA kernel: rows of the input buffer (<= 1024)
 };
 
 struct ScanArgs {
@@ -138,6 +139,9 @@ hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin
 // FP16 pass, few live rows: wave = (64-sample group, strided output slots); grid * 4 must be at
 // least ceil(counts[layer] / 64).
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// LDS-resident MFMA kernel (a.x_rows <= kLdsRowsMax input rows, tile image present); any grid.
+constexpr int kLdsRowsMax = 1024;
+hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,

@@ -39,7 +39,7 @@ const cospdnn_layer_plan kCandidates[] = {
     {COSPDNN_PLAN_MFMA, 1, 4},         {COSPDNN_PLAN_MFMA, 2, 2},         {COSPDNN_PLAN_MFMA, 2, 4},
     {COSPDNN_PLAN_MFMA, 3, 4},         {COSPDNN_PLAN_MFMA, 4, 4},         {COSPDNN_PLAN_MFMA_COMPACT, 1, 4},
     {COSPDNN_PLAN_MFMA_COMPACT, 2, 2}, {COSPDNN_PLAN_MFMA_COMPACT, 2, 4}, {COSPDNN_PLAN_MFMA_COMPACT, 3, 4},
-    {COSPDNN_PLAN_MFMA_COMPACT, 4, 4}, {COSPDNN_PLAN_ROWS, 0, 5},
+    {COSPDNN_PLAN_MFMA_COMPACT, 4, 4}, {COSPDNN_PLAN_MFMA, 5, 1},         {COSPDNN_PLAN_ROWS, 0, 5},
     {COSPDNN_PLAN_ROWS, 1, 5},
 };
 constexpr int kNumCandidates = static_cast<int>(sizeof(kCandidates) / sizeof(kCandidates[0]));
@@ -47,7 +47,7 @@ constexpr int kNumCandidates = static_cast<int>(sizeof(kCandidates) / sizeof(kCa
 int check_plan(const cospdnn_layer_plan& p) {
   if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_ROWS) return fail(COSPDNN_ERR_ARG, "plan: kernel");
   const bool mfma = p.kernel == COSPDNN_PLAN_MFMA || p.kernel == COSPDNN_PLAN_MFMA_COMPACT;
-  if (mfma && (p.order < 0 || p.order > 4)) return fail(COSPDNN_ERR_ARG, "plan: order");
+  if (mfma && (p.order < 0 || p.order > 5)) return fail(COSPDNN_ERR_ARG, "plan: order");
   if (p.kernel == COSPDNN_PLAN_ROWS && (p.order < 0 || p.order > 1)) return fail(COSPDNN_ERR_ARG, "plan: rows split");
   if ((mfma || p.kernel == COSPDNN_PLAN_ROWS) && (p.groups_per_cu < 1 || p.groups_per_cu > 16))
     return fail(COSPDNN_ERR_ARG, "plan: groups_per_cu");
@@ -73,7 +73,7 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[6] = mfma ? 1.0f : 0.0f;
   const bool rows_k = c.kernel == COSPDNN_PLAN_ROWS;
   f[7] = mfma || rows_k ? 0.0f : 1.0f;
-  f[8] = mfma ? static_cast<float>(c.order) / 4.0f : 0.0f;
+  f[8] = mfma ? static_cast<float>(c.order) / 5.0f : 0.0f;
   f[9] = mfma ? static_cast<float>(c.groups_per_cu) / 4.0f : 0.0f;
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
   const double removed = static_cast<double>(std::max<int64_t>(rows_prev - live, 0));

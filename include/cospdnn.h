@@ -71,7 +71,9 @@ typedef struct {
                             (default), 2 sample tiles partitioned over the XCDs (L2 reuse on
                             random layers), 3 block columns partitioned over the XCDs,
                             4 block columns partitioned over the XCDs, sample tiles in turn
-                            (each XCD's share of the tile image stays in its L2) */
+                            (each XCD's share of the tile image stays in its L2), 5 a whole
+                            64-sample input tile in LDS per work-group (layers with at most
+                            1024 input rows; order 1 otherwise) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
   int batch_concurrent;  /* several networks in one cospdnn_run: k > 0 = their main passes
                             run in k concurrent chains on side streams (one hipGraph with

@@ -141,7 +141,7 @@ def test_edge_shapes(engine):
         net.upload(cd.default_opts(max_tile_waste=1e9))
         # every MFMA item order: the XCD-partitioned ones on grids smaller than 8 XCDs too
         for prec, order in [(cd.FP32, 1), (cd.FP16, 0), (cd.FP16, 1), (cd.FP16, 2), (cd.FP16, 3), (cd.FP16, 4),
-                            (cd.FP64, 1)]:
+                            (cd.FP16, 5), (cd.FP64, 1)]:
             out = cd.run([net], cd.default_opts(precision=prec, spmm_order=order))[0]
             assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec, order)
             if prec == cd.FP32 and L > 0:
@@ -161,7 +161,7 @@ def test_item_orders_and_grids_agree(engine, kind):
     net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
     ref = None
     for gather, bits in ((1, 1), (0, 1), (1, 0)):
-        for order in (0, 1, 2, 3, 4):
+        for order in (0, 1, 2, 3, 4, 5):
             for groups in (1, 4):
                 o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1,
                                     spmm_gather=gather, input_bits=bits)
