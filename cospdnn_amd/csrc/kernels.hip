@@ -846,6 +846,10 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
 // --------------------------------------------------------------------------
 constexpr int kLdsRows = kLdsRowsMax;
 constexpr int kLdsWaves = 8;  // 2 per SIMD; one work-group per CU (the tile takes up to 128 KB)
+constexpr int kLdsAhead = 4;  // K chunks whose W fragments and rows are in flight per wave
+// LDS row stride of the tile (halves): 136 bytes, so that the gathered rows of one fragment
+// read, whose starts are otherwise all on the same bank, spread over the banks
+constexpr int kLdsLd = kWaveSamples + 4;
 
 // W fragments and fragment-read rows of one K chunk
 struct LdsChunk {
@@ -863,8 +867,8 @@ __device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsC
   half8_t xfrag[4];
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
-    const _Float16* plo = lds_x + k.rlo * kWaveSamples + 16 * (j & 3) + 4 * tt;
-    const _Float16* phi = lds_x + k.rhi * kWaveSamples + 16 * (j & 3) + 4 * tt;
+    const _Float16* plo = lds_x + k.rlo * kLdsLd + 16 * (j & 3) + 4 * tt;
+    const _Float16* phi = lds_x + k.rhi * kLdsLd + 16 * (j & 3) + 4 * tt;
     const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)plo);
     const fp16x4_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)phi);
     xfrag[tt] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
@@ -878,7 +882,7 @@ __device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsC
 }
 
 __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepArgs a, DevLayer w) {
-  // [x_rows][64] input tile, then per wave [16][kOutLd] FP16 output staging
+  // [x_rows][kLdsLd] input tile, then per wave [16][kOutLd] FP16 output staging
   extern __shared__ __attribute__((aligned(16))) _Float16 lds_x[];
   constexpr int kThreads = kLdsWaves * 64;
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
@@ -899,7 +903,7 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
   const bool want_band = a.band_next != nullptr;
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
-  _Float16* ostage = lds_x + static_cast<int64_t>(x_rows) * kWaveSamples + wave * 16 * kOutLd;
+  _Float16* ostage = lds_x + static_cast<int64_t>(x_rows) * kLdsLd + wave * 16 * kOutLd;
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     // the tile: x_rows rows x 64 samples; thread tid moves the 16-byte pieces tid, tid + 512,
@@ -919,7 +923,12 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int i = i0 + u * kThreads + tid;
-          if (i < pieces) *reinterpret_cast<uint4*>(lds_x + (i >> 3) * kWaveSamples + (i & 7) * 8) = v[u];
+          if (i < pieces) {
+            // two 8-byte writes: a row starts on an 8-byte boundary
+            uint2* d = reinterpret_cast<uint2*>(lds_x + (i >> 3) * kLdsLd + (i & 7) * 8);
+            d[0] = make_uint2(v[u].x, v[u].y);
+            d[1] = make_uint2(v[u].z, v[u].w);
+          }
         }
       }
     }
@@ -936,20 +945,21 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
 #pragma unroll
         for (int m = 0; m < 2; ++m) acc[tt][m] = bias4;
       const int ch0 = kptr[c], ch1 = kptr[c + 1];
-      // K chunks in order, two register sets loaded two chunks ahead (no register that a
-      // load in flight writes is copied)
-      // load them only when the block column has any (all-empty inputs: no chunk, acc = bias)
-      LdsChunk kp, kq;
+      // K chunks in order, kLdsAhead register sets loaded that many chunks ahead (the W
+      // fragments come from L2; no register that a load in flight writes is copied); they
+      // are loaded only when the block column has any chunk (all-empty inputs: acc = bias)
+      LdsChunk kc[kLdsAhead];
       if (ch1 > ch0) {
-        lds_chunk_load(kp, w, ch0, lane, j, g);
-        lds_chunk_load(kq, w, min(ch0 + 1, ch1 - 1), lane, j, g);
+#pragma unroll
+        for (int u = 0; u < kLdsAhead; ++u) lds_chunk_load(kc[u], w, min(ch0 + u, ch1 - 1), lane, j, g);
       }
-      for (int ch = ch0; ch < ch1; ch += 2) {
-        lds_chunk_mfma(acc, kp, lds_x, j);
-        lds_chunk_load(kp, w, min(ch + 2, ch1 - 1), lane, j, g);
-        if (ch + 1 < ch1) {
-          lds_chunk_mfma(acc, kq, lds_x, j);
-          lds_chunk_load(kq, w, min(ch + 3, ch1 - 1), lane, j, g);
+      for (int ch = ch0; ch < ch1; ch += kLdsAhead) {
+        // the reloads are unconditional (clamped to the last chunk), so every pass issues the
+        // same loads and the wait before each chunk covers only that chunk's set
+#pragma unroll
+        for (int u = 0; u < kLdsAhead; ++u) {
+          if (ch + u < ch1) lds_chunk_mfma(acc, kc[u], lds_x, j);
+          lds_chunk_load(kc[u], w, min(ch + u + kLdsAhead, ch1 - 1), lane, j, g);
         }
       }
       // epilogue: acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r
@@ -1476,7 +1486,7 @@ hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin
   return hipGetLastError();
 }
 int spmm_mfma_lds_bytes(int x_rows) {
-  return (x_rows * kWaveSamples + kLdsWaves * 16 * kOutLd) * static_cast<int>(sizeof(_Float16));
+  return (x_rows * kLdsLd + kLdsWaves * 16 * kOutLd) * static_cast<int>(sizeof(_Float16));
 }
 hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   if (a.x_rows < 1 || a.x_rows > kLdsRows || w.kptr == nullptr) return hipErrorInvalidValue;
