@@ -903,7 +903,8 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
   const bool want_band = a.band_next != nullptr;
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
-  _Float16* ostage = lds_x + static_cast<int64_t>(x_rows) * kLdsLd + wave * 16 * kOutLd;
+  // staging areas from a 16-byte boundary (x_rows may be odd)
+  _Float16* ostage = lds_x + (static_cast<int64_t>(x_rows) * kLdsLd + 7) / 8 * 8 + wave * 16 * kOutLd;
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     // the tile: x_rows rows x 64 samples; thread tid moves the 16-byte pieces tid, tid + 512,
@@ -1486,7 +1487,7 @@ hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin
   return hipGetLastError();
 }
 int spmm_mfma_lds_bytes(int x_rows) {
-  return (x_rows * kLdsLd + kLdsWaves * 16 * kOutLd) * static_cast<int>(sizeof(_Float16));
+  return ((x_rows * kLdsLd + 7) / 8 * 8 + kLdsWaves * 16 * kOutLd) * static_cast<int>(sizeof(_Float16));
 }
 hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   if (a.x_rows < 1 || a.x_rows > kLdsRows || w.kptr == nullptr) return hipErrorInvalidValue;

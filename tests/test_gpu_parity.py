@@ -121,8 +121,9 @@ def test_full_config_categories(engine, name):
 
 def test_edge_shapes(engine):
     rng = np.random.default_rng(5)
+    # N = 81: an odd row count (the LDS-resident kernel's staging areas start past the tile)
     for N, L, S, bias, K in [(80, 3, 1, -0.3, 20), (80, 3, 257, -0.3, 20), (96, 1, 300, -0.3, 32),
-                             (64, 4, 600, -5.0, 8), (48, 3, 90, 0.2, 6), (128, 0, 50, -0.3, 4)]:
+                             (64, 4, 600, -5.0, 8), (48, 3, 90, 0.2, 6), (128, 0, 50, -0.3, 4), (81, 3, 130, -0.3, 12)]:
         say(f"edge N={N} L={L} S={S} bias={bias}")
         layers = []
         for _ in range(L):
