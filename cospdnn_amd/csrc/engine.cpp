@@ -368,7 +368,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       if (order_l == 5 && x_rows <= kLdsRowsMax) {
         a.order = 1;
         a.x_rows = x_rows;
-        const int lds_grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((S_run + 63) / 64, e->cu_count)));
+        const int lds_grid =
+            capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((S_run + 63) / 64, e->cu_count))));
         HIP_TRY(launch_spmm_mfma_lds(a, ls.view, lds_grid, st));
       } else {
         a.order = order_l == 5 ? 1 : order_l;

@@ -78,8 +78,10 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
   const double removed = static_cast<double>(std::max<int64_t>(rows_prev - live, 0));
   f[11] = static_cast<float>(removed / (removed + 1.0));
-  // gather mode allowed after a compacting scan (plain MFMA) or compacted stores
-  f[12] = mfma && c.kernel == COSPDNN_PLAN_MFMA ? 1.0f : 0.0f;
+  // gather mode allowed after a compacting scan (plain MFMA) or compacted stores; the
+  // LDS-resident kernel (order 5 on inputs of at most kLdsRowsMax rows) always stores compacted
+  const bool lds_resident = c.order == 5 && net->N <= kLdsRowsMax;
+  f[12] = mfma && c.kernel == COSPDNN_PLAN_MFMA && !lds_resident ? 1.0f : 0.0f;
   // the row-gather kernel and its work split (0: all groups at once, 1: groups per XCD)
   f[13] = rows_k ? 0.5f * static_cast<float>(1 + c.order) : 0.0f;
 }

@@ -65,3 +65,28 @@ def test_a_network_twice_in_one_call_is_refused(engine):
     with pytest.raises(cd.CospdnnError):
         cd.run([net, net])
     net.close()
+
+
+def test_c5_six_networks_full_size(engine):
+    """The six uneven networks of the batched workload (N 1024..16384, L 120 or 480, 10000 to
+    60000 samples each, random rows, tile images as the bench uploads them) in one call: one
+    after another, in 2 and 3 concurrent chains, eager and from one hipGraph (replayed
+    twice); every network's categories equal the FP64 oracle's."""
+    specs = cd.config_nets("c5")
+    want = []
+    for sp in specs:
+        say(f"c5 oracle {sp.name}: N={sp.N} L={sp.L} S={sp.S}")
+        want.append(oracle_cats(sp.N, sp.L, sp.S, sp.bias, sp.kind, sp.seed))
+    nets = [make(engine, sp.N, sp.L, sp.S, sp.bias, sp.kind, sp.seed) for sp in specs]
+    for mode, opts in [("one after another", cd.default_opts(batch_concurrent=0)),
+                       ("2 chains, eager", cd.default_opts(batch_concurrent=2)),
+                       ("3 chains, graph", cd.default_opts(batch_concurrent=3, use_graph=1)),
+                       ("3 chains, graph replay", cd.default_opts(batch_concurrent=3, use_graph=1)),
+                       ("2 chains, graph", cd.default_opts(batch_concurrent=2, use_graph=1))]:
+        outs = cd.run(nets, opts)
+        say(f"c5 {mode}: {outs[0].ms_total:.1f} ms, categories {[len(o.categories) for o in outs]}")
+        for sp, o, w in zip(specs, outs, want):
+            assert np.array_equal(o.categories, w), (sp.name, mode)
+            assert o.mfma_layers == sp.L, (sp.name, mode)
+    for n in nets:
+        n.close()
