@@ -34,6 +34,7 @@ from ._lib import (  # noqa: F401
     lib,
     ref_bcsc_to_csc,
     run,
+    run_noalloc,
     tune_exhaustive,
     tune_keep,
     KEEP_CANDIDATES,

@@ -90,7 +90,7 @@ EXPORTS = [
     "cospdnn_engine_create", "cospdnn_engine_destroy", "cospdnn_engine_info",
     "cospdnn_net_create", "cospdnn_net_destroy", "cospdnn_net_set_layer_csc", "cospdnn_net_set_layer_ref_bcsc",
     "cospdnn_net_generate_layers", "cospdnn_net_set_input_dense", "cospdnn_net_set_input_coo",
-    "cospdnn_net_generate_input", "cospdnn_net_upload", "cospdnn_run",
+    "cospdnn_net_generate_input", "cospdnn_net_upload", "cospdnn_run", "cospdnn_run_noalloc",
     "cospdnn_generate_layer", "cospdnn_generate_input_rows", "cospdnn_layer_stats", "cospdnn_layer_tiles_decode",
     "cospdnn_ref_bcsc_to_csc",
     "cospdnn_net_set_plan", "cospdnn_num_candidates", "cospdnn_candidate", "cospdnn_net_layer_features",
@@ -147,6 +147,7 @@ def lib():
         L.cospdnn_net_upload.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts)]
         L.cospdnn_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Opts),
                                   ctypes.POINTER(Result)]
+        L.cospdnn_run_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
         L.cospdnn_generate_layer.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_float, _c_i32p, _c_i32p, _c_f32p]
         L.cospdnn_generate_input_rows.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
@@ -317,6 +318,19 @@ def run(nets, opts=None, dump_cap_rows=0):
     o = opts if opts is not None else default_opts()
     n = len(nets)
     handles = (ctypes.c_void_p * n)(*[x.h for x in nets])
+    return _run_with(nets, dump_cap_rows,
+                     lambda res: _check(lib().cospdnn_run(handles, n, ctypes.byref(o), res), "cospdnn_run"))
+
+
+def run_noalloc(net, opts=None):
+    """One network through the allocation-free entry point (cospdnn_run_noalloc)."""
+    o = opts if opts is not None else default_opts()
+    return _run_with([net], 0, lambda res: _check(lib().cospdnn_run_noalloc(net.h, ctypes.byref(o), res),
+                                                  "cospdnn_run_noalloc"))[0]
+
+
+def _run_with(nets, dump_cap_rows, call):
+    n = len(nets)
     res = (Result * n)()
     keep = []
     for i, net in enumerate(nets):
@@ -336,7 +350,7 @@ def run(nets, opts=None, dump_cap_rows=0):
             res[i].dump_ids = _p(di, _c_i64p)
             res[i].dump_cap_rows = dump_cap_rows
         keep.append((cats, live, lms, dv, di, pos))
-    _check(lib().cospdnn_run(handles, n, ctypes.byref(o), res), "cospdnn_run")
+    call(res)
     outs = []
     for i in range(n):
         cats, live, lms, dv, di, pos = keep[i]

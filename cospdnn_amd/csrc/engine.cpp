@@ -13,11 +13,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -457,8 +459,10 @@ struct ExtraRead {
 // Reads several passes and the extra arrays with one copy kernel into pinned host memory
 // and one synchronisation: per pass its row counts and kept-row counts, then its live ids,
 // retired ids and flagged ids, each list as long as the device-side count says (up to the
-// buffer's capacity).
-int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::vector<ExtraRead>& extras = {}) {
+// buffer's capacity).  plain: no pinned staging (the allocation-free entry point); the list
+// lengths, then the lists, are copied into ordinary host memory, with two synchronisations.
+int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::vector<ExtraRead>& extras = {},
+                bool plain = false) {
   const int L = net->L;
   hipStream_t st = net->eng->stream;
   CopyJobs jobs;
@@ -487,9 +491,26 @@ int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::
     if (w.counts < 0 || w.cats < 0 || w.rets < 0 || w.flags < 0) return fail(COSPDNN_ERR_ARG, "too many result arrays");
   for (int64_t x : xat)
     if (x < 0) return fail(COSPDNN_ERR_ARG, "too many result arrays");
-  CALL_TRY(host_stage(net, static_cast<size_t>(off)));
-  int32_t* h = net->hpin;
-  HIP_TRY(launch_copy_jobs(jobs, h, st));
+  std::vector<int32_t> hplain;
+  int32_t* h = nullptr;
+  if (plain) {
+    hplain.assign(static_cast<size_t>(off), 0);
+    h = hplain.data();
+    for (int j = 0; j < jobs.n; ++j)
+      if (jobs.job[j].len) HIP_TRY(hipMemcpyAsync(h + jobs.job[j].count_at, jobs.job[j].len, sizeof(int32_t),
+                                                  hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int j = 0; j < jobs.n; ++j) {
+      const CopyJob& jb = jobs.job[j];
+      const int32_t n = jb.len ? std::min(h[jb.count_at], jb.cap) : jb.cap;
+      if (jb.len) h[jb.count_at] = n;
+      if (n > 0) HIP_TRY(hipMemcpyAsync(h + jb.off, jb.src, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    }
+  } else {
+    CALL_TRY(host_stage(net, static_cast<size_t>(off)));
+    h = net->hpin;
+    HIP_TRY(launch_copy_jobs(jobs, h, st));
+  }
   HIP_TRY(hipStreamSynchronize(st));
   for (size_t r = 0; r < reqs.size(); ++r) {
     PassOut& out = *reqs[r].out;
@@ -505,8 +526,29 @@ int read_passes(cospdnn_net* net, const std::vector<PassRead>& reqs, const std::
   return COSPDNN_OK;
 }
 
-int read_pass(cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out, bool sort_flags = true) {
-  return read_passes(net, {PassRead{&rb, cat_side, sort_flags, &out}});
+int read_pass(cospdnn_net* net, const RunBufs& rb, int cat_side, PassOut& out, bool sort_flags = true,
+              bool plain = false) {
+  return read_passes(net, {PassRead{&rb, cat_side, sort_flags, &out}}, {}, plain);
+}
+
+// The buffers of a guarded FP16 forward: the main pass, the FP32 tier (rec, up to gcap32
+// rows), the FP64 tier (rep, up to gcap64 rows), the tiers' device-side counters gs
+// {FP32 rows, FP64 rows, overflow} and the FP64 tier's sample ids gids.  The network's own
+// (allocated on demand), or views into the allocation-free entry point's fixed array
+// (`fixed`: tiers are bounded by the capacities, results are read without pinned memory).
+struct GuardWs {
+  RunBufs* main;
+  RunBufs* rec;
+  RunBufs* rep;
+  int32_t* gs;
+  int32_t* gids;
+  int64_t gcap32, gcap64;
+  bool fixed;
+};
+
+GuardWs net_ws(cospdnn_net* net) {
+  return GuardWs{&net->main, &net->rec, &net->rep, net->gs.as<int32_t>(), net->gids.as<int32_t>(), net->gcap32,
+                 net->gcap64, false};
 }
 
 // The FP32 tier: the rows undecided at the checkpoint, layer-major in FP32 (MFMA FP32 on
@@ -544,13 +586,21 @@ RunCfg tier64_cfg() {
 
 // Layer-major FP64 pass over the given original sample ids (bit-identical to the FP64
 // oracle); alive[i] for ids[i].
-int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uint8_t>& alive) {
+int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uint8_t>& alive,
+              const GuardWs* fixed = nullptr) {
   hipStream_t st = net->eng->stream;
   alive.assign(ids.size(), 0);
   if (ids.empty()) return COSPDNN_OK;
   const int32_t nrows = static_cast<int32_t>(ids.size());
-  RunBufs& rb = net->rep;
-  CALL_TRY(ensure_bufs(rb, nrows, net->L, net->N, COSPDNN_FP64, true, st));
+  RunBufs& rb = fixed ? *fixed->rep : net->rep;
+  if (fixed) {
+    if (nrows > fixed->gcap64)
+      return fail(COSPDNN_ERR_ARG, "cospdnn_run_noalloc: " + std::to_string(nrows) +
+                                       " rows left for the FP64 tier, its fixed capacity is " +
+                                       std::to_string(fixed->gcap64) + " (use cospdnn_run)");
+  } else {
+    CALL_TRY(ensure_bufs(rb, nrows, net->L, net->N, COSPDNN_FP64, true, st));
+  }
   HIP_TRY(hipMemcpyAsync(rb.flag_ids.p, ids.data(), sizeof(int32_t) * nrows, hipMemcpyHostToDevice, st));
   HIP_TRY(launch_set_i32(rb.flag_count.as<int32_t>(), nrows, st));
   CALL_TRY(gather_input_rows(net, rb.flag_ids.as<int32_t>(), rb.flag_count.as<int32_t>(), nrows, rb.in16.as<_Float16>(),
@@ -560,7 +610,7 @@ int fp64_rows(cospdnn_net* net, const std::vector<int32_t>& ids, std::vector<uin
   CALL_TRY(enqueue_forward(net, rb, rb.in16.as<_Float16>(), rb.ld, nrows, cfg, false, st, &side));
   HIP_TRY(hipStreamSynchronize(st));
   PassOut p;
-  CALL_TRY(read_pass(net, rb, side, p));
+  CALL_TRY(read_pass(net, rb, side, p, true, fixed != nullptr));
   for (int32_t local : p.cats) alive[local] = 1;
   return COSPDNN_OK;
 }
@@ -577,11 +627,8 @@ int ensure_side_streams(cospdnn_engine* e, int n) {
   return COSPDNN_OK;
 }
 
-// Checks the options against the network, creates its events and main-pass buffers, and
-// fills cfg.  Nothing is enqueued.
-int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
-  cospdnn_engine* e = net->eng;
-  const int N = net->N, L = net->L;
+// Checks the options against the network and fills cfg (no allocation, nothing enqueued).
+int run_config(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
@@ -593,13 +640,6 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
     return fail(COSPDNN_ERR_PRECISION, "FP16 path needs weights and inputs exactly representable in FP16");
   if (o.precision != COSPDNN_FP16 && !net->input_fp16_exact)
     return fail(COSPDNN_ERR_PRECISION, "inputs must be exactly representable in FP16 (binary inputs are)");
-  if (o.layer_timing != 0 && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
-    for (hipEvent_t ev : net->layer_ev) (void)hipEventDestroy(ev);
-    net->layer_ev.assign(2 * static_cast<size_t>(L), nullptr);
-    for (auto& ev : net->layer_ev) HIP_TRY(hipEventCreate(&ev));
-  }
-  for (auto& ev : net->ev)
-    if (!ev) HIP_TRY(hipEventCreate(&ev));
   cfg = RunCfg();
   cfg.precision = o.precision;
   cfg.guard_layer = o.precision == COSPDNN_FP16 ? o.guard_layer : (o.precision == COSPDNN_FP32 ? o.guard_layer32 : -1);
@@ -613,6 +653,21 @@ int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.keep_ppm = o.precision == COSPDNN_FP16 ? o.keep_holes_ppm : 0;
   cfg.gather = o.precision == COSPDNN_FP16 ? o.spmm_gather : 0;
   cfg.bits = o.precision == COSPDNN_FP16 ? o.input_bits : 0;
+  return COSPDNN_OK;
+}
+
+// run_config, then the network's events and main-pass buffers.  Nothing is enqueued.
+int run_prepare(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
+  cospdnn_engine* e = net->eng;
+  const int N = net->N, L = net->L;
+  CALL_TRY(run_config(net, o, cfg));
+  if (o.layer_timing != 0 && net->layer_ev.size() != 2 * static_cast<size_t>(L)) {
+    for (hipEvent_t ev : net->layer_ev) (void)hipEventDestroy(ev);
+    net->layer_ev.assign(2 * static_cast<size_t>(L), nullptr);
+    for (auto& ev : net->layer_ev) HIP_TRY(hipEventCreate(&ev));
+  }
+  for (auto& ev : net->ev)
+    if (!ev) HIP_TRY(hipEventCreate(&ev));
   CALL_TRY(ensure_bufs(net->main, net->S, L, N, cfg.precision, false, e->stream));
   return COSPDNN_OK;
 }
@@ -660,12 +715,12 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
 // into gids, count gs[1]).  Both passes take their row counts from device memory and
 // never synchronise with the host; more rows than the buffers hold set gs[2] (the host
 // then redoes the tiers one after another).
-int enqueue_device_guard(cospdnn_net* net, const cospdnn_opts& o, hipStream_t s1) {
+int enqueue_device_guard(cospdnn_net* net, const GuardWs& ws, const cospdnn_opts& o, hipStream_t s1) {
   const int L = net->L;
-  int32_t* gs = net->gs.as<int32_t>();
-  RunBufs& rec = net->rec;
-  RunBufs& rep = net->rep;
-  CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), gs, net->gcap32, rec.in16.as<_Float16>(), rec.ld, s1));
+  int32_t* gs = ws.gs;
+  RunBufs& rec = *ws.rec;
+  RunBufs& rep = *ws.rep;
+  CALL_TRY(gather_input_rows(net, ws.main->flag_ids.as<int32_t>(), gs, ws.gcap32, rec.in16.as<_Float16>(), rec.ld, s1));
   RunCfg c32 = tier32_cfg(o, L);
   c32.check_from = -1;  // no host checks: the pass runs on the device only
   // When all-32 is a fixed point from its checkpoint on, the checkpoint scan retires or
@@ -680,17 +735,17 @@ int enqueue_device_guard(cospdnn_net* net, const cospdnn_opts& o, hipStream_t s1
   h32.rows_dev = gs;
   h32.overflow = gs + 2;
   int side = 0;
-  CALL_TRY(enqueue_forward(net, rec, rec.in16.as<_Float16>(), rec.ld, net->gcap32, c32, false, s1, &side, &h32));
-  HIP_TRY(launch_compose_ids(net->main.flag_ids.as<int32_t>(), rec.flag_ids.as<int32_t>(), rec.flag_count.as<int32_t>(),
-                             static_cast<int32_t>(net->gcap64), net->gids.as<int32_t>(), gs + 1, gs + 2, s1));
-  CALL_TRY(gather_input_rows(net, net->gids.as<int32_t>(), gs + 1, net->gcap64, rep.in16.as<_Float16>(), rep.ld, s1));
+  CALL_TRY(enqueue_forward(net, rec, rec.in16.as<_Float16>(), rec.ld, ws.gcap32, c32, false, s1, &side, &h32));
+  HIP_TRY(launch_compose_ids(ws.main->flag_ids.as<int32_t>(), rec.flag_ids.as<int32_t>(), rec.flag_count.as<int32_t>(),
+                             static_cast<int32_t>(ws.gcap64), ws.gids, gs + 1, gs + 2, s1));
+  CALL_TRY(gather_input_rows(net, ws.gids, gs + 1, ws.gcap64, rep.in16.as<_Float16>(), rep.ld, s1));
   RunCfg c64 = tier64_cfg();
   c64.check_from = -1;
   c64.grid_cap = net->eng->cu_count;
   PassHooks h64;
   h64.rows_dev = gs + 1;
   h64.overflow = gs + 2;
-  CALL_TRY(enqueue_forward(net, rep, rep.in16.as<_Float16>(), rep.ld, net->gcap64, c64, false, s1, &side, &h64));
+  CALL_TRY(enqueue_forward(net, rep, rep.in16.as<_Float16>(), rep.ld, ws.gcap64, c64, false, s1, &side, &h64));
   return COSPDNN_OK;
 }
 
@@ -707,7 +762,7 @@ int enqueue_main_guarded(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& 
     HIP_TRY(launch_clamp_count(net->main.flag_count.as<int32_t>(), gs, static_cast<int32_t>(net->gcap32), gs + 2, st));
     HIP_TRY(hipEventRecord(net->gev[0], st));
     HIP_TRY(hipStreamWaitEvent(s1, net->gev[0], 0));
-    CALL_TRY(enqueue_device_guard(net, o, s1));
+    CALL_TRY(enqueue_device_guard(net, net_ws(net), o, s1));
     HIP_TRY(hipEventRecord(net->gev[1], s1));
     return COSPDNN_OK;
   };
@@ -771,16 +826,20 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
                   const std::vector<char>& alive, const std::vector<char>& first, int64_t n_flagged,
                   int64_t n_replayed, cospdnn_result* res);
 
-int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int cat_side, cospdnn_result* res) {
+int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int cat_side, cospdnn_result* res,
+               const GuardWs* fixed = nullptr) {
   cospdnn_engine* e = net->eng;
   hipStream_t st = e->stream;
   const int N = net->N, L = net->L;
   const int64_t S = net->S;
+  const bool plain = fixed != nullptr;
+  RunBufs& main = fixed ? *fixed->main : net->main;
+  RunBufs& rec = fixed ? *fixed->rec : net->rec;
   std::vector<char> alive(S, 0), first(S, 0);
   std::vector<int64_t> live(L + 1, -1);
   int64_t n_flagged = 0, n_replayed = 0;
   PassOut p1;
-  CALL_TRY(read_pass(net, net->main, cat_side, p1));
+  CALL_TRY(read_pass(net, main, cat_side, p1, true, plain));
   for (int l = 0; l <= L; ++l) live[l] = p1.counts[L + 2 + l + 1];  // kept rows (holes not counted)
   if (res->positions)
     for (int l = 0; l <= L; ++l) res->positions[l] = p1.counts[l + 1];
@@ -792,16 +851,23 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     // tier 2: the flagged rows again, layer-major in FP32 (MFMA FP32 on tiled layers)
     const int32_t nflag = static_cast<int32_t>(p1.flags.size());
     n_flagged = nflag;
-    CALL_TRY(ensure_bufs(net->rec, nflag, L, N, COSPDNN_FP32, true, st));
-    HIP_TRY(hipMemcpyAsync(net->main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
-    CALL_TRY(gather_input_rows(net, net->main.flag_ids.as<int32_t>(), net->main.flag_count.as<int32_t>(), nflag,
-                               net->rec.in16.as<_Float16>(), net->rec.ld, st));
+    if (fixed) {
+      if (nflag > fixed->gcap32)
+        return fail(COSPDNN_ERR_ARG, "cospdnn_run_noalloc: " + std::to_string(nflag) +
+                                         " rows flagged for the FP32 tier, its fixed capacity is " +
+                                         std::to_string(fixed->gcap32) + " (use cospdnn_run)");
+    } else {
+      CALL_TRY(ensure_bufs(rec, nflag, L, N, COSPDNN_FP32, true, st));
+    }
+    HIP_TRY(hipMemcpyAsync(main.flag_ids.p, p1.flags.data(), sizeof(int32_t) * nflag, hipMemcpyHostToDevice, st));
+    CALL_TRY(gather_input_rows(net, main.flag_ids.as<int32_t>(), main.flag_count.as<int32_t>(), nflag,
+                               rec.in16.as<_Float16>(), rec.ld, st));
     const RunCfg rcfg = tier32_cfg(o, L);
     int rside = 0;
-    CALL_TRY(enqueue_forward(net, net->rec, net->rec.in16.as<_Float16>(), net->rec.ld, nflag, rcfg, false, st, &rside));
+    CALL_TRY(enqueue_forward(net, rec, rec.in16.as<_Float16>(), rec.ld, nflag, rcfg, false, st, &rside));
     HIP_TRY(hipStreamSynchronize(st));
     PassOut p2;
-    CALL_TRY(read_pass(net, net->rec, rside, p2));
+    CALL_TRY(read_pass(net, rec, rside, p2, true, plain));
     for (int32_t id : p1.flags) alive[id] = 0;
     for (int32_t local : p2.cats) alive[p1.flags[local]] = 1;
     for (int32_t local : p2.flags) replay_ids.push_back(p1.flags[local]);
@@ -812,7 +878,7 @@ int run_finish(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, int c
     // tier 3: FP64 replay, bit-identical to the FP64 oracle
     std::sort(replay_ids.begin(), replay_ids.end());
     std::vector<uint8_t> a64;
-    CALL_TRY(fp64_rows(net, replay_ids, a64));
+    CALL_TRY(fp64_rows(net, replay_ids, a64, fixed));
     for (size_t i = 0; i < replay_ids.size(); ++i) alive[replay_ids[i]] = static_cast<char>(a64[i]);
     n_replayed = static_cast<int64_t>(replay_ids.size());
   }
@@ -1020,6 +1086,90 @@ int run_one(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
   return COSPDNN_OK;
 }
 
+// Views of one pass's buffers (as ensure_bufs would allocate them for cap rows) laid out
+// from byte `at` of base, each on a 256-byte boundary; base == nullptr only measures.
+// Returns the end of the last view.
+size_t carve_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, char* base, size_t at) {
+  rb.cap = cap;
+  rb.L = L;
+  rb.ld = round_up(std::max<int64_t>(cap, 1), kTileS) + 64;
+  const size_t ld = static_cast<size_t>(rb.ld);
+  auto take = [&](DevBuf& b, size_t bytes) {
+    at = static_cast<size_t>(round_up(static_cast<int64_t>(at), 256));
+    if (base) b.borrow(base + at, bytes);
+    at += bytes;
+  };
+  take(rb.flags, 6 * ld);
+  take(rb.ids, sizeof(int32_t) * 2 * ld);
+  take(rb.qmap, sizeof(int32_t) * ld);
+  take(rb.qsrc, sizeof(int32_t) * ld);
+  take(rb.counts, sizeof(int32_t) * 2 * (L + 2));
+  take(rb.flag_ids, sizeof(int32_t) * ld);
+  take(rb.flag_count, sizeof(int32_t));
+  take(rb.flag_mark, ld);
+  take(rb.work, sizeof(int32_t));
+  take(rb.ret_ids, sizeof(int32_t) * ld);
+  take(rb.ret_count, sizeof(int32_t));
+  const size_t elems = static_cast<size_t>(N) * ld;
+  DevBuf* act = precision == COSPDNN_FP16 ? rb.act16 : (precision == COSPDNN_FP32 ? rb.act32 : rb.act64);
+  const size_t esz = precision == COSPDNN_FP16 ? 2 : (precision == COSPDNN_FP32 ? 4 : 8);
+  take(act[0], elems * esz);
+  take(act[1], elems * esz);
+  if (need_in16) take(rb.in16, elems * 2);
+  return at;
+}
+
+// The guarded FP16 forward of one network in the fixed array: the main pass, then the FP32
+// and FP64 tiers one after the other (as run_finish does), every buffer a view into the
+// array, the results read back without pinned memory.  Calls on one device are serialised.
+int run_noalloc(cospdnn_net* net, const cospdnn_opts& o, cospdnn_result* res) {
+  cospdnn_engine* e = net->eng;
+  hipStream_t st = e->stream;
+  if (o.precision != COSPDNN_FP16) return fail(COSPDNN_ERR_ARG, "cospdnn_run_noalloc: precision must be COSPDNN_FP16");
+  if (o.layer_timing != 0 || o.dump_layer >= 0)
+    return fail(COSPDNN_ERR_ARG, "cospdnn_run_noalloc: no per-layer timing and no layer dump");
+  RunCfg cfg;
+  CALL_TRY(run_config(net, o, cfg));
+  const int N = net->N, L = net->L;
+  const int64_t S = net->S;
+  void* base = nullptr;
+  size_t cap = 0;
+  CALL_TRY(noalloc_scratch(e->device, &base, &cap));
+  std::lock_guard<std::mutex> lk(noalloc_mutex(e->device));
+  struct Views {
+    RunBufs main, rec, rep;
+  } v;
+  const int64_t gcap32 = std::min<int64_t>(S, std::max<int64_t>(4096, S / 4));
+  const int64_t gcap64 = std::min<int64_t>(gcap32, kFewRows);
+  auto carve = [&](char* b) {
+    size_t at = carve_bufs(v.main, S, L, N, COSPDNN_FP16, false, b, 0);
+    at = carve_bufs(v.rec, gcap32, L, N, COSPDNN_FP32, true, b, at);
+    return carve_bufs(v.rep, gcap64, L, N, COSPDNN_FP64, true, b, at);
+  };
+  const size_t need = carve(nullptr);
+  if (need > cap)
+    return fail(COSPDNN_ERR_ARG, "cospdnn_run_noalloc: the network needs " + std::to_string(need) +
+                                     " bytes of working memory, the fixed array holds " + std::to_string(cap) +
+                                     " (use cospdnn_run)");
+  carve(static_cast<char*>(base));
+  // the array keeps the last call's contents: everything this call uses starts at zero
+  HIP_TRY(hipMemsetAsync(base, 0, need, st));
+  const GuardWs ws{&v.main, &v.rec, &v.rep, nullptr, nullptr, gcap32, gcap64, true};
+  const auto t0 = std::chrono::steady_clock::now();
+  int cat_side = 0;
+  CALL_TRY(enqueue_forward(net, v.main, net->store.as<_Float16>(), net->ld_store, S, cfg, false, st, &cat_side));
+  HIP_TRY(hipStreamSynchronize(st));
+  const auto t1 = std::chrono::steady_clock::now();
+  CALL_TRY(run_finish(net, o, cfg, cat_side, res, &ws));
+  HIP_TRY(hipStreamSynchronize(st));
+  const auto t2 = std::chrono::steady_clock::now();
+  const auto ms = [](std::chrono::steady_clock::duration d) {
+    return static_cast<float>(std::chrono::duration<double, std::milli>(d).count());
+  };
+  set_times(res, ms(t1 - t0), ms(t2 - t0));
+  return COSPDNN_OK;
+}
+
 // Several networks at once: their main passes run in `chains` concurrent chains, each a
 // stream of its own forked from (and joined back to) the engine's stream, either eagerly
 // or from one hipGraph that holds all the branches; then each network's guard tiers.
@@ -1212,6 +1362,10 @@ int cospdnn_engine_create(int device, cospdnn_engine** out) {
   e->bpc_m32h = spmm_mfma32_blocks_per_cu(Dtype::F16);
   e->bpc_m32f = spmm_mfma32_blocks_per_cu(Dtype::F32);
   e->bpc_rows = spmm_simt_rows_blocks_per_cu(Dtype::F64, Dtype::F64);
+  // the device address of the allocation-free entry point's fixed array on this device
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  CALL_TRY(noalloc_scratch(device, &scratch, &scratch_bytes));
   *out = e.release();
   return COSPDNN_OK;
 }
@@ -1479,6 +1633,15 @@ int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, 
   if (n_nets > 1 && o.batch_concurrent > 0) return run_batch(nets, n_nets, o, results);
   for (int i = 0; i < n_nets; ++i) CALL_TRY(run_one(nets[i], o, &results[i]));
   return COSPDNN_OK;
+}
+
+int cospdnn_run_noalloc(cospdnn_net* net, const cospdnn_opts* opts, cospdnn_result* result) {
+  if (!net || !result) return fail(COSPDNN_ERR_ARG, "bad argument");
+  cospdnn_opts o;
+  cospdnn_default_opts(&o);
+  if (opts) o = *opts;
+  HIP_TRY(hipSetDevice(net->eng->device));
+  return run_noalloc(net, o, result);
 }
 
 int cospdnn_generate_layer(uint64_t seed, int kind, int N, int K, int layer, float wval, int32_t* colptr,

@@ -6,6 +6,7 @@
 #include <array>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,14 +38,22 @@ const char* last_error_text();
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool own = true;  // false: a view into memory the buffer does not own (never freed here)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { reset(); }
   void reset() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    own = true;
+  }
+  void borrow(void* q, size_t n) {
+    reset();
+    p = q;
+    bytes = n;
+    own = false;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
@@ -92,6 +101,12 @@ struct RunBufs {
 };
 
 int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need_in16, hipStream_t st);
+
+// The fixed device array behind cospdnn_run_noalloc (noalloc.hip): its address on `device`,
+// looked up once per device (with that device current) and kept, and its size in bytes.
+int noalloc_scratch(int device, void** ptr, size_t* bytes);
+// Serialises the calls that work in one device's fixed array.
+std::mutex& noalloc_mutex(int device);
 
 }  // namespace cospdnn_detail
 

@@ -158,6 +158,23 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts);
 /* runs the networks one after another on the engine's stream; results[i] for nets[i] */
 int cospdnn_run(cospdnn_net* const* nets, int n_nets, const cospdnn_opts* opts, cospdnn_result* results);
 
+/* Allocation-free variant of cospdnn_run for one uploaded network.  It has no counterpart in
+   the reference; it is for callers that cannot allocate memory once they are running (serving
+   loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree, hipHostMalloc or
+   other device or pinned allocation; host-side bookkeeping uses ordinary heap memory.
+   Every working buffer (activations, per-sample flags and position maps, the FP32 and FP64
+   guard tiers) is a view into one fixed device array of COSPDNN_NOALLOC_SCRATCH_BYTES that each
+   GPU the library runs on holds until the process ends; a network that needs more is refused
+   with COSPDNN_ERR_ARG (about 4*N*S bytes for the main pass plus the tiers: N = 1024 fits up to
+   some 35000 samples).  Same results as cospdnn_run (precision COSPDNN_FP16 only; the guard
+   tiers run after the main pass, one after the other; no per-layer timing, no layer dump,
+   use_graph ignored); COSPDNN_ERR_ARG also when more rows need the FP32 tier than
+   max(4096, S/4) or the FP64 tier than 512.  Calls share their GPU's array and keep its last
+   contents: each call clears what it uses, and calls on one device are serialised inside the
+   library (calls overlapping on one GPU would otherwise race).  ms_total is host wall time. */
+#define COSPDNN_NOALLOC_SCRATCH_BYTES (256ll << 20)
+int cospdnn_run_noalloc(cospdnn_net* net, const cospdnn_opts* opts, cospdnn_result* result);
+
 /* host-only helpers (no device needed) */
 int cospdnn_generate_layer(uint64_t seed, int kind, int N, int K, int layer, float wval, int32_t* colptr,
                            int32_t* rows, float* vals);
