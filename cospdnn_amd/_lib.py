@@ -46,6 +46,7 @@ class Opts(ctypes.Structure):
         ("keep_holes_ppm", ctypes.c_int),
         ("spmm_gather", ctypes.c_int),
         ("input_bits", ctypes.c_int),
+        ("weight_mask", ctypes.c_int),
     ]
 
 

@@ -155,6 +155,11 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
     CALL_TRY(dupload(ls->kptr, bl.kptr));
     CALL_TRY(dupload(ls->gidx, bl.gidx));
     CALL_TRY(dupload(ls->wt, bl.wt));
+    if (bl.has_mask) {
+      CALL_TRY(dupload(ls->wmask, bl.wmask));
+      ls->view.wmask = ls->wmask.as<uint16_t>();
+      ls->view.wmask_val = bl.wmask_val;
+    }
     if (bl.has_tiles32) {
       CALL_TRY(dupload(ls->wt32, bl.wt32));
       ls->view.wt32 = ls->wt32.as<float>();
@@ -163,6 +168,22 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
     ls->view.kptr = ls->kptr.as<int32_t>();
     ls->view.gidx = ls->gidx.as<int32_t>();
     ls->view.wt = ls->wt.as<_Float16>();
+    if (bl.has_band) {
+      // the band image: FP16 fragments only where the layer has no mask (the mask words hold it)
+      CALL_TRY(dupload(ls->bptr, bl.bptr));
+      CALL_TRY(dupload(ls->bgidx, bl.bgidx));
+      ls->view.bands = bl.bands;
+      ls->view.band_nchunk = static_cast<int>(bl.band_nchunk);
+      ls->view.bptr = ls->bptr.as<int32_t>();
+      ls->view.bgidx = ls->bgidx.as<int32_t>();
+      if (!bl.bmask.empty()) {
+        CALL_TRY(dupload(ls->bmask, bl.bmask));
+        ls->view.bmask = ls->bmask.as<uint16_t>();
+      } else {
+        CALL_TRY(dupload(ls->bwt, bl.bwt));
+        ls->view.bwt = ls->bwt.as<_Float16>();
+      }
+    }
     ls->view.one_chunk = bl.nchunk == bl.nblk ? 1 : 0;
     ls->view.full_blocks = std::all_of(bl.out_idx.begin(), bl.out_idx.end(), [](int32_t o) { return o >= 0; }) ? 1 : 0;
   }
@@ -195,10 +216,16 @@ struct RunCfg {
                              // at most this many ppm of the rows are removed (0: never)
   int gather = 0;            // FP16 pass: MFMA layers after a compacting scan gather by output position
   int bits = 0;              // FP16 pass: an MFMA layer 0 reads the bit-packed copy of a binary input
+  int wmask = 0;             // FP16 pass: MFMA layers read the mask image where the layer has one
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
 constexpr double kTileBudget = 0.4;  // MFMA tile images: at most this share of the HBM per engine
+// Band images (order 6) for layers whose tile image wastes more than this (random rows) and
+// whose input has at most kBandMaxN rows: past that a group of 512 outputs uses too few of a
+// tile's rows for streaming all of them through LDS to beat gathering the used ones.
+constexpr double kBandMinWaste = 4.0;
+constexpr int kBandMaxN = 16384;
 
 int grid_rows(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   const int64_t items = (S_run + kFewRowsTile - 1) / kFewRowsTile * static_cast<int64_t>(nblk);
@@ -276,8 +303,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
       }
     }
-    // the LDS-resident kernel (order 5) takes tiles of input positions and stores compacted
-    x.gather = x.mfma && cfg.gather && !compact && x.order != 5;
+    // the LDS-resident (order 5) and band (order 6) kernels take tiles of input positions and
+    // store compacted
+    x.gather = x.mfma && cfg.gather && !compact && x.order != 5 && x.order != 6;
     return x;
   };
   for (int l = 0; l <= L; ++l) {
@@ -328,6 +356,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     }
 
     const LayerStore& ls = *net->dl[l];
+    // the layer's images; the mask image only where the pass reads it
+    DevLayer view = ls.view;
+    if (!(fp16 && cfg.wmask)) view.wmask = view.bmask = nullptr;
     StepArgs a;
     const bool fp64 = cfg.precision == COSPDNN_FP64;
     auto side_ptr = [&](int sd) -> void* { return fp16 ? rb.act16[sd].p : (fp64 ? rb.act64[sd].p : rb.act32[sd].p); };
@@ -350,7 +381,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.band = band_on ? cfg.band : 0.0f;
     a.work = rb.work.as<int32_t>();
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
-    const int nblk = ls.view.nblk;
+    const int nblk = view.nblk;
     const bool mfma_l = lx.mfma;
     const int order_l = lx.order, groups_l = lx.groups;
     if (mfma_l) {
@@ -367,15 +398,21 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       // input store's neurons for layer 0, the previous layer's output slots after it);
       // otherwise order 1
       const int x_rows = l == 0 ? net->N : net->dl[l - 1]->view.n_out;
-      if (order_l == 5 && x_rows <= kLdsRowsMax) {
+      const bool band_ok = view.bptr != nullptr && (view.bmask != nullptr || view.bwt != nullptr);
+      if (order_l == 6 && band_ok) {
+        a.order = 1;
+        a.x_rows = x_rows;
+        const int band_grid = std::max(kXcds, capg(e->cu_count));
+        HIP_TRY(launch_spmm_mfma_band(a, view, band_grid, st));
+      } else if (order_l == 5 && x_rows <= kLdsRowsMax) {
         a.order = 1;
         a.x_rows = x_rows;
         const int lds_grid =
             capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((S_run + 63) / 64, e->cu_count))));
-        HIP_TRY(launch_spmm_mfma_lds(a, ls.view, lds_grid, st));
+        HIP_TRY(launch_spmm_mfma_lds(a, view, lds_grid, st));
       } else {
-        a.order = order_l == 5 ? 1 : order_l;
-        HIP_TRY(launch_spmm_mfma(a, ls.view, grid, st));
+        a.order = order_l >= 5 ? 1 : order_l;
+        HIP_TRY(launch_spmm_mfma(a, view, grid, st));
       }
     } else if (fp16 && lx.rows) {
       // groups per CU, but never below what the work split needs (split 0: one wave per
@@ -383,28 +420,28 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       const int64_t need = order_l == 1 ? kXcds : ((S_run + 63) / 64 + 3) / 4;
       const int grid = static_cast<int>(std::max<int64_t>(need, capg(groups_l * e->cu_count)));
       a.order = order_l;
-      HIP_TRY(launch_spmm_wave_rows(a, ls.view, grid, st));
+      HIP_TRY(launch_spmm_wave_rows(a, view, grid, st));
     } else if (fp16) {
       if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, ls.view, Dtype::F16, Dtype::F16, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
+        HIP_TRY(launch_spmm_simt_rows(a, view, Dtype::F16, Dtype::F16, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
       else
-        HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F16, capg(grid_for(e, e->bpc_simt16, S_run, nblk)), st));
+        HIP_TRY(launch_spmm_simt(a, view, Dtype::F16, Dtype::F16, capg(grid_for(e, e->bpc_simt16, S_run, nblk)), st));
     } else if (fp64) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F64;
       if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F64, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
+        HIP_TRY(launch_spmm_simt_rows(a, view, tin, Dtype::F64, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
       else
-        HIP_TRY(launch_spmm_simt(a, ls.view, tin, Dtype::F64, capg(grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk)), st));
+        HIP_TRY(launch_spmm_simt(a, view, tin, Dtype::F64, capg(grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk)), st));
     } else if (few && !(ls.has_tiles32 && cfg.mfma32)) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_simt_rows(a, ls.view, tin, Dtype::F32, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
+      HIP_TRY(launch_spmm_simt_rows(a, view, tin, Dtype::F32, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
     } else if (ls.has_tiles32 && cfg.mfma32) {
       const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_mfma32(a, ls.view, tin, capg(grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk)), st));
+      HIP_TRY(launch_spmm_mfma32(a, view, tin, capg(grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk)), st));
     } else if (l == 0) {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F16, Dtype::F32, capg(grid_for(e, e->bpc_simt1632, S_run, nblk)), st));
+      HIP_TRY(launch_spmm_simt(a, view, Dtype::F16, Dtype::F32, capg(grid_for(e, e->bpc_simt1632, S_run, nblk)), st));
     } else {
-      HIP_TRY(launch_spmm_simt(a, ls.view, Dtype::F32, Dtype::F32, capg(grid_for(e, e->bpc_simt32, S_run, nblk)), st));
+      HIP_TRY(launch_spmm_simt(a, view, Dtype::F32, Dtype::F32, capg(grid_for(e, e->bpc_simt32, S_run, nblk)), st));
     }
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
   }
@@ -632,7 +669,7 @@ int run_config(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   if (!net->uploaded) return fail(COSPDNN_ERR_STATE, "network not uploaded");
   if (o.precision != COSPDNN_FP16 && o.precision != COSPDNN_FP32 && o.precision != COSPDNN_FP64)
     return fail(COSPDNN_ERR_ARG, "precision");
-  if (o.spmm_order < 0 || o.spmm_order > 5) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..5");
+  if (o.spmm_order < 0 || o.spmm_order > 6) return fail(COSPDNN_ERR_ARG, "spmm_order must be 0..6");
   if (o.mfma_groups_per_cu < 1 || o.mfma_groups_per_cu > 16)
     return fail(COSPDNN_ERR_ARG, "mfma_groups_per_cu must be in 1..16");
   if (o.guard_tail < 0) return fail(COSPDNN_ERR_ARG, "guard_tail must be >= 0");
@@ -653,6 +690,7 @@ int run_config(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.keep_ppm = o.precision == COSPDNN_FP16 ? o.keep_holes_ppm : 0;
   cfg.gather = o.precision == COSPDNN_FP16 ? o.spmm_gather : 0;
   cfg.bits = o.precision == COSPDNN_FP16 ? o.input_bits : 0;
+  cfg.wmask = o.precision == COSPDNN_FP16 ? o.weight_mask : 0;
   return COSPDNN_OK;
 }
 
@@ -683,7 +721,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
   return gc.exec && gc.precision == cfg.precision && gc.guard_layer == cfg.guard_layer && gc.band == cfg.band &&
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
-         gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather && gc.bits == cfg.bits &&
+         gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather && gc.bits == cfg.bits && gc.wmask == cfg.wmask &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
@@ -706,6 +744,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.keep_ppm = cfg.keep_ppm;
   gc.gather = cfg.gather;
   gc.bits = cfg.bits;
+  gc.wmask = cfg.wmask;
   gc.plan_version = plan_version;
 }
 
@@ -986,7 +1025,9 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
   for (int l = 0; l < L; ++l) {
     const double lv = live[l] >= 0 ? static_cast<double>(live[l]) : static_cast<double>(S);
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
-    bool m = net->dl[l]->has_tiles && ((o.precision == COSPDNN_FP16 && cfg.use_mfma) || (o.precision == COSPDNN_FP32 && cfg.mfma32));
+    // the FP32 path takes the FP32 MFMA kernel only where the layer has an FP32 tile image
+    bool m = (o.precision == COSPDNN_FP16 && cfg.use_mfma && net->dl[l]->has_tiles) ||
+             (o.precision == COSPDNN_FP32 && cfg.mfma32 && net->dl[l]->has_tiles32);
     if (o.precision == COSPDNN_FP16 && !net->plan.empty() && net->plan[l].kernel != COSPDNN_PLAN_AUTO)
       m = net->dl[l]->has_tiles &&
           (net->plan[l].kernel == COSPDNN_PLAN_MFMA || net->plan[l].kernel == COSPDNN_PLAN_MFMA_COMPACT);
@@ -1335,6 +1376,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->keep_holes_ppm = 0;
   o->spmm_gather = 1;
   o->input_bits = 1;
+  o->weight_mask = 1;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {
@@ -1566,9 +1608,10 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
         BlockedLayer& b = bl[l - l0];
         if (b.has_tiles) {
           const int64_t tb = static_cast<int64_t>(b.wt.size()) * 2 + static_cast<int64_t>(b.wt32.size()) * 4 +
-                             static_cast<int64_t>(b.gidx.size()) * 4;
+                             static_cast<int64_t>(b.gidx.size()) * 4 + static_cast<int64_t>(b.wmask.size()) * 2;
           if (static_cast<double>(net->eng->tile_bytes + tb) > kTileBudget * static_cast<double>(net->eng->hbm_bytes)) {
-            b.has_tiles = b.has_tiles32 = false;
+            b.has_tiles = b.has_tiles32 = b.has_mask = false;
+            b.wmask.clear();
             b.wt.clear();
             b.wt32.clear();
             b.gidx.clear();
@@ -1579,11 +1622,29 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
           }
         }
       }
-      // inputs of layer l through the row order of layer l-1 (rows of the input store: neurons)
+      // inputs of layer l through the row order of layer l-1 (rows of the input store: neurons);
+      // then the band image (bands are ranges of those rows) of the random layers it serves
       th.clear();
-      for (int l = std::max(l0, 1); l < l1; ++l)
-        th.emplace_back([&, l]() { remap_inputs(bl[l - l0], l > l0 ? bl[l - 1 - l0].out_idx : carry, net->N); });
+      for (int l = l0; l < l1; ++l)
+        th.emplace_back([&, l]() {
+          BlockedLayer& b = bl[l - l0];
+          if (l >= 1) remap_inputs(b, l > l0 ? bl[l - 1 - l0].out_idx : carry, net->N);
+          if (b.has_tiles && b.tile_waste > kBandMinWaste && net->N <= kBandMaxN) build_band_image(b, net->N);
+        });
       for (auto& t : th) t.join();
+      for (int l = l0; l < l1; ++l) {
+        BlockedLayer& b = bl[l - l0];
+        if (!b.has_band) continue;
+        const int64_t bb = static_cast<int64_t>(b.bgidx.size()) * 4 + static_cast<int64_t>(b.bptr.size()) * 4 +
+                           (b.bmask.empty() ? static_cast<int64_t>(b.bwt.size()) * 2
+                                            : static_cast<int64_t>(b.bmask.size()) * 2);
+        if (static_cast<double>(net->eng->tile_bytes + bb) > kTileBudget * static_cast<double>(net->eng->hbm_bytes)) {
+          b.has_band = false;
+        } else {
+          net->eng->tile_bytes += bb;
+          net->tile_bytes += bb;
+        }
+      }
       for (int l = l0; l < l1; ++l) {
         CALL_TRY(upload_layer(net, l, bl[l - l0], stable[l - l0] != 0));
         if (!net->gen_layers) net->layers[l] = CscLayer();

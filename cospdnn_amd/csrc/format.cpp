@@ -82,6 +82,34 @@ bool all32_fixed_point(const CscLayer& w, int N, float bias) {
   return true;
 }
 
+namespace {
+
+// The mask image of a tile image whose nonzero elements all hold one value (see format.hpp).
+void build_mask_image(BlockedLayer& out) {
+  out.has_mask = false;
+  out.wmask.clear();
+  uint16_t val = 0;
+  for (uint16_t h : out.wt) {
+    if (h == 0) continue;
+    if (val == 0) val = h;
+    if (h != val) return;
+  }
+  out.wmask.assign(static_cast<size_t>(out.nchunk) * 64, 0);
+  for (int64_t ch = 0; ch < out.nchunk; ++ch)
+    for (int lane = 0; lane < 64; ++lane) {
+      uint16_t bits = 0;
+      for (int m = 0; m < 2; ++m)
+        for (int e = 0; e < 8; ++e)
+          if (out.wt[((static_cast<size_t>(ch) * 2 + m) * 64 + lane) * 8 + e] != 0)
+            bits |= static_cast<uint16_t>(1u << ((e & 1) * 8 + 4 * m + e / 2));
+      out.wmask[static_cast<size_t>(ch) * 64 + lane] = bits;
+    }
+  out.wmask_val = val;
+  out.has_mask = true;
+}
+
+}  // namespace
+
 bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) {
   if (N <= 0 || static_cast<int>(w.colptr.size()) != N + 1 || w.colptr[0] != 0) return false;
   const int64_t nnz = w.colptr[N];
@@ -218,7 +246,85 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
     ch0 += nch;
   }
   out.kptr[out.nblk] = static_cast<int32_t>(ch0);
+  build_mask_image(out);
   return true;
+}
+
+void build_band_image(BlockedLayer& bl, int in_rows) {
+  const int nb = std::max(1, (in_rows + kBandRows - 1) / kBandRows);
+  bl.bands = nb;
+  bl.bptr.assign(static_cast<size_t>(bl.nblk) * (nb + 1), 0);
+  bl.bgidx.clear();
+  bl.bwt.clear();
+  bl.bmask.clear();
+  std::vector<int32_t> u;
+  std::vector<float> dense;  // [kpad][32] of one band segment
+  int64_t ch0 = 0;
+  for (int c = 0; c < bl.nblk; ++c) {
+    u.clear();
+    for (int o = 0; o < kBlockOut; ++o) {
+      const size_t slot = static_cast<size_t>(c) * kBlockOut + o;
+      if (bl.out_idx[slot] < 0) continue;
+      for (int32_t q = bl.optr[slot]; q < bl.optr[slot + 1]; ++q) u.push_back(bl.orow[q]);
+    }
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    size_t at = 0;
+    for (int b = 0; b < nb; ++b) {
+      bl.bptr[static_cast<size_t>(c) * (nb + 1) + b] = static_cast<int32_t>(ch0);
+      const size_t beg = at;
+      while (at < u.size() && u[at] < (b + 1) * kBandRows) ++at;
+      const int rows = static_cast<int>(at - beg);
+      const int nch = (rows + kChunkK - 1) / kChunkK;
+      if (nch == 0) continue;
+      const int kpad = nch * kChunkK;
+      for (int k = 0; k < kpad; ++k)
+        bl.bgidx.push_back((k < rows ? u[beg + k] : u[beg]) - b * kBandRows);
+      dense.assign(static_cast<size_t>(kpad) * kBlockOut, 0.0f);
+      for (int o = 0; o < kBlockOut; ++o) {
+        const size_t slot = static_cast<size_t>(c) * kBlockOut + o;
+        if (bl.out_idx[slot] < 0) continue;
+        for (int32_t q = bl.optr[slot]; q < bl.optr[slot + 1]; ++q) {
+          const int32_t r = bl.orow[q];
+          if (r < b * kBandRows || r >= (b + 1) * kBandRows) continue;
+          const int kpos = static_cast<int>(std::lower_bound(u.begin() + beg, u.begin() + at, r) - (u.begin() + beg));
+          dense[static_cast<size_t>(kpos) * kBlockOut + o] += bl.oval[q];
+        }
+      }
+      for (int ch = 0; ch < nch; ++ch) {
+        for (int m = 0; m < 2; ++m)
+          for (int lane = 0; lane < 64; ++lane) {
+            const int o = m * 16 + (lane & 15);
+            for (int e = 0; e < 8; ++e) {
+              const int k = ch * kChunkK + 8 * (lane >> 4) + e;
+              bl.bwt.push_back(float_to_half_bits(dense[static_cast<size_t>(k) * kBlockOut + o]));
+            }
+          }
+      }
+      ch0 += nch;
+    }
+    bl.bptr[static_cast<size_t>(c) * (nb + 1) + nb] = static_cast<int32_t>(ch0);
+  }
+  bl.band_nchunk = ch0;
+  // mask words of the band image (the weights are those of the tile image, so its one value)
+  auto mask_words = [&]() -> bool {
+    bl.bmask.assign(static_cast<size_t>(ch0) * 64, 0);
+    for (int64_t ch = 0; ch < ch0; ++ch)
+      for (int lane = 0; lane < 64; ++lane) {
+        uint16_t bits = 0;
+        for (int m = 0; m < 2; ++m)
+          for (int e = 0; e < 8; ++e) {
+            const uint16_t h = bl.bwt[((static_cast<size_t>(ch) * 2 + m) * 64 + lane) * 8 + e];
+            if (h == 0) continue;
+            if (h != bl.wmask_val) return false;
+            bits |= static_cast<uint16_t>(1u << ((e & 1) * 8 + 4 * m + e / 2));
+          }
+        bl.bmask[static_cast<size_t>(ch) * 64 + lane] = bits;
+      }
+    return true;
+  };
+  if (!bl.has_mask || !mask_words()) bl.bmask.clear();
+  bl.has_band = ch0 > 0;  // (a layer without any input row has no chunk to run)
 }
 
 bool csc_from_reference_bcsc(const std::vector<float>& values, const std::vector<int>& row_idx,

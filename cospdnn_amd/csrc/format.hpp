@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "gen.hpp"
+#include "kernels.hpp"
 
 namespace cospdnn {
 
@@ -41,6 +42,25 @@ struct BlockedLayer {
   std::vector<int32_t> gidx;   // nchunk*32
   std::vector<uint16_t> wt;    // nchunk*1024 FP16 bit patterns (v_mfma_f32_16x16x32_f16 A order)
   std::vector<float> wt32;     // nchunk*1024 FP32 values (v_mfma_f32_16x16x4f32 A order)
+  // Mask image: when every nonzero of the tile image is one FP16 value (wmask_val), the image
+  // is a bit per element: per chunk and lane one 16-bit word, bit 4m + e/2 for the even
+  // elements e of fragment m (m = 0, 1), bit 8 + 4m + e/2 for the odd ones (128 bytes a
+  // chunk instead of 2 KB); the kernels expand it to the same fragments in registers.
+  bool has_mask = false;
+  uint16_t wmask_val = 0;
+  std::vector<uint16_t> wmask;  // nchunk*64
+  // Band image (input rows in bands of kBandRows activation rows): per block column the
+  // sorted union of its input rows, split at band boundaries, each band's part padded to
+  // chunks of 32, so that every chunk reads one band; band-relative row ids, fragments (and
+  // mask words) in the same layout as the tile image.  bptr[c * (bands + 1) + b] = first chunk
+  // of block column c in band b.
+  bool has_band = false;
+  int bands = 0;
+  int64_t band_nchunk = 0;
+  std::vector<int32_t> bptr;
+  std::vector<int32_t> bgidx;
+  std::vector<uint16_t> bwt;
+  std::vector<uint16_t> bmask;  // when has_mask
   bool has_tiles32 = false;    // wt32 built (tile_waste <= kMaxWasteFp32Tiles)
   int64_t nchunk = 0;
   double tile_waste = 0.0;     // MFMA MACs per useful MAC (nchunk*32*32 / nnz)
@@ -55,6 +75,11 @@ constexpr double kMaxWasteFp32Tiles = 4.0;
 // Builds both images. Tiles are built only when tile_waste <= max_waste.
 // Returns false (and leaves `out` unspecified) if the CSC is malformed.
 bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out);
+
+// The band image of a layer from its SIMT image (rows already in the activation-row order of
+// the layer's input, in_rows of them).  Fragments follow the values' FP16 bits; mask words are
+// built when the tile image has a mask (one value throughout).
+void build_band_image(BlockedLayer& bl, int in_rows);
 
 // True when an all-32 input row gives an all-32 output row whatever the summation order
 // in FP32 or FP64: every weight is nonnegative and, for every column j (empty ones

@@ -480,10 +480,37 @@ __device__ __forceinline__ void expand_bits(uint32_t wbits, uint4 (&o)[4]) {
   for (int q = 0; q < 4; ++q) o[q] = make_uint4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
 }
 
+// The chunk's two W fragments from its mask-image word (the lane's 16 bits, see format.hpp):
+// dword k of fragment m holds elements 2k (low half) and 2k+1 (high half), each the FP16 value
+// v where its bit is set and 0 elsewhere.  The odd elements' bits move up by 8 so that bit
+// 4m + k holds element 2k and bit 16 + 4m + k element 2k + 1: one shift, mask and 24-bit
+// multiply per dword (x * v with x = lo + hi * 2^16 and v < 2^16 cannot carry between halves).
+__device__ __forceinline__ void expand_mask(uint32_t bits, uint32_t v, half8_t& w0, half8_t& w1) {
+  const uint32_t sp = (bits & 0xFFu) | ((bits & 0xFF00u) << 8);
+  uint32_t d[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) d[q] = __umul24((sp >> q) & 0x10001u, v);
+  w0 = __builtin_bit_cast(half8_t, u32x4_t{d[0], d[1], d[2], d[3]});
+  w1 = __builtin_bit_cast(half8_t, u32x4_t{d[4], d[5], d[6], d[7]});
+}
+
+// W fragments of chunk ch, as loaded (MASK: the lane's mask word in wm; otherwise w0, w1)
+template <bool MASK>
+__device__ __forceinline__ void load_w(const DevLayer& w, int64_t ch, int lane, half8_t& w0, half8_t& w1,
+                                       uint32_t& wm) {
+  if constexpr (MASK) {
+    wm = w.wmask[ch * 64 + lane];
+  } else {
+    w0 = *reinterpret_cast<const half8_t*>(w.wt + (ch * 2 + 0) * 512 + lane * 8);
+    w1 = *reinterpret_cast<const half8_t*>(w.wt + (ch * 2 + 1) * 512 + lane * 8);
+  }
+}
+
 // GATHER: tiles of output positions after a compacting scan.  BITS: layer 0 of an identity
 // layout with a binary input, read from its bit-packed neuron-major copy: a step's 32 rows x
 // 64 samples are 256 bytes instead of 4 KB, expanded to FP16 in registers.
-template <bool GATHER, bool BITS>
+// MASK: the W fragments come from the layer's mask image (expanded in registers).
+template <bool GATHER, bool BITS, bool MASK>
 __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, float (*wlds)[32 * kStageLd], int n,
                                           int n_out) {
   // ident: no row left the pass in the scan before this layer, every position maps to itself
@@ -554,6 +581,8 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
   int rn0, rn1, rn2, rn3;  // row ids of step nx1
   uint4 d0, d1, d2, d3;
   half8_t w0, w1;
+  uint32_t wm = 0;  // MASK: the lane's mask word of the next step's chunk
+  const uint32_t wval = w.wmask_val;
   int qv;
   // gather mode: this step's 32 rows of the lane's sample (gd), the row ids of step nx1 (in
   // lanes 0..31 of grow1) and the input position of the lane's sample in nx1's tile
@@ -593,8 +622,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     d3 = load_stream(base + static_cast<int64_t>(gidx[ch + 24]) * ldx);
     qv = ident ? cur.t * kWaveSamples + lane : a.qmap[min(cur.t * kWaveSamples + lane, n - 1)];
   }
-  w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 0) * 512 + lane * 8);
-  w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(cur.ch) * 2 + 1) * 512 + lane * 8);
+  load_w<MASK>(w, cur.ch, lane, w0, w1, wm);
   // accumulators start at the bias: D' + b comes out of the MFMA chain
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
   const bool full = w.full_blocks != 0;
@@ -643,7 +671,8 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     const int p_own = cur.t * kWaveSamples + lane;
     // the output position of the lane's sample (-1: none)
     const int q_own = GATHER ? (p_own < n_out ? p_own : -1) : (p_own < n ? (BITS ? p_own : qv) : -1);
-    const half8_t cw0 = w0, cw1 = w1;
+    half8_t cw0 = w0, cw1 = w1;
+    const uint32_t cwm = wm;
     // output rows j and 16 + j of the block column (slot = row, -1 for padding)
     const int cov0 = cur.c * 32 + j < w.n_out ? cur.c * 32 + j : -1;
     const int cov1 = cur.c * 32 + 16 + j < w.n_out ? cur.c * 32 + 16 + j : -1;
@@ -693,10 +722,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
     {
       // W fragments of the next step (gather mode: after this step's MFMAs, so that the
       // gathered rows in flight and the W registers in use do not both need copies)
-      if constexpr (!GATHER) {
-        w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
-        w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
-      }
+      if constexpr (!GATHER) load_w<MASK>(w, pch, lane, w0, w1, wm);
       if constexpr (BITS) {
         brow1 = gidx[(nx2.valid ? nx2.ch : pch) * kChunk + (lane >> 1)];
       } else if constexpr (GATHER) {
@@ -731,15 +757,13 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
                                            3, 4, 5, 6, 7);
       }
     }
+    if constexpr (MASK) expand_mask(cwm, wval, cw0, cw1);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw0, acc[t][0], 0, 0, 0);
       acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[t], cw1, acc[t][1], 0, 0, 0);
     }
-    if constexpr (GATHER) {
-      w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 0) * 512 + lane * 8);
-      w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(pch) * 2 + 1) * 512 + lane * 8);
-    }
+    if constexpr (GATHER) load_w<MASK>(w, pch, lane, w0, w1, wm);
     if (last) {
       // D'[i][jj] of a 16x16 tile sits in lane jj + 16*(i/4), element i%4: acc[t][m][r] of
       // lane j + 16g is output 16m + j of sample 16g + 4t + r.
@@ -825,11 +849,24 @@ __global__ __launch_bounds__(256, 4) void spmm_mfma_kernel(StepArgs a, DevLayer 
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
   const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
   if (n_out != n && a.qsrc != nullptr)
-    mfma_body<true, false>(a, w, wlds, n, n_out);
+    mfma_body<true, false, false>(a, w, wlds, n, n_out);
   else if (n_out == n && a.xbits != nullptr)
-    mfma_body<false, true>(a, w, wlds, n, n_out);
+    mfma_body<false, true, false>(a, w, wlds, n, n_out);
   else
-    mfma_body<false, false>(a, w, wlds, n, n_out);
+    mfma_body<false, false, false>(a, w, wlds, n, n_out);
+}
+
+// The same kernel on a layer's mask image (the launcher picks it when the layer has one).
+__global__ __launch_bounds__(256, 4) void spmm_mfma_mask_kernel(StepArgs a, DevLayer w) {
+  __shared__ __attribute__((aligned(16))) float wlds[kWavesPerGroup][32 * kStageLd];
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  if (n_out != n && a.qsrc != nullptr)
+    mfma_body<true, false, true>(a, w, wlds, n, n_out);
+  else if (n_out == n && a.xbits != nullptr)
+    mfma_body<false, true, true>(a, w, wlds, n, n_out);
+  else
+    mfma_body<false, false, true>(a, w, wlds, n, n_out);
 }
 
 // --------------------------------------------------------------------------
@@ -851,19 +888,24 @@ constexpr int kLdsAhead = 4;  // K chunks whose W fragments and rows are in flig
 // read, whose starts are otherwise all on the same bank, spread over the banks
 constexpr int kLdsLd = kWaveSamples + 4;
 
-// W fragments and fragment-read rows of one K chunk
+// W fragments (MASK: the lane's mask word) and fragment-read rows of one K chunk
 struct LdsChunk {
   half8_t w0, w1;
+  uint32_t wm;
   int rlo, rhi;
 };
+template <bool MASK>
 __device__ __forceinline__ void lds_chunk_load(LdsChunk& k, const DevLayer& w, int ch, int lane, int j, int g) {
   k.rlo = w.gidx[ch * kChunk + g * 8 + (j >> 2)];
   k.rhi = w.gidx[ch * kChunk + g * 8 + 4 + (j >> 2)];
-  k.w0 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(ch) * 2 + 0) * 512 + lane * 8);
-  k.w1 = *reinterpret_cast<const half8_t*>(w.wt + (static_cast<int64_t>(ch) * 2 + 1) * 512 + lane * 8);
+  load_w<MASK>(w, ch, lane, k.w0, k.w1, k.wm);
 }
 // the chunk's X fragments from the LDS tile, then its 8 MFMAs
-__device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsChunk& k, const _Float16* lds_x, int j) {
+template <bool MASK>
+__device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsChunk& k, const _Float16* lds_x, int j,
+                                               uint32_t wval) {
+  half8_t w0 = k.w0, w1 = k.w1;
+  if constexpr (MASK) expand_mask(k.wm, wval, w0, w1);
   half8_t xfrag[4];
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
@@ -876,15 +918,104 @@ __device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsC
   }
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
-    acc[tt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], k.w0, acc[tt][0], 0, 0, 0);
-    acc[tt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], k.w1, acc[tt][1], 0, 0, 0);
+    acc[tt][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], w0, acc[tt][0], 0, 0, 0);
+    acc[tt][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(xfrag[tt], w1, acc[tt][1], 0, 0, 0);
   }
 }
+
+// Epilogue of the LDS-staged MFMA kernels for block column c of a 64-sample tile:
+// acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r.  Bias is in acc.
+// fast: the tile's 64 samples go to 64 consecutive positions from q0, a multiple of 8 (stores
+// through the wave's staging area ostage, [16][kOutLd], as 8 lanes x 16 B per output row);
+// otherwise each value goes to its sample's position (q_own of the owning lane, -1: none).
+// Then the liveness / unsaturated / guard-band flags of the lane's sample.
+__device__ __forceinline__ void lds_epilogue(const StepArgs& a, const DevLayer& w, const float4_t (&acc)[4][2], int c,
+                                             int q_own, int q0, bool fast, _Float16* ostage,
+                                             _Float16* __restrict__ Y, int64_t ldy) {
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 15, g = lane >> 4;
+  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
+  const bool full = w.full_blocks != 0;
+  const bool want_unsat = a.unsat_next != nullptr;
+  const bool want_band = a.band_next != nullptr;
+  // epilogue: acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r
+  const int cov0 = c * 32 + j < w.n_out ? c * 32 + j : -1;
+  const int cov1 = c * 32 + 16 + j < w.n_out ? c * 32 + 16 + j : -1;
+  const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
+  const uint32_t ba = full ? flag_bits<0, true>(acc, true, true, 0.0f) : flag_bits<0, false>(acc, ok0, ok1, 0.0f);
+  uint32_t fl = (ba >> j) & 1u;
+  if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
+  if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
+  if (fast) {
+    // FP16 outputs through the wave's staging area, 16 rows at a time, stored as 8 lanes x
+    // 16 B per row
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      uint32_t pk[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int tt = e / 2, r = 2 * (e % 2);
+        const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r], 0.0f, 32.0f));
+        const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r + 1], 0.0f, 32.0f));
+        pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
+                (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
+      }
+      uint4* o = reinterpret_cast<uint4*>(ostage + j * kOutLd + 16 * g);
+      o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+      o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      _Float16* ybase = Y + q0 + c8;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int slot = c * 32 + 16 * m + r8 + 8 * k;
+        const uint4 v = *reinterpret_cast<const uint4*>(ostage + (r8 + 8 * k) * kOutLd + c8);
+        if (full || slot < w.n_out) store_stream(ybase + static_cast<int64_t>(slot) * ldy, v);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+  } else {
+    // each value to its sample's position: sample 16g + 4tt + r is owned by that lane
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qs = __shfl(q_own, 16 * g + 4 * tt + r, 64);
+        if (qs >= 0) {
+          if (cov0 >= 0)
+            Y[static_cast<int64_t>(cov0) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][0][r], 0.0f, 32.0f));
+          if (cov1 >= 0)
+            Y[static_cast<int64_t>(cov1) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][1][r], 0.0f, 32.0f));
+        }
+      }
+  }
+  if (q_own >= 0) {
+    if (fl & 1u) a.alive_next[q_own] = 1;
+    if (fl & 2u) a.unsat_next[q_own] = 1;
+    if (fl & 4u) a.band_next[q_own] = 1;
+  }
+}
+
+template <bool MASK>
+__device__ __forceinline__ void mfma_lds_body(const StepArgs& a, const DevLayer& w, _Float16* lds_x);
 
 __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepArgs a, DevLayer w) {
   // [x_rows][kLdsLd] input tile, then per wave [16][kOutLd] FP16 output staging
   extern __shared__ __attribute__((aligned(16))) _Float16 lds_x[];
+  mfma_lds_body<false>(a, w, lds_x);
+}
+
+// The same kernel on a layer's mask image.
+__global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_mask_kernel(StepArgs a, DevLayer w) {
+  extern __shared__ __attribute__((aligned(16))) _Float16 lds_x[];
+  mfma_lds_body<true>(a, w, lds_x);
+}
+
+template <bool MASK>
+__device__ __forceinline__ void mfma_lds_body(const StepArgs& a, const DevLayer& w, _Float16* lds_x) {
   constexpr int kThreads = kLdsWaves * 64;
+  const uint32_t wval = w.wmask_val;
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
   const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
   const bool ident = n_out == n;
@@ -893,14 +1024,10 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int j = lane & 15, g = lane >> 4;
-  const int r8 = lane >> 3, c8 = (lane & 7) * 8;
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
   _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
   const int64_t ldx = x_ld(a), ldy = y_ld(a);
   const int nblk = w.nblk;
-  const bool full = w.full_blocks != 0;
-  const bool want_unsat = a.unsat_next != nullptr;
-  const bool want_band = a.band_next != nullptr;
   const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
   const cint_ptr_t kptr = (cint_ptr_t)w.kptr;
   // staging areas from a 16-byte boundary (x_rows may be odd)
@@ -952,76 +1079,205 @@ __global__ __launch_bounds__(kLdsWaves * 64, 1) void spmm_mfma_lds_kernel(StepAr
       LdsChunk kc[kLdsAhead];
       if (ch1 > ch0) {
 #pragma unroll
-        for (int u = 0; u < kLdsAhead; ++u) lds_chunk_load(kc[u], w, min(ch0 + u, ch1 - 1), lane, j, g);
+        for (int u = 0; u < kLdsAhead; ++u) lds_chunk_load<MASK>(kc[u], w, min(ch0 + u, ch1 - 1), lane, j, g);
       }
       for (int ch = ch0; ch < ch1; ch += kLdsAhead) {
         // the reloads are unconditional (clamped to the last chunk), so every pass issues the
         // same loads and the wait before each chunk covers only that chunk's set
 #pragma unroll
         for (int u = 0; u < kLdsAhead; ++u) {
-          if (ch + u < ch1) lds_chunk_mfma(acc, kc[u], lds_x, j);
-          lds_chunk_load(kc[u], w, min(ch + u + kLdsAhead, ch1 - 1), lane, j, g);
+          if (ch + u < ch1) lds_chunk_mfma<MASK>(acc, kc[u], lds_x, j, wval);
+          lds_chunk_load<MASK>(kc[u], w, min(ch + u + kLdsAhead, ch1 - 1), lane, j, g);
         }
       }
-      // epilogue: acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r
-      const int cov0 = c * 32 + j < w.n_out ? c * 32 + j : -1;
-      const int cov1 = c * 32 + 16 + j < w.n_out ? c * 32 + 16 + j : -1;
-      const bool ok0 = full || cov0 >= 0, ok1 = full || cov1 >= 0;
-      const uint32_t ba = full ? flag_bits<0, true>(acc, true, true, 0.0f) : flag_bits<0, false>(acc, ok0, ok1, 0.0f);
-      uint32_t fl = (ba >> j) & 1u;
-      if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
-      if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
-      if (fast) {
-        // FP16 outputs through the wave's staging area, 16 rows at a time, stored as 8 lanes x
-        // 16 B per row
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-          uint32_t pk[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int tt = e / 2, r = 2 * (e % 2);
-            const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r], 0.0f, 32.0f));
-            const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r + 1], 0.0f, 32.0f));
-            pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
-                    (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
-          }
-          uint4* o = reinterpret_cast<uint4*>(ostage + j * kOutLd + 16 * g);
-          o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-          o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          _Float16* ybase = Y + q0 + c8;
-#pragma unroll
-          for (int k = 0; k < 2; ++k) {
-            const int slot = c * 32 + 16 * m + r8 + 8 * k;
-            const uint4 v = *reinterpret_cast<const uint4*>(ostage + (r8 + 8 * k) * kOutLd + c8);
-            if (full || slot < w.n_out) store_stream(ybase + static_cast<int64_t>(slot) * ldy, v);
-          }
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        }
-      } else {
-        // each value to its sample's position: sample 16g + 4tt + r is owned by that lane
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int qs = __shfl(q_own, 16 * g + 4 * tt + r, 64);
-            if (qs >= 0) {
-              if (cov0 >= 0)
-                Y[static_cast<int64_t>(cov0) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][0][r], 0.0f, 32.0f));
-              if (cov1 >= 0)
-                Y[static_cast<int64_t>(cov1) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][1][r], 0.0f, 32.0f));
-            }
-          }
-      }
-      if (q_own >= 0) {
-        if (fl & 1u) a.alive_next[q_own] = 1;
-        if (fl & 2u) a.unsat_next[q_own] = 1;
-        if (fl & 4u) a.band_next[q_own] = 1;
-      }
+      lds_epilogue(a, w, acc, c, q_own, q0, fast, ostage, Y, ldy);
     }
   }
+}
+
+// --------------------------------------------------------------------------
+// Band-streaming MFMA SpMM (plan order 6) for random layers whose 64-sample input tile does
+// not fit LDS.  Gathering a block column's rows from L2 re-reads every input row once per
+// block column that uses it (~28 times at N = 4096); here a work item is (64-sample tile,
+// group of 16 block columns = 512 outputs): the item's 8 waves keep the accumulators of two
+// block columns each in registers and stream the tile's input rows through LDS in bands of
+// kBandRows rows (two LDS buffers), so each input row moves from L2 once per group.  The band
+// image splits every block column's K chunks at band boundaries (band-relative row ids), so
+// a chunk reads one band.  Per step (item, band): the next band's rows are loaded into
+// registers, the current band's chunks run from LDS (X fragments by ds_read_b64_tr_b16 at the
+// rows' LDS addresses, same MFMA chain as the other kernels), the loaded rows are written to
+// the other buffer, one barrier.  The W fragments and row ids of a step's first chunks are
+// loaded at the end of the step before (ahead of the next band's rows, so waiting for them
+// never waits for the band).  Tiles are partitioned over the XCDs (blockIdx % 8), and the
+// groups of a tile follow each other, so a tile's rows are read from HBM once into its XCD's
+// L2.  Epilogue and compacted stores as in the LDS-resident kernel.
+// --------------------------------------------------------------------------
+constexpr int kBandWaves = 8;  // 2 per SIMD: one work-group per CU
+constexpr int kBandBcPerWave = 2;
+constexpr int kBandThreads = kBandWaves * 64;
+constexpr int kBandGroup = kBandWaves * kBandBcPerWave;            // block columns per item
+constexpr int kBandPieces = kBandRows * 8 / kBandThreads;          // 16-byte pieces per thread and band
+static_assert(kBandPieces * kBandThreads == kBandRows * 8, "whole 16-byte pieces per thread");
+constexpr size_t kBandBufHalves = static_cast<size_t>(kBandRows) * kLdsLd;
+
+size_t spmm_mfma_band_bytes() {
+  return (2 * kBandBufHalves + static_cast<size_t>(kBandWaves) * 16 * kOutLd) * sizeof(_Float16);
+}
+
+// Rows [b kBandRows, (b + 1) kBandRows) of tile t into registers: piece k of thread tid is row
+// b kBandRows + k kBandThreads / 8 + tid / 8, samples 64 t + 8 (tid % 8)..+7 (each load
+// instruction moves 8 whole 128-byte row segments); rows past x_rows are zero.
+__device__ __forceinline__ void band_load(uint4 (&v)[kBandPieces], const _Float16* __restrict__ X, int64_t ldx, int t,
+                                          int b, int x_rows, int tid) {
+  const _Float16* src = X + static_cast<int64_t>(t) * kWaveSamples + (tid & 7) * 8;
+#pragma unroll
+  for (int k = 0; k < kBandPieces; ++k) {
+    const int row = b * kBandRows + k * (kBandThreads / 8) + (tid >> 3);
+    v[k] = row < x_rows ? load_stream(src + static_cast<int64_t>(row) * ldx) : make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+__device__ __forceinline__ void band_store(const uint4 (&v)[kBandPieces], _Float16* buf, int tid) {
+#pragma unroll
+  for (int k = 0; k < kBandPieces; ++k) {
+    // two 8-byte writes: rows start on 8-byte boundaries (136-byte stride)
+    uint2* d = reinterpret_cast<uint2*>(buf + (k * (kBandThreads / 8) + (tid >> 3)) * kLdsLd + (tid & 7) * 8);
+    d[0] = make_uint2(v[k].x, v[k].y);
+    d[1] = make_uint2(v[k].z, v[k].w);
+  }
+}
+
+template <bool MASK>
+struct BandCfg {
+  static constexpr int kAhead = MASK ? 6 : 3;  // K chunks in flight per block column
+};
+
+// Loads of the first kAhead chunks of [lo, hi) (clamped; unconditional, so the number of loads
+// in flight is the same on every path).
+template <bool MASK>
+__device__ __forceinline__ void band_prefetch(LdsChunk (&kc)[BandCfg<MASK>::kAhead], const DevLayer& wb, int lo, int hi,
+                                              int lane, int j, int g) {
+#pragma unroll
+  for (int u = 0; u < BandCfg<MASK>::kAhead; ++u)
+    lds_chunk_load<MASK>(kc[u], wb, hi > lo ? min(lo + u, hi - 1) : 0, lane, j, g);
+}
+
+// The chunks [lo, hi) of one block column from the band in buf (kc holds the first kAhead).
+template <bool MASK>
+__device__ __forceinline__ void band_chunks(float4_t (&acc)[4][2], LdsChunk (&kc)[BandCfg<MASK>::kAhead],
+                                            const DevLayer& wb, int lo, int hi, const _Float16* buf, int lane, int j,
+                                            int g, uint32_t wval) {
+  constexpr int A = BandCfg<MASK>::kAhead;
+  for (int ch = lo; ch < hi; ch += A) {
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+      if (ch + u < hi) lds_chunk_mfma<MASK>(acc, kc[u], buf, j, wval);
+      if (ch + u + A < hi) lds_chunk_load<MASK>(kc[u], wb, ch + u + A, lane, j, g);
+    }
+  }
+}
+
+template <bool MASK>
+__device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer& w, _Float16* lds) {
+  constexpr int A = BandCfg<MASK>::kAhead;
+  const uint32_t wval = w.wmask_val;
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  const bool ident = n_out == n;
+  const int x_rows = a.x_rows;
+  const int nb = w.bands;
+  const int ntiles = (n + kWaveSamples - 1) / kWaveSamples;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int j = lane & 15, g = lane >> 4;
+  const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+  _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
+  const int nblk = w.nblk;
+  const float4_t bias4 = float4_t{a.bias, a.bias, a.bias, a.bias};
+  DevLayer wb = w;  // the band image in the fields the chunk loader reads
+  wb.gidx = w.bgidx;
+  wb.wt = w.bwt;
+  wb.wmask = w.bmask;
+  const cint_ptr_t bptr = (cint_ptr_t)w.bptr;
+  _Float16* buf0 = lds;
+  _Float16* buf1 = lds + kBandBufHalves;
+  _Float16* ostage = lds + 2 * kBandBufHalves + wave * 16 * kOutLd;
+  const int ngroups = (nblk + kBandGroup - 1) / kBandGroup;
+  // this XCD's tiles t = xcd (mod 8), each with its groups one after another; this
+  // work-group's items are wl, wl + wgs, ... of the XCD's list
+  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+  const int wgs = (static_cast<int>(gridDim.x) - xcd + kXcds - 1) / kXcds;
+  const int wl = static_cast<int>(blockIdx.x) / kXcds;
+  const int ntx = ntiles > xcd ? (ntiles - xcd + kXcds - 1) / kXcds : 0;
+  const int items = ntx * ngroups;
+  const int my_items = wl < items ? (items - wl + wgs - 1) / wgs : 0;
+  const int steps = my_items * nb;
+  if (steps == 0) return;  // uniform per work-group
+  auto tile_of = [&](int s) { return xcd + kXcds * ((wl + (s / nb) * wgs) / ngroups); };
+  auto group_of = [&](int s) { return (wl + (s / nb) * wgs) % ngroups; };
+  // chunk range of the wave's block column q (0, 1) of group G in band b
+  auto range = [&](int G, int q, int b, int& lo, int& hi) {
+    const int c = G * kBandGroup + wave + kBandWaves * q;
+    lo = c < nblk ? bptr[c * (nb + 1) + b] : 0;
+    hi = c < nblk ? bptr[c * (nb + 1) + b + 1] : 0;
+  };
+
+  uint4 v[kBandPieces];
+  LdsChunk kc0[A], kc1[A];
+  int lo0, hi0, lo1, hi1;
+  // step 0: its band into buffer 0, its first chunks in flight
+  band_load(v, X, ldx, tile_of(0), 0, x_rows, tid);
+  band_store(v, buf0, tid);
+  range(group_of(0), 0, 0, lo0, hi0);
+  range(group_of(0), 1, 0, lo1, hi1);
+  band_prefetch<MASK>(kc0, wb, lo0, hi0, lane, j, g);
+  band_prefetch<MASK>(kc1, wb, lo1, hi1, lane, j, g);
+  __syncthreads();
+  float4_t acc0[4][2], acc1[4][2];
+  for (int s = 0; s < steps; ++s) {
+    const int b = s % nb;
+    const int t = tile_of(s), G = group_of(s);
+    const bool has_next = s + 1 < steps;
+    if (has_next) band_load(v, X, ldx, tile_of(s + 1), (s + 1) % nb, x_rows, tid);
+    if (b == 0) {
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int m = 0; m < 2; ++m) acc0[tt][m] = acc1[tt][m] = bias4;
+    }
+    const _Float16* buf = (s & 1) ? buf1 : buf0;
+    band_chunks<MASK>(acc0, kc0, wb, lo0, hi0, buf, lane, j, g, wval);
+    band_chunks<MASK>(acc1, kc1, wb, lo1, hi1, buf, lane, j, g, wval);
+    if (b == nb - 1) {
+      // this lane's sample (lane L owns sample 64t + L) and its output position
+      const int p_own = t * kWaveSamples + lane;
+      const int q_own = p_own < n ? (ident ? p_own : a.qmap[p_own]) : -1;
+      const int q0 = __builtin_amdgcn_readfirstlane(q_own);
+      const bool fast = __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
+      const int c0 = G * kBandGroup + wave, c1 = c0 + kBandWaves;
+      if (c0 < nblk) lds_epilogue(a, w, acc0, c0, q_own, q0, fast, ostage, Y, ldy);
+      if (c1 < nblk) lds_epilogue(a, w, acc1, c1, q_own, q0, fast, ostage, Y, ldy);
+    }
+    if (has_next) {
+      const int bn = (s + 1) % nb, Gn = group_of(s + 1);
+      range(Gn, 0, bn, lo0, hi0);
+      range(Gn, 1, bn, lo1, hi1);
+      band_prefetch<MASK>(kc0, wb, lo0, hi0, lane, j, g);
+      band_prefetch<MASK>(kc1, wb, lo1, hi1, lane, j, g);
+      band_store(v, (s & 1) ? buf0 : buf1, tid);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBandThreads, 1) void spmm_mfma_band_kernel(StepArgs a, DevLayer w) {
+  extern __shared__ __attribute__((aligned(16))) _Float16 lds_band[];
+  mfma_band_body<false>(a, w, lds_band);
+}
+
+__global__ __launch_bounds__(kBandThreads, 1) void spmm_mfma_band_mask_kernel(StepArgs a, DevLayer w) {
+  extern __shared__ __attribute__((aligned(16))) _Float16 lds_band[];
+  mfma_band_body<true>(a, w, lds_band);
 }
 
 // --------------------------------------------------------------------------
@@ -1468,7 +1724,10 @@ hipError_t launch_scan(const ScanArgs& a, int64_t max_rows, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_spmm_mfma(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(spmm_mfma_kernel, dim3(grid), dim3(256), 0, st, a, w);
+  if (w.wmask != nullptr)
+    hipLaunchKernelGGL(spmm_mfma_mask_kernel, dim3(grid), dim3(256), 0, st, a, w);
+  else
+    hipLaunchKernelGGL(spmm_mfma_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
 }
 hipError_t launch_spmm_simt_rows(const StepArgs& a, const DevLayer& w, Dtype tin, Dtype tout, int grid, hipStream_t st) {
@@ -1491,7 +1750,20 @@ int spmm_mfma_lds_bytes(int x_rows) {
 }
 hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   if (a.x_rows < 1 || a.x_rows > kLdsRows || w.kptr == nullptr) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(spmm_mfma_lds_kernel, dim3(grid), dim3(kLdsWaves * 64), spmm_mfma_lds_bytes(a.x_rows), st, a, w);
+  if (w.wmask != nullptr)
+    hipLaunchKernelGGL(spmm_mfma_lds_mask_kernel, dim3(grid), dim3(kLdsWaves * 64), spmm_mfma_lds_bytes(a.x_rows), st,
+                       a, w);
+  else
+    hipLaunchKernelGGL(spmm_mfma_lds_kernel, dim3(grid), dim3(kLdsWaves * 64), spmm_mfma_lds_bytes(a.x_rows), st, a, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
+  if (w.bptr == nullptr || w.bands < 1 || grid < kXcds) return hipErrorInvalidValue;
+  if (w.bmask != nullptr)
+    hipLaunchKernelGGL(spmm_mfma_band_mask_kernel, dim3(grid), dim3(kBandThreads), spmm_mfma_band_bytes(), st, a, w);
+  else
+    hipLaunchKernelGGL(spmm_mfma_band_kernel, dim3(grid), dim3(kBandThreads), spmm_mfma_band_bytes(), st, a, w);
   return hipGetLastError();
 }
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
@@ -1565,8 +1837,18 @@ int spmm_mfma32_blocks_per_cu(Dtype tin) {
 }
 
 hipError_t configure_kernels() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_lds_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, spmm_mfma_lds_bytes(kLdsRows));
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_lds_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, spmm_mfma_lds_bytes(kLdsRows));
+  if (e != hipSuccess) return e;
+  const hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_lds_mask_kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, spmm_mfma_lds_bytes(kLdsRows));
+  if (e2 != hipSuccess) return e2;
+  const int band_bytes = static_cast<int>(spmm_mfma_band_bytes());
+  const hipError_t e3 = hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_band_kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, band_bytes);
+  if (e3 != hipSuccess) return e3;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_band_mask_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, band_bytes);
 }
 
 int spmm_mfma_resident_per_cu() {

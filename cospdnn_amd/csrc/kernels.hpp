@@ -18,6 +18,7 @@
 namespace cospdnn {
 
 constexpr int kTileS = 256;  // samples per work tile (both SpMM kernels)
+constexpr int kBandRows = 512;  // activation rows per band of a layer's band image
 constexpr int kXcds = 8;     // MI355X: 8 XCDs, work-groups dispatched to them round-robin
 
 struct DevLayer {
@@ -32,6 +33,18 @@ struct DevLayer {
   const int32_t* kptr = nullptr;     // [nblk+1] (MFMA image; null if absent)
   const int32_t* gidx = nullptr;
   const _Float16* wt = nullptr;
+  // mask image (null: none or not used): per chunk and lane a 16-bit word, every set bit the
+  // FP16 value wmask_val (see format.hpp); the MFMA kernels then read it instead of wt
+  const uint16_t* wmask = nullptr;
+  uint32_t wmask_val = 0;
+  // band image (null: none): input rows in bands of kBandRows, chunks per (block column, band)
+  // at bptr[c * (bands + 1) + b], band-relative row ids bgidx, fragments bwt or mask words bmask
+  int bands = 0;
+  int band_nchunk = 0;
+  const int32_t* bptr = nullptr;
+  const int32_t* bgidx = nullptr;
+  const _Float16* bwt = nullptr;
+  const uint16_t* bmask = nullptr;
   const float* wt32 = nullptr;       // FP32 tile image (MFMA FP32 kernel)
   int one_chunk = 0;                 // every block column has exactly one K chunk (kptr[c] = c)
   int full_blocks = 0;               // every out_idx slot holds an output (no padding slots)
@@ -142,6 +155,9 @@ hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid,
 // LDS-resident MFMA kernel (a.x_rows <= kLdsRowsMax input rows, tile image present); any grid.
 constexpr int kLdsRowsMax = 1024;
 hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// Band-streaming MFMA kernel (band image present, a.x_rows input rows; w.bmask used when set,
+// w.bwt otherwise); one work-group of 512 threads per CU, any grid of at least 8 work-groups.
+hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,

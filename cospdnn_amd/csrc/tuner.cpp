@@ -40,14 +40,14 @@ const cospdnn_layer_plan kCandidates[] = {
     {COSPDNN_PLAN_MFMA, 3, 4},         {COSPDNN_PLAN_MFMA, 4, 4},         {COSPDNN_PLAN_MFMA_COMPACT, 1, 4},
     {COSPDNN_PLAN_MFMA_COMPACT, 2, 2}, {COSPDNN_PLAN_MFMA_COMPACT, 2, 4}, {COSPDNN_PLAN_MFMA_COMPACT, 3, 4},
     {COSPDNN_PLAN_MFMA_COMPACT, 4, 4}, {COSPDNN_PLAN_MFMA, 5, 1},         {COSPDNN_PLAN_ROWS, 0, 5},
-    {COSPDNN_PLAN_ROWS, 1, 5},
+    {COSPDNN_PLAN_ROWS, 1, 5},         {COSPDNN_PLAN_MFMA, 6, 1},
 };
 constexpr int kNumCandidates = static_cast<int>(sizeof(kCandidates) / sizeof(kCandidates[0]));
 
 int check_plan(const cospdnn_layer_plan& p) {
   if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_ROWS) return fail(COSPDNN_ERR_ARG, "plan: kernel");
   const bool mfma = p.kernel == COSPDNN_PLAN_MFMA || p.kernel == COSPDNN_PLAN_MFMA_COMPACT;
-  if (mfma && (p.order < 0 || p.order > 5)) return fail(COSPDNN_ERR_ARG, "plan: order");
+  if (mfma && (p.order < 0 || p.order > 6)) return fail(COSPDNN_ERR_ARG, "plan: order");
   if (p.kernel == COSPDNN_PLAN_ROWS && (p.order < 0 || p.order > 1)) return fail(COSPDNN_ERR_ARG, "plan: rows split");
   if ((mfma || p.kernel == COSPDNN_PLAN_ROWS) && (p.groups_per_cu < 1 || p.groups_per_cu > 16))
     return fail(COSPDNN_ERR_ARG, "plan: groups_per_cu");
@@ -80,7 +80,7 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[11] = static_cast<float>(removed / (removed + 1.0));
   // gather mode allowed after a compacting scan (plain MFMA) or compacted stores; the
   // LDS-resident kernel (order 5 on inputs of at most kLdsRowsMax rows) always stores compacted
-  const bool lds_resident = c.order == 5 && net->N <= kLdsRowsMax;
+  const bool lds_resident = (c.order == 5 && net->N <= kLdsRowsMax) || (c.order == 6 && ls.view.bptr != nullptr);
   f[12] = mfma && c.kernel == COSPDNN_PLAN_MFMA && !lds_resident ? 1.0f : 0.0f;
   // the row-gather kernel and its work split (0: all groups at once, 1: groups per XCD)
   f[13] = rows_k ? 0.5f * static_cast<float>(1 + c.order) : 0.0f;

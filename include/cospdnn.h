@@ -73,7 +73,10 @@ typedef struct {
                             4 block columns partitioned over the XCDs, sample tiles in turn
                             (each XCD's share of the tile image stays in its L2), 5 a whole
                             64-sample input tile in LDS per work-group (layers with at most
-                            1024 input rows; order 1 otherwise) */
+                            1024 input rows; order 1 otherwise), 6 the input tile streamed
+                            through LDS in bands of 512 rows while a work-group accumulates 16
+                            block columns (random layers of at most 16384 neurons, which have
+                            a band image; order 1 otherwise) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */
   int batch_concurrent;  /* several networks in one cospdnn_run: k > 0 = their main passes
                             run in k concurrent chains on side streams (one hipGraph with
@@ -96,6 +99,9 @@ typedef struct {
   int input_bits;        /* FP16 MFMA layer 0 with a binary input: 1 (default) = read the input's
                             bit-packed copy (1 bit per value instead of 16), expanded to
                             FP16 in registers; 0 = read the FP16 input */
+  int weight_mask;       /* FP16 MFMA layers whose weights are all one value: 1 (default) = read the
+                            weight tiles as a bit mask (128 bytes a K chunk instead of 2 KB),
+                            expanded to the same MFMA operands in registers; 0 = the FP16 tiles */
 } cospdnn_opts;
 
 typedef struct {

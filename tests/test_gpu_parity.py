@@ -173,7 +173,7 @@ def test_edge_shapes(engine):
         net.upload(cd.default_opts(max_tile_waste=1e9))
         # every MFMA item order: the XCD-partitioned ones on grids smaller than 8 XCDs too
         for prec, order in [(cd.FP32, 1), (cd.FP16, 0), (cd.FP16, 1), (cd.FP16, 2), (cd.FP16, 3), (cd.FP16, 4),
-                            (cd.FP16, 5), (cd.FP64, 1)]:
+                            (cd.FP16, 5), (cd.FP16, 6), (cd.FP64, 1)]:
             out = cd.run([net], cd.default_opts(precision=prec, spmm_order=order))[0]
             assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias, prec, order)
             if prec == cd.FP16:
@@ -186,27 +186,28 @@ def test_edge_shapes(engine):
 @pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded", "random"])
 def test_item_orders_and_grids_agree(engine, kind):
     """Every MFMA work order and grid size, with compacted layers gathered by output position
-    or stored compacted from input positions, and with layer 0 reading the bit-packed or the
-    FP16 input, computes the same layers: categories equal to the oracle's, final activations
+    or stored compacted from input positions, with layer 0 reading the bit-packed or the
+    FP16 input, and with the weights read from the mask image or the FP16 tiles, computes
+    the same layers: categories equal to the oracle's, final activations
     identical (each output is one MFMA chain with the same operands)."""
     N, L, S, bias, seed = (2048, 24, 5000, -0.4, 31) if kind == cd.GEN_BANDED else (1024, 24, 5000, -0.3, 32)
     say(f"orders kind={kind}")
     alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, kind, seed), 0, S)
     net = gen_net(engine, N, L, S, bias, kind, seed, 64.0)
     ref = None
-    for gather, bits in ((1, 1), (0, 1), (1, 0)):
-        for order in (0, 1, 2, 3, 4, 5):
+    for gather, bits, wmask in ((1, 1, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0), (0, 1, 0)):
+        for order in (0, 1, 2, 3, 4, 5, 6):
             for groups in (1, 4):
                 o = cd.default_opts(precision=cd.FP16, spmm_order=order, mfma_groups_per_cu=groups, dump_layer=L - 1,
-                                    spmm_gather=gather, input_bits=bits)
+                                    spmm_gather=gather, input_bits=bits, weight_mask=wmask)
                 out = cd.run([net], o, dump_cap_rows=S)[0]
                 assert out.mfma_layers == L
-                assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, bits, order, groups)
+                assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, bits, wmask, order, groups)
                 got = np.zeros((S, N), np.float32)
                 got[out.dump_ids] = out.dump_vals
                 if ref is None:
                     ref = got
-                assert np.array_equal(got, ref), (gather, bits, order, groups)
+                assert np.array_equal(got, ref), (gather, bits, wmask, order, groups)
     net.close()
 
 

@@ -96,7 +96,7 @@ def test_plans_keep_categories_exact(engine):
         assert out.mfma_layers == want_mfma
         assert np.array_equal(out.categories, np.flatnonzero(alive)), plan[0]
     with pytest.raises(cd.CospdnnError):
-        net.set_plan([(cd.PLAN_MFMA, 6, 4)] * L)  # bad order
+        net.set_plan([(cd.PLAN_MFMA, 7, 4)] * L)  # bad order
     with pytest.raises(cd.CospdnnError):
         net.set_plan([(5, 1, 4)] * L)  # bad kernel
     net.set_plan(None)
