@@ -3,6 +3,7 @@
 // networks in one run on one GPU.  The reference requires equal layer counts
 // (cospdnn_double.cu:243-248); --allow-uneven lifts that.
 // Options: as cospdnn.out, plus --allow-uneven.
+#include <algorithm>
 #include <fstream>
 
 #include "cli_common.hpp"
@@ -50,7 +51,8 @@ int main(int argc, char* argv[]) {
   c1.resize(r[0].n_categories);
   c2.resize(r[1].n_categories);
   std::cout << "categories = " << c1.size() << " and " << c2.size() << std::endl;
-  std::cout << "Total Kernel Exec Time = " << (r[0].ms_total + r[1].ms_total) << "ms" << std::endl;
+  // one concurrent run: both results report its time
+  std::cout << "Total Kernel Exec Time = " << std::max(r[0].ms_total, r[1].ms_total) << "ms" << std::endl;
   if (!f.categories.empty()) {
     std::ofstream out(f.categories);
     for (int64_t id : c1) out << "1\t" << (id + 1) << "\n";

@@ -62,6 +62,24 @@ int graph_challenge_run(const std::vector<std::vector<float>>& input, const std:
                         const std::vector<std::vector<int>>& fuse_list, int batch, int neuron, float bias,
                         int blocksize_c, int blocksize_r, int gpu_id, int gpu_num, CompatNetResult* out);
 
+// Two networks on one engine in one concurrent cospdnn_run (test_graph_challenge_tc_co's
+// inputs as arrays of two); out[k] for network k.
+int graph_challenge_run_co(const std::vector<std::vector<float>>* input, const std::vector<std::vector<float>>* weight,
+                           const std::vector<std::vector<int>>* row_idx, const std::vector<std::vector<int>>* ptr,
+                           const std::vector<std::vector<int>>* fuse_list, const int* batch, const int* neuron,
+                           const float* bias, int blocksize_c, int blocksize_r, int gpu_id, int gpu_num,
+                           CompatNetResult* out);
+
+// The BCSC arrays test_graph_challenge_tc takes, from a layer in CSC (column j = output j, rows
+// in the fused row order of the dataset files): block column c = outputs [c bs_c, (c+1) bs_c),
+// one bs_c x bs_r block per row window [w bs_r, (w+1) bs_r) the block column reads, values
+// [out][in] inside a block (utils/matrix.h:1245), block start rows, block-column pointers.
+// Blocks start at multiples of bs_r (the reference's builder may also start them elsewhere,
+// matrix.h:1231-1239; its kernel and this engine read either).
+int reference_bcsc_from_csc(const std::vector<int32_t>& colptr, const std::vector<int32_t>& rows,
+                            const std::vector<float>& vals, int neuron, int blocksize_c, int blocksize_r,
+                            std::vector<float>& values, std::vector<int>& row_idx, std::vector<int>& ptr);
+
 // Reference dataset readers (src/cospdnn.cpp:14-79, utils/matrix.h:270-303).  They return a
 // cospdnn status code; `message` receives the text the reference prints on failure.
 int read_input_tsv(const std::string& dir, int neuron, int batch, std::vector<std::vector<float>>& input,

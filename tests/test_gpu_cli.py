@@ -63,3 +63,37 @@ def test_concurrent_task_cli(tmp_path):
     r = subprocess.run([os.path.join(PKG, "cospdnn_double.out"), "1024", "500", "8", "4096", "500", "12", "0",
                         str(tmp_path)], capture_output=True, text=True, timeout=300)
     assert "must have the same number of layers" in r.stderr
+
+
+@pytest.mark.parametrize("kind", [cd.GEN_BANDED, cd.GEN_RANDOM], ids=["banded_fused", "random"])
+def test_cpp_mirror_single(tmp_path, kind):
+    """graph_challenge_tc.out uses the engine only through the C++ mirror, as the reference's
+    main() does (src/cospdnn.cpp:80-188): reads the dataset, builds the 16x16 BCSC arrays and the
+    fuse lists, calls test_graph_challenge_tc; categories equal to the FP64 oracle's."""
+    N, L, S, seed = 1024, 14, 2500, 47
+    print(f"mirror single kind={kind}", flush=True)
+    write_dataset(str(tmp_path), N, L, S, kind, seed)
+    want = oracle_categories(N, L, S, -0.3, orc.BANDED if kind else orc.RANDOM, seed)
+    cats = tmp_path / "cats.txt"
+    r = subprocess.run([os.path.join(PKG, "graph_challenge_tc.out"), str(N), str(S), str(L), "1", "0", str(tmp_path),
+                        f"--categories={cats}"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "Kernel Exec Time = " in r.stdout and "[END]..." in r.stdout
+    assert np.array_equal(read_ids(cats), want)
+
+
+def test_cpp_mirror_concurrent(tmp_path):
+    """graph_challenge_tc_co.out: two networks through test_graph_challenge_tc_co (one engine, one
+    concurrent run), as the reference's concurrent main() (src/cospdnn_double.cpp:81-197)."""
+    seed = 48
+    print("mirror double", flush=True)
+    write_dataset(str(tmp_path), 1024, 12, 2000, cd.GEN_RANDOM, seed)
+    write_dataset(str(tmp_path), 4096, 12, 1500, cd.GEN_BANDED, seed)
+    cats = tmp_path / "cats.txt"
+    r = subprocess.run([os.path.join(PKG, "graph_challenge_tc_co.out"), "1024", "2000", "12", "4096", "1500", "12",
+                        "0", str(tmp_path), f"--categories={cats}"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "Total Kernel Exec Time = " in r.stdout
+    got = np.loadtxt(cats, dtype=np.int64, ndmin=2)
+    assert np.array_equal(got[got[:, 0] == 1, 1], oracle_categories(1024, 12, 2000, -0.3, orc.RANDOM, seed))
+    assert np.array_equal(got[got[:, 0] == 2, 1], oracle_categories(4096, 12, 1500, -0.35, orc.BANDED, seed))
