@@ -47,6 +47,7 @@ class Opts(ctypes.Structure):
         ("spmm_gather", ctypes.c_int),
         ("input_bits", ctypes.c_int),
         ("weight_mask", ctypes.c_int),
+        ("weight_budget_mb", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,7 @@ class Result(ctypes.Structure):
         ("alg_bytes", ctypes.c_double),
         ("mfma_layers", ctypes.c_int64),
         ("positions", _c_i64p),
+        ("streamed_layers", ctypes.c_int64),
     ]
 
 
@@ -362,7 +364,7 @@ def _run_with(nets, dump_cap_rows, call):
         out.positions = pos.copy()
         out.layer_ms = lms[: nets[i].L].copy()
         for f in ("ms_total", "ms_spmm", "ms_recompute", "n_flagged", "n_replayed", "n_changed", "edges", "edges_per_sec",
-                  "alg_bytes", "mfma_layers", "dump_rows"):
+                  "alg_bytes", "mfma_layers", "dump_rows", "streamed_layers"):
             setattr(out, f, getattr(r, f))
         if dv is not None:
             out.dump_vals = dv[: r.dump_rows].copy()

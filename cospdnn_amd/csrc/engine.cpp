@@ -129,8 +129,28 @@ void remap_inputs(BlockedLayer& bl, const std::vector<int32_t>& prev_order, int 
   for (int32_t& g : bl.gidx) g = row_of[g];
 }
 
-// Device images of one layer from its host images (built by build_layer, inputs remapped).
-int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stable) {
+// The streamed MFMA image of a layer in pinned host memory: kptr | gidx | mask words (or FP16
+// fragments when the layer has no mask), each part on a 256-byte boundary.
+int stage_streamed_image(LayerStore& ls, const BlockedLayer& bl) {
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  ls.stream_mask = bl.has_mask;
+  ls.off_gidx = al(bl.kptr.size() * sizeof(int32_t));
+  ls.off_w = ls.off_gidx + al(bl.gidx.size() * sizeof(int32_t));
+  const size_t wbytes = bl.has_mask ? bl.wmask.size() * sizeof(uint16_t) : bl.wt.size() * sizeof(uint16_t);
+  ls.img_bytes = ls.off_w + al(wbytes);
+  HIP_TRY(hipHostMalloc(&ls.host_img, ls.img_bytes, hipHostMallocDefault));
+  char* h = static_cast<char*>(ls.host_img);
+  std::memcpy(h, bl.kptr.data(), bl.kptr.size() * sizeof(int32_t));
+  std::memcpy(h + ls.off_gidx, bl.gidx.data(), bl.gidx.size() * sizeof(int32_t));
+  std::memcpy(h + ls.off_w, bl.has_mask ? static_cast<const void*>(bl.wmask.data()) : static_cast<const void*>(bl.wt.data()),
+              wbytes);
+  ls.streamed = true;
+  return COSPDNN_OK;
+}
+
+// Device images of one layer from its host images (built by build_layer, inputs remapped);
+// streamed: the MFMA image goes to pinned host memory instead (weight budget exceeded).
+int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stable, bool streamed) {
   auto ls = std::make_unique<LayerStore>();
   // the first N slots hold a permutation of the neurons, the rest is padding
   std::vector<int32_t> rows(bl.out_idx.size());
@@ -151,7 +171,13 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
   ls->fp16_exact = bl.fp16_exact;
   ls->nnz = bl.nnz;
   ls->sat_stable = sat_stable;
-  if (bl.has_tiles) {
+  if (bl.has_tiles && streamed) {
+    CALL_TRY(stage_streamed_image(*ls, bl));
+    if (bl.has_mask) ls->view.wmask_val = bl.wmask_val;
+    ls->view.one_chunk = bl.nchunk == bl.nblk ? 1 : 0;
+    ls->view.full_blocks = std::all_of(bl.out_idx.begin(), bl.out_idx.end(), [](int32_t o) { return o >= 0; }) ? 1 : 0;
+    ++net->streamed_layers;
+  } else if (bl.has_tiles) {
     CALL_TRY(dupload(ls->kptr, bl.kptr));
     CALL_TRY(dupload(ls->gidx, bl.gidx));
     CALL_TRY(dupload(ls->wt, bl.wt));
@@ -283,7 +309,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   auto layer_launch = [&](int l) {
     LayerLaunch x;
     const LayerStore& ls = *net->dl[l];
-    x.mfma = fp16 && ls.has_tiles && cfg.use_mfma;
+    // a streamed image holding mask words serves only passes that read the mask
+    const bool tiles = ls.has_tiles && !(ls.streamed && ls.stream_mask && !cfg.wmask);
+    x.mfma = fp16 && tiles && cfg.use_mfma;
     x.order = cfg.order;
     x.groups = cfg.groups_per_cu;
     bool compact = false;
@@ -297,7 +325,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         x.groups = pl.groups_per_cu;
       }
       if (pl.kernel == COSPDNN_PLAN_MFMA || pl.kernel == COSPDNN_PLAN_MFMA_COMPACT) {
-        x.mfma = ls.has_tiles;
+        x.mfma = tiles;
         x.order = pl.order;
         x.groups = pl.groups_per_cu;
         compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
@@ -308,6 +336,25 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     x.gather = x.mfma && cfg.gather && !compact && x.order != 5 && x.order != 6;
     return x;
   };
+  // Streamed MFMA images (weight budget exceeded): layer l's image is copied into slot l % 2 on
+  // the copy stream, one layer ahead, after the SpMM that last read the slot; the SpMM waits for
+  // its copy.  The copy stream forks from st and joins it at the end (a captured graph holds both).
+  const bool streaming = fp16 && net->streamed_layers > 0 && net->copy_st != nullptr;
+  auto streamed_mfma = [&](int l) { return l < L && net->dl[l]->streamed && layer_launch(l).mfma; };
+  auto stream_copy = [&](int l) -> int {
+    if (!streamed_mfma(l)) return COSPDNN_OK;
+    const LayerStore& ls = *net->dl[l];
+    HIP_TRY(hipStreamWaitEvent(net->copy_st, net->slot_free[l & 1], 0));
+    HIP_TRY(hipMemcpyAsync(net->wslot[l & 1].p, ls.host_img, ls.img_bytes, hipMemcpyHostToDevice, net->copy_st));
+    HIP_TRY(hipEventRecord(net->slot_copied[l & 1], net->copy_st));
+    return COSPDNN_OK;
+  };
+  if (streaming) {
+    HIP_TRY(hipEventRecord(net->slot_free[0], st));
+    HIP_TRY(hipEventRecord(net->slot_free[1], st));
+    HIP_TRY(hipStreamWaitEvent(net->copy_st, net->slot_free[0], 0));
+    CALL_TRY(stream_copy(0));
+  }
   for (int l = 0; l <= L; ++l) {
     const int cur = l & 1, nxt = cur ^ 1;
     ScanArgs sa;
@@ -359,6 +406,19 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     // the layer's images; the mask image only where the pass reads it
     DevLayer view = ls.view;
     if (!(fp16 && cfg.wmask)) view.wmask = view.bmask = nullptr;
+    const bool slot_l = streaming && streamed_mfma(l);
+    if (streaming) CALL_TRY(stream_copy(l + 1));
+    if (slot_l) {
+      // this layer's MFMA image is in slot l % 2 once its copy is done
+      char* base = static_cast<char*>(net->wslot[l & 1].p);
+      view.kptr = reinterpret_cast<const int32_t*>(base);
+      view.gidx = reinterpret_cast<const int32_t*>(base + ls.off_gidx);
+      if (ls.stream_mask)
+        view.wmask = reinterpret_cast<const uint16_t*>(base + ls.off_w);
+      else
+        view.wt = reinterpret_cast<const _Float16*>(base + ls.off_w);
+      HIP_TRY(hipStreamWaitEvent(st, net->slot_copied[l & 1], 0));
+    }
     StepArgs a;
     const bool fp64 = cfg.precision == COSPDNN_FP64;
     auto side_ptr = [&](int sd) -> void* { return fp16 ? rb.act16[sd].p : (fp64 ? rb.act64[sd].p : rb.act32[sd].p); };
@@ -444,6 +504,11 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
       HIP_TRY(launch_spmm_simt(a, view, Dtype::F32, Dtype::F32, capg(grid_for(e, e->bpc_simt32, S_run, nblk)), st));
     }
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
+    if (slot_l) HIP_TRY(hipEventRecord(net->slot_free[l & 1], st));
+  }
+  if (streaming) {
+    HIP_TRY(hipEventRecord(net->copy_done, net->copy_st));
+    HIP_TRY(hipStreamWaitEvent(st, net->copy_done, 0));
   }
   *cat_side = (L & 1) ^ 1;
   return COSPDNN_OK;
@@ -1026,15 +1091,17 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
     const double lv = live[l] >= 0 ? static_cast<double>(live[l]) : static_cast<double>(S);
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
     // the FP32 path takes the FP32 MFMA kernel only where the layer has an FP32 tile image
-    bool m = (o.precision == COSPDNN_FP16 && cfg.use_mfma && net->dl[l]->has_tiles) ||
-             (o.precision == COSPDNN_FP32 && cfg.mfma32 && net->dl[l]->has_tiles32);
+    const LayerStore& lsl = *net->dl[l];
+    const bool tiles16 = lsl.has_tiles && !(lsl.streamed && lsl.stream_mask && !cfg.wmask);
+    bool m = (o.precision == COSPDNN_FP16 && cfg.use_mfma && tiles16) ||
+             (o.precision == COSPDNN_FP32 && cfg.mfma32 && lsl.has_tiles32);
     if (o.precision == COSPDNN_FP16 && !net->plan.empty() && net->plan[l].kernel != COSPDNN_PLAN_AUTO)
-      m = net->dl[l]->has_tiles &&
-          (net->plan[l].kernel == COSPDNN_PLAN_MFMA || net->plan[l].kernel == COSPDNN_PLAN_MFMA_COMPACT);
+      m = tiles16 && (net->plan[l].kernel == COSPDNN_PLAN_MFMA || net->plan[l].kernel == COSPDNN_PLAN_MFMA_COMPACT);
     mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;
   res->mfma_layers = mfma_layers;
+  res->streamed_layers = net->streamed_layers;
   res->ms_spmm = 0.0;
   if (o.layer_timing != 0) {
     double sum = 0.0;
@@ -1377,6 +1444,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->spmm_gather = 1;
   o->input_bits = 1;
   o->weight_mask = 1;
+  o->weight_budget_mb = 0;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {
@@ -1582,12 +1650,15 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
     // generate (when synthetic) and build the host images of a batch of layers on host
     // threads, one layer per thread, then upload them in order
     const int L = net->L;
+    const double budget = o.weight_budget_mb > 0 ? static_cast<double>(o.weight_budget_mb) * 1048576.0
+                                                  : kTileBudget * static_cast<double>(net->eng->hbm_bytes);
+    net->streamed_layers = 0;
     const int batch = std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
     std::vector<int32_t> carry;  // out_idx of the last layer of the previous batch
     for (int l0 = 0; l0 < L; l0 += batch) {
       const int l1 = std::min(L, l0 + batch);
       std::vector<BlockedLayer> bl(l1 - l0);
-      std::vector<char> ok(l1 - l0, 0), stable(l1 - l0, 0);
+      std::vector<char> ok(l1 - l0, 0), stable(l1 - l0, 0), streamed(l1 - l0, 0);
       std::vector<std::thread> th;
       for (int l = l0; l < l1; ++l)
         th.emplace_back([&, l]() {
@@ -1603,19 +1674,17 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
       for (auto& t : th) t.join();
       for (int l = l0; l < l1; ++l) {
         if (!ok[l - l0]) return fail(COSPDNN_ERR_FORMAT, "layer " + std::to_string(l) + ": malformed CSC");
-        // tile images of random layers are ~32x their SIMT image: past kTileBudget of the
-        // HBM (all of this engine's networks together) layers keep the SIMT image only
+        // tile images of random layers are ~32x their SIMT image: past the weight budget (all
+        // of this engine's networks together) a layer's MFMA image is streamed from pinned host
+        // memory instead (its FP32 image is dropped: the FP32 tier takes the SIMT kernel there)
         BlockedLayer& b = bl[l - l0];
         if (b.has_tiles) {
           const int64_t tb = static_cast<int64_t>(b.wt.size()) * 2 + static_cast<int64_t>(b.wt32.size()) * 4 +
                              static_cast<int64_t>(b.gidx.size()) * 4 + static_cast<int64_t>(b.wmask.size()) * 2;
-          if (static_cast<double>(net->eng->tile_bytes + tb) > kTileBudget * static_cast<double>(net->eng->hbm_bytes)) {
-            b.has_tiles = b.has_tiles32 = b.has_mask = false;
-            b.wmask.clear();
-            b.wt.clear();
+          if (static_cast<double>(net->eng->tile_bytes + tb) > budget) {
+            streamed[l - l0] = 1;
+            b.has_tiles32 = false;
             b.wt32.clear();
-            b.gidx.clear();
-            b.kptr.clear();
           } else {
             net->eng->tile_bytes += tb;
             net->tile_bytes += tb;
@@ -1629,7 +1698,8 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
         th.emplace_back([&, l]() {
           BlockedLayer& b = bl[l - l0];
           if (l >= 1) remap_inputs(b, l > l0 ? bl[l - 1 - l0].out_idx : carry, net->N);
-          if (b.has_tiles && b.tile_waste > kBandMinWaste && net->N <= kBandMaxN) build_band_image(b, net->N);
+          if (b.has_tiles && !streamed[l - l0] && b.tile_waste > kBandMinWaste && net->N <= kBandMaxN)
+            build_band_image(b, net->N);
         });
       for (auto& t : th) t.join();
       for (int l = l0; l < l1; ++l) {
@@ -1638,7 +1708,7 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
         const int64_t bb = static_cast<int64_t>(b.bgidx.size()) * 4 + static_cast<int64_t>(b.bptr.size()) * 4 +
                            (b.bmask.empty() ? static_cast<int64_t>(b.bwt.size()) * 2
                                             : static_cast<int64_t>(b.bmask.size()) * 2);
-        if (static_cast<double>(net->eng->tile_bytes + bb) > kTileBudget * static_cast<double>(net->eng->hbm_bytes)) {
+        if (static_cast<double>(net->eng->tile_bytes + bb) > budget) {
           b.has_band = false;
         } else {
           net->eng->tile_bytes += bb;
@@ -1646,11 +1716,22 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
         }
       }
       for (int l = l0; l < l1; ++l) {
-        CALL_TRY(upload_layer(net, l, bl[l - l0], stable[l - l0] != 0));
+        CALL_TRY(upload_layer(net, l, bl[l - l0], stable[l - l0] != 0, streamed[l - l0] != 0));
         if (!net->gen_layers) net->layers[l] = CscLayer();
       }
       carry = std::move(bl[l1 - 1 - l0].out_idx);
     }
+  }
+  if (net->streamed_layers > 0) {
+    // two device slots of the largest streamed image, the copy stream and its events
+    size_t most = 0;
+    for (const auto& ls : net->dl)
+      if (ls->streamed) most = std::max(most, ls->img_bytes);
+    for (auto& slot : net->wslot) CALL_TRY(dalloc(slot, most));
+    if (!net->copy_st) HIP_TRY(hipStreamCreateWithFlags(&net->copy_st, hipStreamNonBlocking));  // default priority
+    for (hipEvent_t* ev : {&net->slot_copied[0], &net->slot_copied[1], &net->slot_free[0], &net->slot_free[1],
+                           &net->copy_done})
+      if (!*ev) HIP_TRY(hipEventCreateWithFlags(ev, hipEventDisableTiming));
   }
   CALL_TRY(dalloc(net->store, sizeof(_Float16) * static_cast<size_t>(net->N) * net->ld_store));
   if (net->input_mode == 1) {

@@ -81,6 +81,19 @@ struct LayerStore {
   bool fp16_exact = true;
   bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
   int64_t nnz = 0;
+  // Streamed MFMA image (the weight budget was exceeded): kptr | gidx | mask words (or FP16
+  // fragments) in pinned host memory, copied into one of the network's two device slots one
+  // layer ahead of its SpMM (the reference streams every layer so, cospdnn.cu:170-210)
+  bool streamed = false;
+  bool stream_mask = false;  // the streamed image holds mask words (else FP16 fragments)
+  void* host_img = nullptr;  // hipHostMalloc
+  size_t img_bytes = 0, off_gidx = 0, off_w = 0;
+  LayerStore() = default;
+  LayerStore(const LayerStore&) = delete;
+  LayerStore& operator=(const LayerStore&) = delete;
+  ~LayerStore() {
+    if (host_img) (void)hipHostFree(host_img);
+  }
 };
 
 struct RunBufs {
@@ -209,7 +222,16 @@ struct cospdnn_net {
   std::vector<hipEvent_t> layer_ev;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   double est_main_ms = -1.0;  // last measured main-pass time (chains of a concurrent run)
+  // weight streaming: device slots (streamed layer l uses slot l % 2), the copy stream and its
+  // events (slot contents copied / slot free again / all copies of a pass done)
+  int64_t streamed_layers = 0;
+  DevBuf wslot[2];
+  hipStream_t copy_st = nullptr;
+  hipEvent_t slot_copied[2] = {nullptr, nullptr}, slot_free[2] = {nullptr, nullptr}, copy_done = nullptr;
   ~cospdnn_net() {
+    for (hipEvent_t e : {slot_copied[0], slot_copied[1], slot_free[0], slot_free[1], copy_done})
+      if (e) (void)hipEventDestroy(e);
+    if (copy_st) (void)hipStreamDestroy(copy_st);
     for (hipEvent_t e : layer_ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);

@@ -102,6 +102,11 @@ typedef struct {
   int weight_mask;       /* FP16 MFMA layers whose weights are all one value: 1 (default) = read the
                             weight tiles as a bit mask (128 bytes a K chunk instead of 2 KB),
                             expanded to the same MFMA operands in registers; 0 = the FP16 tiles */
+  int weight_budget_mb;  /* upload: MFMA weight images resident in HBM, all of the engine's networks
+                            together, at most this many MiB (0, the default: 40% of the HBM);
+                            the layers past it keep their images in pinned host memory and the
+                            FP16 pass streams them into two device slots, one layer ahead of
+                            its SpMM, on a copy stream (eager and in captured graphs) */
 } cospdnn_opts;
 
 typedef struct {
@@ -128,6 +133,7 @@ typedef struct {
   int64_t* positions;    /* caller-provided (may be NULL), capacity L+1: row positions entering
                             layer l in the first pass, live rows plus the holes of a kept
                             layout (keep_holes_ppm) */
+  int64_t streamed_layers; /* layers of the network whose MFMA images are streamed from host memory */
 } cospdnn_result;
 
 const char* cospdnn_version(void);

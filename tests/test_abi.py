@@ -58,3 +58,5 @@ def test_options_struct_matches_the_header():
     assert o.keep_holes_ppm == 0
     assert o.spmm_gather == 1
     assert o.input_bits == 1
+    assert o.weight_mask == 1
+    assert o.weight_budget_mb == 0
