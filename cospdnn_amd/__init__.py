@@ -35,6 +35,7 @@ from ._lib import (  # noqa: F401
     ref_bcsc_to_csc,
     run,
     run_noalloc,
+    similarity_order,
     tune_exhaustive,
     tune_keep,
     KEEP_CANDIDATES,

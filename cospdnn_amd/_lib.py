@@ -95,7 +95,7 @@ EXPORTS = [
     "cospdnn_net_generate_layers", "cospdnn_net_set_input_dense", "cospdnn_net_set_input_coo",
     "cospdnn_net_generate_input", "cospdnn_net_upload", "cospdnn_run", "cospdnn_run_noalloc",
     "cospdnn_generate_layer", "cospdnn_generate_input_rows", "cospdnn_layer_stats", "cospdnn_layer_tiles_decode",
-    "cospdnn_ref_bcsc_to_csc",
+    "cospdnn_ref_bcsc_to_csc", "cospdnn_similarity_order",
     "cospdnn_net_set_plan", "cospdnn_num_candidates", "cospdnn_candidate", "cospdnn_net_layer_features",
     "cospdnn_tune_exhaustive", "cospdnn_tune_keep", "cospdnn_mlp_create", "cospdnn_mlp_destroy", "cospdnn_mlp_num_params",
     "cospdnn_mlp_get_params", "cospdnn_mlp_set_params", "cospdnn_mlp_loss_grad", "cospdnn_mlp_train",
@@ -151,6 +151,7 @@ def lib():
         L.cospdnn_run.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(Opts),
                                   ctypes.POINTER(Result)]
         L.cospdnn_run_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Opts), ctypes.POINTER(Result)]
+        L.cospdnn_similarity_order.argtypes = [ctypes.c_int, _c_i32p, _c_i32p, ctypes.c_int, ctypes.c_int, _c_i32p]
         L.cospdnn_generate_layer.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_float, _c_i32p, _c_i32p, _c_f32p]
         L.cospdnn_generate_input_rows.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
@@ -435,6 +436,17 @@ def ref_bcsc_to_csc(N, values, row_idx, ptr, fuse=None, blocksize_c=16, blocksiz
                                          _p(cp, _c_i32p), _p(rw, _c_i32p), _p(vl, _c_f32p)), "cospdnn_ref_bcsc_to_csc")
     nnz = cp[N]
     return cp, rw[:nnz], vl[:nnz]
+
+
+def similarity_order(N, colptr, rows, group=32, overlap=True):
+    """Output order of one layer that groups outputs reading the same inputs (see
+    cospdnn_similarity_order); order[i] = the output at position i."""
+    cp = np.ascontiguousarray(colptr, dtype=np.int32)
+    rw = np.ascontiguousarray(rows, dtype=np.int32)
+    order = np.zeros(N, dtype=np.int32)
+    _check(lib().cospdnn_similarity_order(N, _p(cp, _c_i32p), _p(rw, _c_i32p), group, 1 if overlap else 0,
+                                          _p(order, _c_i32p)), "cospdnn_similarity_order")
+    return order
 
 
 def candidates():

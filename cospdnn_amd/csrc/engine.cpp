@@ -1786,6 +1786,24 @@ int cospdnn_run_noalloc(cospdnn_net* net, const cospdnn_opts* opts, cospdnn_resu
   return run_noalloc(net, o, result);
 }
 
+int cospdnn_similarity_order(int N, const int32_t* colptr, const int32_t* rows, int group, int overlap,
+                             int32_t* order) {
+  if (N <= 0 || !colptr || !rows || !order || group < 1 || colptr[0] != 0) return fail(COSPDNN_ERR_ARG, "bad argument");
+  std::vector<std::vector<int32_t>> sr(N);
+  for (int j = 0; j < N; ++j) {
+    if (colptr[j + 1] < colptr[j]) return fail(COSPDNN_ERR_FORMAT, "colptr not monotone");
+    sr[j].assign(rows + colptr[j], rows + colptr[j + 1]);
+    for (int32_t r : sr[j])
+      if (r < 0 || r >= N) return fail(COSPDNN_ERR_FORMAT, "row out of range");
+    std::sort(sr[j].begin(), sr[j].end());
+    sr[j].erase(std::unique(sr[j].begin(), sr[j].end()), sr[j].end());
+  }
+  std::vector<int32_t> ord;
+  similarity_order(sr, N, group, overlap != 0, ord);
+  std::copy(ord.begin(), ord.end(), order);
+  return COSPDNN_OK;
+}
+
 int cospdnn_generate_layer(uint64_t seed, int kind, int N, int K, int layer, float wval, int32_t* colptr,
                            int32_t* rows, float* vals) {
   if (!colptr || !rows || !vals || N <= 0 || K <= 0 || K > N || layer < 0) return fail(COSPDNN_ERR_ARG, "bad argument");

@@ -66,6 +66,102 @@ uint64_t set_signature(const int32_t* r, int n) {
 
 }  // namespace
 
+void similarity_order(const std::vector<std::vector<int32_t>>& sorted_rows, int N, int group, bool overlap,
+                      std::vector<int32_t>& order) {
+  order.clear();
+  order.reserve(N);
+  std::vector<char> placed(N, 0);
+  if (!overlap) {
+    // identical input sets only: walk outputs in natural order, each followed by the unplaced
+    // outputs with the same set
+    std::unordered_map<uint64_t, std::vector<int32_t>> groups;
+    std::vector<uint64_t> sig(N);
+    for (int j = 0; j < N; ++j) {
+      sig[j] = set_signature(sorted_rows[j].data(), static_cast<int>(sorted_rows[j].size()));
+      groups[sig[j]].push_back(j);
+    }
+    for (int j = 0; j < N; ++j) {
+      if (placed[j]) continue;
+      for (int32_t k : groups[sig[j]])
+        if (!placed[k] && sorted_rows[k] == sorted_rows[j]) {
+          placed[k] = 1;
+          order.push_back(k);
+        }
+    }
+    return;
+  }
+  // Greedy overlap grouping.  Each group starts with the lowest-numbered unplaced output and
+  // takes, one at a time, an unplaced output that shares the most rows with the group's union
+  // so far (among equals the one whose score reached that count last; no overlap left: the
+  // next unplaced output in natural order).  Scores live in a bucket queue indexed by the shared-row count, fed by an
+  // inverted index (row -> outputs), so a group costs the outputs its union's rows touch.
+  std::vector<int32_t> rptr(N + 1, 0), rout;
+  for (int j = 0; j < N; ++j)
+    for (int32_t r : sorted_rows[j]) ++rptr[r + 1];
+  for (int r = 0; r < N; ++r) rptr[r + 1] += rptr[r];
+  rout.resize(rptr[N]);
+  {
+    std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int32_t r : sorted_rows[j]) rout[fill[r]++] = j;
+  }
+  std::vector<int32_t> score(N, 0), touched;
+  std::vector<char> in_union(N, 0);
+  std::vector<int32_t> union_rows;
+  std::vector<std::vector<int32_t>> bucket;  // bucket[s]: outputs whose score reached s (lazy)
+  int next_free = 0;
+  auto add_rows = [&](int32_t j) {
+    for (int32_t r : sorted_rows[j]) {
+      if (in_union[r]) continue;
+      in_union[r] = 1;
+      union_rows.push_back(r);
+      for (int32_t q = rptr[r]; q < rptr[r + 1]; ++q) {
+        const int32_t o = rout[q];
+        if (placed[o]) continue;
+        if (score[o] == 0) touched.push_back(o);
+        const int s = ++score[o];
+        if (static_cast<int>(bucket.size()) <= s) bucket.resize(s + 1);
+        bucket[s].push_back(o);
+      }
+    }
+  };
+  while (static_cast<int>(order.size()) < N) {
+    while (placed[next_free]) ++next_free;
+    int32_t j = next_free;
+    int top = 0;
+    for (int k = 0; k < group && static_cast<int>(order.size()) < N; ++k) {
+      if (k > 0) {
+        // an unplaced output with the highest score: the latest valid entry of the highest
+        // non-empty bucket (stale entries, placed or since moved up, are dropped on the way)
+        j = -1;
+        while (top > 0 && j < 0) {
+          std::vector<int32_t>& b = bucket[top];
+          while (!b.empty() && (placed[b.back()] || score[b.back()] != top)) b.pop_back();
+          if (b.empty()) {
+            --top;
+          } else {
+            j = b.back();
+            b.pop_back();
+          }
+        }
+        if (j < 0) {
+          while (placed[next_free]) ++next_free;
+          j = next_free;
+        }
+      }
+      placed[j] = 1;
+      order.push_back(j);
+      add_rows(j);
+      top = static_cast<int>(bucket.size()) - 1;
+    }
+    for (int32_t o : touched) score[o] = 0;
+    touched.clear();
+    for (int32_t r : union_rows) in_union[r] = 0;
+    union_rows.clear();
+    for (auto& b : bucket) b.clear();
+  }
+}
+
 bool all32_fixed_point(const CscLayer& w, int N, float bias) {
   if (N <= 0 || static_cast<int>(w.colptr.size()) != N + 1) return false;
   const double u = std::ldexp(1.0, -24);
@@ -124,31 +220,17 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
   out.nnz = nnz;
   out.nblk = (N + kBlockOut - 1) / kBlockOut;
 
-  // Column order: walk columns in natural order and emit every column that
-  // shares the current column's input set right after it.
+  // Column order: outputs that read the same inputs go to the same block column (see
+  // similarity_order); the tile image then gathers each shared input row once.
   std::vector<std::vector<int32_t>> sorted_rows(N);
-  std::unordered_map<uint64_t, std::vector<int32_t>> groups;
-  std::vector<uint64_t> sig(N);
   for (int j = 0; j < N; ++j) {
     std::vector<int32_t>& r = sorted_rows[j];
     r.assign(w.rows.begin() + w.colptr[j], w.rows.begin() + w.colptr[j + 1]);
     std::sort(r.begin(), r.end());
     r.erase(std::unique(r.begin(), r.end()), r.end());
-    sig[j] = set_signature(r.data(), static_cast<int>(r.size()));
-    groups[sig[j]].push_back(j);
   }
   std::vector<int32_t> order;
-  order.reserve(N);
-  std::vector<char> placed(N, 0);
-  for (int j = 0; j < N; ++j) {
-    if (placed[j]) continue;
-    for (int32_t k : groups[sig[j]]) {
-      if (!placed[k] && sorted_rows[k] == sorted_rows[j]) {
-        placed[k] = 1;
-        order.push_back(k);
-      }
-    }
-  }
+  similarity_order(sorted_rows, N, kBlockOut, N <= kSimilarityMaxN, order);
 
   out.out_idx.assign(static_cast<size_t>(out.nblk) * kBlockOut, -1);
   for (int i = 0; i < N; ++i) out.out_idx[i] = order[i];

@@ -72,6 +72,17 @@ struct BlockedLayer {
 // FP32 tile images are built only up to this waste (they serve the FP32 guard pass).
 constexpr double kMaxWasteFp32Tiles = 4.0;
 
+// Output order of a layer (sorted_rows[j]: the sorted distinct input rows of output j) for
+// block columns of `group` outputs.  overlap = false: outputs with identical input sets follow
+// each other, otherwise natural order.  overlap = true: greedy grouping by shared inputs (each
+// group seeded with the lowest-numbered free output, then a free output sharing the most rows
+// with the group's union), which also puts identical sets together.
+void similarity_order(const std::vector<std::vector<int32_t>>& sorted_rows, int N, int group, bool overlap,
+                      std::vector<int32_t>& order);
+// Layers up to this many neurons are grouped by overlap (upload cost grows with N times the
+// union size); larger ones by identical input sets.
+constexpr int kSimilarityMaxN = 16384;
+
 // Builds both images. Tiles are built only when tile_waste <= max_waste.
 // Returns false (and leaves `out` unspecified) if the CSC is malformed.
 bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out);

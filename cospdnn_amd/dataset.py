@@ -7,7 +7,8 @@ Layout (README.md:5-9 and src/cospdnn.cpp:14-79 of the reference):
 
 Banded layers are stored in the fused row order the similarity transform
 produces: the 32 inputs read by one group of outputs are consecutive fused rows,
-and the fuse list maps them back to neurons.  Random layers use the identity.
+and the fuse list maps them back to neurons.  Random layers use the identity, and so does
+every layer with fused=False (the raw network, as cospdnn_amd.transform takes it).
 """
 import os
 
@@ -22,7 +23,7 @@ def banded_fuse(N, K=32):
     return ((r % K) * s + r // K).astype(np.int64)
 
 
-def write_dataset(path, N, L, S, kind, seed, K=32, wval=0.0625, chunk=4096):
+def write_dataset(path, N, L, S, kind, seed, K=32, wval=0.0625, chunk=4096, fused=True):
     os.makedirs(os.path.join(path, f"neuron{N}"), exist_ok=True)
     os.makedirs(os.path.join(path, "fuse_matrix", f"neuron{N}"), exist_ok=True)
     with open(os.path.join(path, f"sparse-images-{N}.tsv"), "w") as f:
@@ -30,7 +31,7 @@ def write_dataset(path, N, L, S, kind, seed, K=32, wval=0.0625, chunk=4096):
             rows = _lib.generate_input_rows(seed, N, s0, min(chunk, S - s0))
             ss, nn = np.nonzero(rows)
             f.write("".join(f"{s0 + a + 1}\t{b + 1}\t1\n" for a, b in zip(ss.tolist(), nn.tolist())))
-    fuse = banded_fuse(N, K) if kind == _lib.GEN_BANDED else np.arange(N, dtype=np.int64)
+    fuse = banded_fuse(N, K) if kind == _lib.GEN_BANDED and fused else np.arange(N, dtype=np.int64)
     inv = np.empty(N, dtype=np.int64)
     inv[fuse] = np.arange(N)
     with open(os.path.join(path, "fuse_matrix", f"neuron{N}", f"{N}_{L}_fuse.tsv"), "w") as ff:

@@ -198,6 +198,14 @@ int cospdnn_layer_stats(int N, const int32_t* colptr, const int32_t* rows, const
    returns the triple count in *n_out (capacity cap); used to pin the tile layout */
 int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals,
                                int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out);
+/* Similarity order of one layer (host): the output order that puts outputs reading the same
+   inputs into the same group of `group` consecutive outputs; order[i] = the output at position i.
+   overlap = 1: greedy by shared inputs (each group seeded with the lowest-numbered free output,
+   then a free output sharing the most inputs with the group's union); 0: identical input sets
+   only.  The transform the reference expects done offline (README.md:9, the fuse lists read at
+   cospdnn.cpp:14-44); cospdnn_amd/transform.py applies it to a dataset directory. */
+int cospdnn_similarity_order(int N, const int32_t* colptr, const int32_t* rows, int group, int overlap,
+                             int32_t* order);
 /* reference BCSC + fuse -> CSC (host); colptr capacity N+1, rows/vals capacity cap */
 int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const int32_t* row_idx,
                             int64_t n_row_idx, const int32_t* ptr, int64_t n_ptr, const int32_t* fuse,

@@ -60,7 +60,8 @@ def test_tile_waste():
     cp, rw, vl = cd.generate_layer(3, cd.GEN_RANDOM, 1024, 32, 0)
     st = cd.layer_stats(1024, cp, rw, vl)
     # union of 32 columns x 32 random rows of 1024 ~ 1024 * (1 - e^-1) ~ 647 -> ~21 chunks per block
-    assert 18 < st["tile_waste"] < 23
+    # in natural order; grouping by shared rows brings it to ~17.4
+    assert 15 < st["tile_waste"] < 19.5
     st = cd.layer_stats(4, np.array([0, 1, 2, 3, 4], np.int32), np.array([0, 1, 2, 3], np.int32),
                         np.array([0.1, 1, 1, 1], np.float32))
     assert not st["fp16_exact"]

@@ -97,3 +97,24 @@ def test_cpp_mirror_concurrent(tmp_path):
     got = np.loadtxt(cats, dtype=np.int64, ndmin=2)
     assert np.array_equal(got[got[:, 0] == 1, 1], oracle_categories(1024, 12, 2000, -0.3, orc.RANDOM, seed))
     assert np.array_equal(got[got[:, 0] == 2, 1], oracle_categories(4096, 12, 1500, -0.35, orc.BANDED, seed))
+
+
+def test_transformed_dataset_runs_through_both_drivers(tmp_path):
+    """A banded network written with natural neuron labels, run through the similarity transform
+    (cospdnn_amd.transform: relabelled neurons, fuse lists), then through cospdnn.out (fuse lists
+    read as the reference does) and through graph_challenge_tc.out (the reference's 16x16 BCSC
+    built from the fused files): categories equal the FP64 oracle's on the raw network."""
+    from cospdnn_amd import transform as tf
+
+    N, L, S, seed = 1024, 10, 2000, 49
+    print("transformed dataset", flush=True)
+    src, dst = str(tmp_path / "raw"), str(tmp_path / "fused")
+    write_dataset(src, N, L, S, cd.GEN_BANDED, seed, fused=False)
+    tf.transform_dataset(N, L, src, dst)
+    want = oracle_categories(N, L, S, -0.3, orc.BANDED, seed)
+    for exe in ("cospdnn.out", "graph_challenge_tc.out"):
+        cats = tmp_path / f"cats_{exe}.txt"
+        r = subprocess.run([os.path.join(PKG, exe), str(N), str(S), str(L), "1", "0", dst, f"--categories={cats}"],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+        assert np.array_equal(read_ids(cats), want), exe
