@@ -362,13 +362,17 @@ def bench_batch(cd, np, args, specs):
     seq_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=0)
     outs_s, wall_s = timed_runs_many(cd, nets, seq_opts, args.steps, args.warmup)
     best = None
+    concurrent = {}
     for chains in (2, 3):
-        bat_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=chains)
-        outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
-        log(f"[bench] {chains} chains: {wall_c * 1e3 / args.steps:.2f} ms")
-        if best is None or wall_c < best[1]:
-            best = (outs_c, wall_c, chains)
-    outs_b, wall_b, chains = best
+        for share in (0, 1):
+            bat_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=chains,
+                                       batch_cu_share=share)
+            outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
+            concurrent[f"{chains} chains, cu_share {share}"] = wall_c * 1e3 / args.steps
+            log(f"[bench] {chains} chains, cu_share {share}: {wall_c * 1e3 / args.steps:.2f} ms")
+            if best is None or wall_c < best[1]:
+                best = (outs_c, wall_c, chains, share)
+    outs_b, wall_b, chains, share = best
     same = all(np.array_equal(a.categories, b.categories) for a, b in zip(outs_s[0], outs_b[0]))
     ms_s = wall_s * 1e3 / args.steps
     ms_b = wall_b * 1e3 / args.steps
@@ -392,6 +396,8 @@ def bench_batch(cd, np, args, specs):
                    + ", random 32 nnz/column of 0.0625", "networks": len(specs), "parallelism": "single GPU"},
         "sequential": {"ms_per_step": ms_s, "value": edges / (ms_s * 1e-3)},
         "concurrent_chains": chains,
+        "concurrent_cu_share": share,
+        "concurrent_ms_by_setting": concurrent,
         "concurrent_over_sequential": ms_s / ms_b,
         "categories_equal": same,
         "tuning": tuning,

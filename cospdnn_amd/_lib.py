@@ -47,6 +47,7 @@ class Opts(ctypes.Structure):
         ("spmm_gather", ctypes.c_int),
         ("input_bits", ctypes.c_int),
         ("weight_mask", ctypes.c_int),
+        ("batch_cu_share", ctypes.c_int),
         ("weight_budget_mb", ctypes.c_int),
     ]
 

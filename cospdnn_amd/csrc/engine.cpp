@@ -238,6 +238,7 @@ struct RunCfg {
   int tail = 0;  // guarded passes: rows that die in the last `tail` layers are flagged too
   int stop_after_scan = -1;  // >= 0: the pass ends with the scan of this layer (no rows can be left)
   int grid_cap = 0;          // > 0: at most this many work-groups per SpMM launch
+  int grid_div = 1;          // > 1: SpMM grids divided by this (at least kXcds work-groups)
   int keep_ppm = 0;          // FP16 pass: scans that only drop dead rows keep the layout when
                              // at most this many ppm of the rows are removed (0: never)
   int gather = 0;            // FP16 pass: MFMA layers after a compacting scan gather by output position
@@ -300,7 +301,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   std::vector<char> sat_from(L + 1, 1);
   for (int l = L - 1; l >= 0; --l) sat_from[l] = sat_from[l + 1] && net->dl[l]->sat_stable;
   const bool few = S_run <= kFewRows;
-  auto capg = [&](int g) { return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g; };
+  auto capg = [&](int g) {
+    if (cfg.grid_div > 1) g = std::max(std::min(g, kXcds), g / cfg.grid_div);
+    return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g;
+  };
   // Layer l's SpMM: the options' choice, overridden by the network's per-layer plan (FP16)
   struct LayerLaunch {
     bool mfma = false, gather = false, rows = false;
@@ -432,7 +436,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
     if (l == 0 && lx.mfma && cfg.bits && net->bits.p && input == net->store.as<_Float16>()) {
       a.xbits = net->bits.as<uint32_t>();
-      a.xbits_ld = net->ld_store / 32;
+      a.xbits_rows = net->N;
     }
     a.alive_next = rb.f(nxt, 0);
     a.unsat_next = rb.f(nxt, 1);
@@ -787,6 +791,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
          gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather && gc.bits == cfg.bits && gc.wmask == cfg.wmask &&
+         gc.grid_div == cfg.grid_div &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
 }
@@ -810,6 +815,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.gather = cfg.gather;
   gc.bits = cfg.bits;
   gc.wmask = cfg.wmask;
+  gc.grid_div = cfg.grid_div;
   gc.plan_version = plan_version;
 }
 
@@ -1291,6 +1297,8 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
   std::vector<RunCfg> cfg(n);
   for (int i = 0; i < n; ++i) CALL_TRY(run_prepare(nets[i], o, cfg[i]));
   const int chains = std::max(1, std::min({n, o.batch_concurrent, kMaxSideStreams / 2}));
+  if (o.batch_cu_share != 0 && chains > 1)
+    for (int i = 0; i < n; ++i) cfg[i].grid_div = chains;
   // chain c runs its networks' main passes on side[c] and their guard tiers on side[chains + c]
   CALL_TRY(ensure_side_streams(e, 2 * chains));
   std::vector<char> dev_guard(n, 0);
@@ -1445,6 +1453,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->input_bits = 1;
   o->weight_mask = 1;
   o->weight_budget_mb = 0;
+  o->batch_cu_share = 0;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {

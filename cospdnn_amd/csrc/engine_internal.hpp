@@ -130,7 +130,7 @@ using cospdnn_detail::RunBufs;
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0, gather = 0, bits = 0, wmask = 0;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0, gather = 0, bits = 0, wmask = 0, grid_div = 1;
   int64_t plan_version = -1;
   // the device-side guard pipeline captured with the main pass (0: main pass only)
   int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;
@@ -202,7 +202,7 @@ struct cospdnn_net {
   int64_t ld_store = 0;
   DevBuf packed;       // binary inputs: [S][words] bit-packed sample-major copy of the store
   int words = 0;
-  DevBuf bits;         // binary inputs: [N][ld_store/32] bit-packed neuron-major copy of the store
+  DevBuf bits;         // binary inputs: bit-packed copy of the store, tile-major ([ld_store/64][N][2] words)
   RunBufs main, rec;
   int64_t nnz_total = 0;
   int64_t tile_bytes = 0;  // this network's MFMA tile images (counted in the engine's too)

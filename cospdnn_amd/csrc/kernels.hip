@@ -597,7 +597,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
   if constexpr (BITS) {
     brow1 = gidx[(nx1.valid ? nx1.ch : cur.ch) * kChunk + (lane >> 1)];
     const int brow = gidx[cur.ch * kChunk + (lane >> 1)];
-    bw = a.xbits[static_cast<int64_t>(brow) * a.xbits_ld + cur.t * 2 + bh];
+    bw = a.xbits[(static_cast<int64_t>(cur.t) * a.xbits_rows + brow) * 2 + bh];
   } else if constexpr (GATHER) {
     const int pch = nx1.valid ? nx1.ch : cur.ch;
     const int pt = nx1.valid ? nx1.t : cur.t;
@@ -689,7 +689,7 @@ __device__ __forceinline__ void mfma_body(const StepArgs& a, const DevLayer& w, 
       dst[1] = o[1];
       dst[2] = o[2];
       dst[3] = o[3];
-      bw = a.xbits[static_cast<int64_t>(brow1) * a.xbits_ld + pt * 2 + bh];
+      bw = a.xbits[(static_cast<int64_t>(pt) * a.xbits_rows + brow1) * 2 + bh];
     } else if constexpr (GATHER) {
       // this step's rows of the lane's sample into the wave's LDS slice, then the next
       // step's gathers (row ids and input positions loaded a step earlier)
@@ -1611,6 +1611,8 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(const _Float16* __restri
   if (i >= wpr * N) return;
   const int64_t nrow = i / wpr, wd = i - nrow * wpr;
   const uint4* src = reinterpret_cast<const uint4*>(store + nrow * ld + 32 * wd);
+  // tile-major: the two words of 64-sample tile t of every row are contiguous, so one tile's
+  // bits (N x 8 bytes) occupy few cache lines
   uint32_t out = 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -1622,7 +1624,7 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(const _Float16* __restri
       if ((h[e] & 0x7FFF0000u) != 0u) out |= 1u << (8 * q + 2 * e + 1);
     }
   }
-  bits[i] = out;
+  bits[((wd >> 1) * N + nrow) * 2 + (wd & 1)] = out;
 }
 
 __global__ __launch_bounds__(256) void pack_rows_kernel(const _Float16* __restrict__ store, int64_t ld, int N,

@@ -71,10 +71,11 @@ struct StepArgs {
   // MFMA FP16 kernel: input position of each output position (compacted layers), or null
   // (then a compacted layer's tiles are input positions and its stores are compacted)
   const int32_t* qsrc = nullptr;
-  // MFMA FP16 kernel, layer 0 with a binary input: its bit-packed neuron-major copy
-  // (xbits_ld 32-bit words per neuron row, bit i of word w = sample 32w + i), or null
+  // MFMA FP16 kernel, layer 0 with a binary input: its bit-packed copy, tile-major (for 64-sample
+  // tile t and row n the two words (t * xbits_rows + n) * 2 + h, bit i of word h = sample
+  // 64 t + 32 h + i), or null
   const uint32_t* xbits = nullptr;
-  int64_t xbits_ld = 0;
+  int64_t xbits_rows = 0;
   uint8_t* alive_next = nullptr;
   uint8_t* unsat_next = nullptr;     // MFMA FP16 kernel: may be null (flag not needed after this layer)
   uint8_t* band_next = nullptr;      // null: guard band off for this layer
@@ -164,7 +165,7 @@ hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* i
                               int64_t max_rows, _Float16* dst, int64_t ldd, int N, hipStream_t st);
 // Binary inputs: bit-packed sample-major copy [S][words] (words = ceil(N/32)) of the
 // neuron-major FP16 store, and the gather of selected samples from it.
-// bits[n][w] (ld/32 words per neuron row): bit i = store[n][32w + i] != 0
+// bits[(t * N + n) * 2 + h] (tile-major, t < ld/64): bit i = store[n][64t + 32h + i] != 0
 hipError_t launch_pack_cols(const _Float16* store, int64_t ld, int N, uint32_t* bits, hipStream_t st);
 hipError_t launch_pack_rows(const _Float16* store, int64_t ld, int N, int64_t S, uint32_t* packed, int words,
                             hipStream_t st);

@@ -102,6 +102,10 @@ typedef struct {
   int weight_mask;       /* FP16 MFMA layers whose weights are all one value: 1 (default) = read the
                             weight tiles as a bit mask (128 bytes a K chunk instead of 2 KB),
                             expanded to the same MFMA operands in registers; 0 = the FP16 tiles */
+  int batch_cu_share;    /* several networks with batch_concurrent = k > 1: 1 = every SpMM grid of a
+                            chain is divided by k (at least 8 work-groups), so the chains share the
+                            CUs instead of queueing whole-GPU launches behind each other; 0 (default)
+                            = full grids */
   int weight_budget_mb;  /* upload: MFMA weight images resident in HBM, all of the engine's networks
                             together, at most this many MiB (0, the default: 40% of the HBM);
                             the layers past it keep their images in pinned host memory and the
