@@ -74,10 +74,14 @@ class Result(ctypes.Structure):
         ("mfma_layers", ctypes.c_int64),
         ("positions", _c_i64p),
         ("streamed_layers", ctypes.c_int64),
+        ("layer_kernel", _c_i32p),
     ]
 
 
 PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT, PLAN_ROWS = 0, 1, 2, 3, 4
+# SpMM kernels reported per layer (RunOutput.layer_kernel)
+(KERNEL_SIMT, KERNEL_SIMT_ROWS, KERNEL_MFMA, KERNEL_MFMA_MASK, KERNEL_MFMA_LDS, KERNEL_MFMA_LDS_MASK, KERNEL_MFMA_BAND,
+ KERNEL_MFMA_BAND_MASK, KERNEL_WAVE_ROWS, KERNEL_MFMA32) = range(10)
 TUNE_FEATURES = 14
 
 
@@ -90,7 +94,7 @@ class LayerPlan(ctypes.Structure):
 
 # every symbol include/cospdnn.h declares
 EXPORTS = [
-    "cospdnn_version", "cospdnn_strerror", "cospdnn_last_error", "cospdnn_default_opts",
+    "cospdnn_version", "cospdnn_strerror", "cospdnn_last_error", "cospdnn_kernel_name", "cospdnn_default_opts",
     "cospdnn_engine_create", "cospdnn_engine_destroy", "cospdnn_engine_info",
     "cospdnn_net_create", "cospdnn_net_destroy", "cospdnn_net_set_layer_csc", "cospdnn_net_set_layer_ref_bcsc",
     "cospdnn_net_generate_layers", "cospdnn_net_set_input_dense", "cospdnn_net_set_input_coo",
@@ -129,6 +133,8 @@ def lib():
         L.cospdnn_strerror.restype = ctypes.c_char_p
         L.cospdnn_strerror.argtypes = [ctypes.c_int]
         L.cospdnn_last_error.restype = ctypes.c_char_p
+        L.cospdnn_kernel_name.restype = ctypes.c_char_p
+        L.cospdnn_kernel_name.argtypes = [ctypes.c_int]
         L.cospdnn_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.cospdnn_default_opts.restype = None
         L.cospdnn_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
@@ -347,6 +353,8 @@ def _run_with(nets, dump_cap_rows, call):
         pos = np.zeros(net.L + 1, dtype=np.int64)
         res[i].positions = _p(pos, _c_i64p)
         res[i].layer_ms = _p(lms, _c_f32p)
+        lk = np.full(max(net.L, 1), -1, dtype=np.int32)
+        res[i].layer_kernel = _p(lk, _c_i32p)
         dv = di = None
         if dump_cap_rows > 0:
             dv = np.zeros((dump_cap_rows, net.N), dtype=np.float32)
@@ -354,17 +362,18 @@ def _run_with(nets, dump_cap_rows, call):
             res[i].dump_vals = _p(dv, _c_f32p)
             res[i].dump_ids = _p(di, _c_i64p)
             res[i].dump_cap_rows = dump_cap_rows
-        keep.append((cats, live, lms, dv, di, pos))
+        keep.append((cats, live, lms, dv, di, pos, lk))
     call(res)
     outs = []
     for i in range(n):
-        cats, live, lms, dv, di, pos = keep[i]
+        cats, live, lms, dv, di, pos, lk = keep[i]
         r = res[i]
         out = RunOutput()
         out.categories = cats[: r.n_categories].copy()
         out.live = live.copy()
         out.positions = pos.copy()
         out.layer_ms = lms[: nets[i].L].copy()
+        out.layer_kernel = lk[: nets[i].L].copy()
         for f in ("ms_total", "ms_spmm", "ms_recompute", "n_flagged", "n_replayed", "n_changed", "edges", "edges_per_sec",
                   "alg_bytes", "mfma_layers", "dump_rows", "streamed_layers"):
             setattr(out, f, getattr(r, f))
@@ -377,6 +386,12 @@ def _run_with(nets, dump_cap_rows, call):
 
 def version():
     return lib().cospdnn_version().decode()
+
+
+def kernel_name(kernel):
+    """Device function of a per-layer SpMM kernel id (RunOutput.layer_kernel), or None."""
+    name = lib().cospdnn_kernel_name(int(kernel))
+    return name.decode() if name is not None else None
 
 
 def generate_layer(seed, kind, N, K, l, wval=0.0625):

@@ -266,6 +266,71 @@ int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
   return static_cast<int>(g);
 }
 
+// Layer l's SpMM in a pass with options cfg: the options' choice, overridden by the network's
+// per-layer plan (FP16).
+struct LayerLaunch {
+  bool mfma = false, gather = false, rows = false;
+  int order = 1, groups = 1;
+};
+
+LayerLaunch layer_launch(const cospdnn_net* net, const RunCfg& cfg, int l) {
+  const bool fp16 = cfg.precision == COSPDNN_FP16;
+  LayerLaunch x;
+  const LayerStore& ls = *net->dl[l];
+  // a streamed image holding mask words serves only passes that read the mask
+  const bool tiles = ls.has_tiles && !(ls.streamed && ls.stream_mask && !cfg.wmask);
+  x.mfma = fp16 && tiles && cfg.use_mfma;
+  x.order = cfg.order;
+  x.groups = cfg.groups_per_cu;
+  bool compact = false;
+  if (fp16 && !net->plan.empty()) {
+    const cospdnn_layer_plan& pl = net->plan[l];
+    if (pl.kernel == COSPDNN_PLAN_SIMT) x.mfma = false;
+    if (pl.kernel == COSPDNN_PLAN_ROWS) {
+      x.mfma = false;
+      x.rows = true;
+      x.order = pl.order;
+      x.groups = pl.groups_per_cu;
+    }
+    if (pl.kernel == COSPDNN_PLAN_MFMA || pl.kernel == COSPDNN_PLAN_MFMA_COMPACT) {
+      x.mfma = tiles;
+      x.order = pl.order;
+      x.groups = pl.groups_per_cu;
+      compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
+    }
+  }
+  // the LDS-resident (order 5) and band (order 6) kernels take tiles of input positions and
+  // store compacted
+  x.gather = x.mfma && cfg.gather && !compact && x.order != 5 && x.order != 6;
+  return x;
+}
+
+// The SpMM kernel (COSPDNN_KERNEL_*) that a pass of S_run rows with options cfg launches for
+// layer l; enqueue_forward dispatches on it, and the result reports it per layer.
+int pick_kernel(const cospdnn_net* net, const RunCfg& cfg, int l, int64_t S_run) {
+  const bool fp16 = cfg.precision == COSPDNN_FP16;
+  const bool fp64 = cfg.precision == COSPDNN_FP64;
+  const bool few = S_run <= kFewRows;
+  const LayerLaunch lx = layer_launch(net, cfg, l);
+  const LayerStore& ls = *net->dl[l];
+  if (lx.mfma) {
+    // the mask image where the pass reads it: resident, or streamed in mask words
+    const bool mask = cfg.wmask && (ls.streamed ? ls.stream_mask : ls.view.wmask != nullptr);
+    const bool band_ok = ls.view.bptr != nullptr && ((cfg.wmask && ls.view.bmask != nullptr) || ls.view.bwt != nullptr);
+    // order 5: the whole input tile in LDS, where the input has at most 1024 rows (the input
+    // store's neurons for layer 0, the previous layer's output slots after it)
+    const int x_rows = l == 0 ? net->N : net->dl[l - 1]->view.n_out;
+    if (lx.order == 6 && band_ok)
+      return cfg.wmask && ls.view.bmask != nullptr ? COSPDNN_KERNEL_MFMA_BAND_MASK : COSPDNN_KERNEL_MFMA_BAND;
+    if (lx.order == 5 && x_rows <= kLdsRowsMax) return mask ? COSPDNN_KERNEL_MFMA_LDS_MASK : COSPDNN_KERNEL_MFMA_LDS;
+    return mask ? COSPDNN_KERNEL_MFMA_MASK : COSPDNN_KERNEL_MFMA;
+  }
+  if (fp16 && lx.rows) return COSPDNN_KERNEL_WAVE_ROWS;
+  if (fp16 || fp64) return few ? COSPDNN_KERNEL_SIMT_ROWS : COSPDNN_KERNEL_SIMT;
+  if (ls.has_tiles32 && cfg.mfma32) return COSPDNN_KERNEL_MFMA32;
+  return few ? COSPDNN_KERNEL_SIMT_ROWS : COSPDNN_KERNEL_SIMT;
+}
+
 // Optional parts of a pass: its row count read from device memory (S_run is then only an
 // upper bound, used for grids and buffer clears), and a callback enqueued right after the
 // scan of fork_layer (the guard pipeline forks from there).
@@ -300,51 +365,15 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
   // sat_from[l]: all-32 is a fixed point of layers l..L-1
   std::vector<char> sat_from(L + 1, 1);
   for (int l = L - 1; l >= 0; --l) sat_from[l] = sat_from[l + 1] && net->dl[l]->sat_stable;
-  const bool few = S_run <= kFewRows;
   auto capg = [&](int g) {
     if (cfg.grid_div > 1) g = std::max(std::min(g, kXcds), g / cfg.grid_div);
     return cfg.grid_cap > 0 ? std::min(g, cfg.grid_cap) : g;
-  };
-  // Layer l's SpMM: the options' choice, overridden by the network's per-layer plan (FP16)
-  struct LayerLaunch {
-    bool mfma = false, gather = false, rows = false;
-    int order = 1, groups = 1;
-  };
-  auto layer_launch = [&](int l) {
-    LayerLaunch x;
-    const LayerStore& ls = *net->dl[l];
-    // a streamed image holding mask words serves only passes that read the mask
-    const bool tiles = ls.has_tiles && !(ls.streamed && ls.stream_mask && !cfg.wmask);
-    x.mfma = fp16 && tiles && cfg.use_mfma;
-    x.order = cfg.order;
-    x.groups = cfg.groups_per_cu;
-    bool compact = false;
-    if (fp16 && !net->plan.empty()) {
-      const cospdnn_layer_plan& pl = net->plan[l];
-      if (pl.kernel == COSPDNN_PLAN_SIMT) x.mfma = false;
-      if (pl.kernel == COSPDNN_PLAN_ROWS) {
-        x.mfma = false;
-        x.rows = true;
-        x.order = pl.order;
-        x.groups = pl.groups_per_cu;
-      }
-      if (pl.kernel == COSPDNN_PLAN_MFMA || pl.kernel == COSPDNN_PLAN_MFMA_COMPACT) {
-        x.mfma = tiles;
-        x.order = pl.order;
-        x.groups = pl.groups_per_cu;
-        compact = pl.kernel == COSPDNN_PLAN_MFMA_COMPACT;
-      }
-    }
-    // the LDS-resident (order 5) and band (order 6) kernels take tiles of input positions and
-    // store compacted
-    x.gather = x.mfma && cfg.gather && !compact && x.order != 5 && x.order != 6;
-    return x;
   };
   // Streamed MFMA images (weight budget exceeded): layer l's image is copied into slot l % 2 on
   // the copy stream, one layer ahead, after the SpMM that last read the slot; the SpMM waits for
   // its copy.  The copy stream forks from st and joins it at the end (a captured graph holds both).
   const bool streaming = fp16 && net->streamed_layers > 0 && net->copy_st != nullptr;
-  auto streamed_mfma = [&](int l) { return l < L && net->dl[l]->streamed && layer_launch(l).mfma; };
+  auto streamed_mfma = [&](int l) { return l < L && net->dl[l]->streamed && layer_launch(net, cfg, l).mfma; };
   auto stream_copy = [&](int l) -> int {
     if (!streamed_mfma(l)) return COSPDNN_OK;
     const LayerStore& ls = *net->dl[l];
@@ -374,7 +403,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.ids_out = rb.id(nxt);
     sa.qmap = rb.qmap.as<int32_t>();
     // the output-to-input position map, for a layer in gather mode
-    const LayerLaunch lx = l < L ? layer_launch(l) : LayerLaunch();
+    const LayerLaunch lx = l < L ? layer_launch(net, cfg, l) : LayerLaunch();
     sa.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
     sa.flag_undecided = (l == D) ? 1 : 0;
     sa.flag_ids = rb.flag_ids.as<int32_t>();
@@ -446,66 +475,71 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.work = rb.work.as<int32_t>();
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l], st));
     const int nblk = view.nblk;
-    const bool mfma_l = lx.mfma;
+    const int kern = pick_kernel(net, cfg, l, S_run);
     const int order_l = lx.order, groups_l = lx.groups;
-    if (mfma_l) {
-      // items are strided over the waves of a grid that is resident at once
-      const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
-      // at most the work-groups that are resident at once (items are strided over the grid)
-      const int64_t resident = std::min(groups_l, e->bpc_mfma_resident);
-      int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, resident * e->cu_count)));
-      grid = capg(grid);
-      if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
+    // the images each kernel reads (the view built above must agree with the choice)
+    const bool mask_k = kern == COSPDNN_KERNEL_MFMA_MASK || kern == COSPDNN_KERNEL_MFMA_LDS_MASK;
+    if (lx.mfma && kern != COSPDNN_KERNEL_MFMA_BAND && kern != COSPDNN_KERNEL_MFMA_BAND_MASK &&
+        mask_k != (view.wmask != nullptr))
+      return fail(COSPDNN_ERR_INTERNAL, "layer " + std::to_string(l) + ": mask image and kernel choice disagree");
+    if (lx.mfma) {
       // the unsaturated flags are read only by the checkpoint scan (layer D) and by retirement
       if (!cfg.retire_sat && l + 1 != D) a.unsat_next = nullptr;
-      // order 5: the whole input tile in LDS, where the input has at most 1024 rows (the
-      // input store's neurons for layer 0, the previous layer's output slots after it);
-      // otherwise order 1
-      const int x_rows = l == 0 ? net->N : net->dl[l - 1]->view.n_out;
-      const bool band_ok = view.bptr != nullptr && (view.bmask != nullptr || view.bwt != nullptr);
-      if (order_l == 6 && band_ok) {
+      a.x_rows = l == 0 ? net->N : net->dl[l - 1]->view.n_out;
+    }
+    const Dtype t_in = l == 0 ? Dtype::F16 : (fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32));
+    const Dtype t_out = fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32);
+    switch (kern) {
+      case COSPDNN_KERNEL_MFMA_BAND:
+      case COSPDNN_KERNEL_MFMA_BAND_MASK: {
         a.order = 1;
-        a.x_rows = x_rows;
-        const int band_grid = std::max(kXcds, capg(e->cu_count));
-        HIP_TRY(launch_spmm_mfma_band(a, view, band_grid, st));
-      } else if (order_l == 5 && x_rows <= kLdsRowsMax) {
+        HIP_TRY(launch_spmm_mfma_band(a, view, std::max(kXcds, capg(e->cu_count)), st));
+        break;
+      }
+      case COSPDNN_KERNEL_MFMA_LDS:
+      case COSPDNN_KERNEL_MFMA_LDS_MASK: {
         a.order = 1;
-        a.x_rows = x_rows;
         const int lds_grid =
             capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((S_run + 63) / 64, e->cu_count))));
         HIP_TRY(launch_spmm_mfma_lds(a, view, lds_grid, st));
-      } else {
+        break;
+      }
+      case COSPDNN_KERNEL_MFMA:
+      case COSPDNN_KERNEL_MFMA_MASK: {
+        // items are strided over the waves of a grid that is resident at once
+        const int64_t items = (S_run + 63) / 64 * static_cast<int64_t>(nblk);
+        const int64_t resident = std::min(groups_l, e->bpc_mfma_resident);
+        int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + 3) / 4, resident * e->cu_count)));
+        grid = capg(grid);
+        if (order_l >= 2) grid = std::max(grid, kXcds);  // every XCD's share needs a work-group
         a.order = order_l >= 5 ? 1 : order_l;
         HIP_TRY(launch_spmm_mfma(a, view, grid, st));
+        break;
       }
-    } else if (fp16 && lx.rows) {
-      // groups per CU, but never below what the work split needs (split 0: one wave per
-      // 64-sample group; split 1: one work-group per XCD), whatever the options' grid cap
-      const int64_t need = order_l == 1 ? kXcds : ((S_run + 63) / 64 + 3) / 4;
-      const int grid = static_cast<int>(std::max<int64_t>(need, capg(groups_l * e->cu_count)));
-      a.order = order_l;
-      HIP_TRY(launch_spmm_wave_rows(a, view, grid, st));
-    } else if (fp16) {
-      if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, view, Dtype::F16, Dtype::F16, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
-      else
-        HIP_TRY(launch_spmm_simt(a, view, Dtype::F16, Dtype::F16, capg(grid_for(e, e->bpc_simt16, S_run, nblk)), st));
-    } else if (fp64) {
-      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F64;
-      if (few)
-        HIP_TRY(launch_spmm_simt_rows(a, view, tin, Dtype::F64, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
-      else
-        HIP_TRY(launch_spmm_simt(a, view, tin, Dtype::F64, capg(grid_for(e, l == 0 ? e->bpc_simt1664 : e->bpc_simt64, S_run, nblk)), st));
-    } else if (few && !(ls.has_tiles32 && cfg.mfma32)) {
-      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_simt_rows(a, view, tin, Dtype::F32, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
-    } else if (ls.has_tiles32 && cfg.mfma32) {
-      const Dtype tin = l == 0 ? Dtype::F16 : Dtype::F32;
-      HIP_TRY(launch_spmm_mfma32(a, view, tin, capg(grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk)), st));
-    } else if (l == 0) {
-      HIP_TRY(launch_spmm_simt(a, view, Dtype::F16, Dtype::F32, capg(grid_for(e, e->bpc_simt1632, S_run, nblk)), st));
-    } else {
-      HIP_TRY(launch_spmm_simt(a, view, Dtype::F32, Dtype::F32, capg(grid_for(e, e->bpc_simt32, S_run, nblk)), st));
+      case COSPDNN_KERNEL_WAVE_ROWS: {
+        // groups per CU, but never below what the work split needs (split 0: one wave per
+        // 64-sample group; split 1: one work-group per XCD), whatever the options' grid cap
+        const int64_t need = order_l == 1 ? kXcds : ((S_run + 63) / 64 + 3) / 4;
+        const int grid = static_cast<int>(std::max<int64_t>(need, capg(groups_l * e->cu_count)));
+        a.order = order_l;
+        HIP_TRY(launch_spmm_wave_rows(a, view, grid, st));
+        break;
+      }
+      case COSPDNN_KERNEL_MFMA32:
+        HIP_TRY(launch_spmm_mfma32(a, view, t_in, capg(grid_for(e, l == 0 ? e->bpc_m32h : e->bpc_m32f, S_run, nblk)), st));
+        break;
+      case COSPDNN_KERNEL_SIMT_ROWS:
+        HIP_TRY(launch_spmm_simt_rows(a, view, t_in, t_out, capg(grid_rows(e, e->bpc_rows, S_run, nblk)), st));
+        break;
+      case COSPDNN_KERNEL_SIMT: {
+        int bpc = e->bpc_simt16;
+        if (fp64) bpc = l == 0 ? e->bpc_simt1664 : e->bpc_simt64;
+        if (!fp16 && !fp64) bpc = l == 0 ? e->bpc_simt1632 : e->bpc_simt32;
+        HIP_TRY(launch_spmm_simt(a, view, t_in, t_out, capg(grid_for(e, bpc, S_run, nblk)), st));
+        break;
+      }
+      default:
+        return fail(COSPDNN_ERR_INTERNAL, "no SpMM kernel for layer " + std::to_string(l));
     }
     if (timing) HIP_TRY(hipEventRecord(net->layer_ev[2 * l + 1], st));
     if (slot_l) HIP_TRY(hipEventRecord(net->slot_free[l & 1], st));
@@ -1096,13 +1130,9 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
   for (int l = 0; l < L; ++l) {
     const double lv = live[l] >= 0 ? static_cast<double>(live[l]) : static_cast<double>(S);
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
-    // the FP32 path takes the FP32 MFMA kernel only where the layer has an FP32 tile image
-    const LayerStore& lsl = *net->dl[l];
-    const bool tiles16 = lsl.has_tiles && !(lsl.streamed && lsl.stream_mask && !cfg.wmask);
-    bool m = (o.precision == COSPDNN_FP16 && cfg.use_mfma && tiles16) ||
-             (o.precision == COSPDNN_FP32 && cfg.mfma32 && lsl.has_tiles32);
-    if (o.precision == COSPDNN_FP16 && !net->plan.empty() && net->plan[l].kernel != COSPDNN_PLAN_AUTO)
-      m = tiles16 && (net->plan[l].kernel == COSPDNN_PLAN_MFMA || net->plan[l].kernel == COSPDNN_PLAN_MFMA_COMPACT);
+    const int k = pick_kernel(net, cfg, l, S);
+    if (res->layer_kernel) res->layer_kernel[l] = k;
+    const bool m = k != COSPDNN_KERNEL_SIMT && k != COSPDNN_KERNEL_SIMT_ROWS && k != COSPDNN_KERNEL_WAVE_ROWS;
     mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;
@@ -1424,11 +1454,28 @@ const char* cospdnn_strerror(int code) {
     case COSPDNN_ERR_STATE: return "invalid call order";
     case COSPDNN_ERR_IO: return "file missing or unparsable";
     case COSPDNN_ERR_MISMATCH: return "networks disagree";
+    case COSPDNN_ERR_INTERNAL: return "engine invariant broken";
     default: return "unknown error";
   }
 }
 
 const char* cospdnn_last_error(void) { return last_error_text(); }
+
+const char* cospdnn_kernel_name(int kernel) {
+  switch (kernel) {
+    case COSPDNN_KERNEL_SIMT: return "spmm_simt_kernel";
+    case COSPDNN_KERNEL_SIMT_ROWS: return "spmm_simt_rows_kernel";
+    case COSPDNN_KERNEL_MFMA: return "spmm_mfma_kernel";
+    case COSPDNN_KERNEL_MFMA_MASK: return "spmm_mfma_mask_kernel";
+    case COSPDNN_KERNEL_MFMA_LDS: return "spmm_mfma_lds_kernel";
+    case COSPDNN_KERNEL_MFMA_LDS_MASK: return "spmm_mfma_lds_mask_kernel";
+    case COSPDNN_KERNEL_MFMA_BAND: return "spmm_mfma_band_kernel";
+    case COSPDNN_KERNEL_MFMA_BAND_MASK: return "spmm_mfma_band_mask_kernel";
+    case COSPDNN_KERNEL_WAVE_ROWS: return "spmm_wave_rows_kernel";
+    case COSPDNN_KERNEL_MFMA32: return "spmm_mfma32_f32in_kernel";  // layer 0: spmm_mfma32_f16in_kernel
+    default: return nullptr;
+  }
+}
 
 void cospdnn_default_opts(cospdnn_opts* o) {
   if (!o) return;

@@ -37,11 +37,27 @@ enum {
   COSPDNN_ERR_PRECISION = 5, /* FP16 path asked for values not exact in FP16 */
   COSPDNN_ERR_STATE = 6,     /* call order: layers or input missing, not uploaded */
   COSPDNN_ERR_IO = 7,        /* file missing or unparsable */
-  COSPDNN_ERR_MISMATCH = 8   /* networks of one run disagree where they must agree */
+  COSPDNN_ERR_MISMATCH = 8,  /* networks of one run disagree where they must agree */
+  COSPDNN_ERR_INTERNAL = 9   /* an engine invariant does not hold (a bug); detail in cospdnn_last_error() */
 };
 
 enum { COSPDNN_FP32 = 0, COSPDNN_FP16 = 1, COSPDNN_FP64 = 2 };
 enum { COSPDNN_GEN_RANDOM = 0, COSPDNN_GEN_BANDED = 1 };
+
+/* The SpMM kernel of a layer, reported per layer in cospdnn_result.layer_kernel (name of the
+   device function: cospdnn_kernel_name). */
+enum {
+  COSPDNN_KERNEL_SIMT = 0,            /* one thread per (sample, output): gathers over the CSC */
+  COSPDNN_KERNEL_SIMT_ROWS = 1,       /* the same for passes of at most 512 rows */
+  COSPDNN_KERNEL_MFMA = 2,            /* FP16 MFMA over the layer's tile image */
+  COSPDNN_KERNEL_MFMA_MASK = 3,       /* the same reading the mask image */
+  COSPDNN_KERNEL_MFMA_LDS = 4,        /* FP16 MFMA with the whole 64-sample input tile in LDS (order 5) */
+  COSPDNN_KERNEL_MFMA_LDS_MASK = 5,
+  COSPDNN_KERNEL_MFMA_BAND = 6,       /* FP16 MFMA streaming 512-row bands of the input tile (order 6) */
+  COSPDNN_KERNEL_MFMA_BAND_MASK = 7,
+  COSPDNN_KERNEL_WAVE_ROWS = 8,       /* row-gather kernel (COSPDNN_PLAN_ROWS) */
+  COSPDNN_KERNEL_MFMA32 = 9           /* FP32 MFMA over the FP32 tile image */
+};
 
 typedef struct cospdnn_engine cospdnn_engine;
 typedef struct cospdnn_net cospdnn_net;
@@ -138,11 +154,14 @@ typedef struct {
                             layer l in the first pass, live rows plus the holes of a kept
                             layout (keep_holes_ppm) */
   int64_t streamed_layers; /* layers of the network whose MFMA images are streamed from host memory */
+  int32_t* layer_kernel;   /* caller-provided (may be NULL), capacity L: the SpMM kernel
+                              (COSPDNN_KERNEL_*) the main pass launches for each layer */
 } cospdnn_result;
 
 const char* cospdnn_version(void);
 const char* cospdnn_strerror(int code);
 const char* cospdnn_last_error(void); /* detail of the last failure on this thread */
+const char* cospdnn_kernel_name(int kernel); /* device function of a COSPDNN_KERNEL_* id, or NULL */
 void cospdnn_default_opts(cospdnn_opts* opts);
 
 /* engine: one device, one stream (default priority) */
