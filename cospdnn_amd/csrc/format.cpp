@@ -190,6 +190,9 @@ void build_mask_image(BlockedLayer& out) {
     if (val == 0) val = h;
     if (h != val) return;
   }
+  // the kernels expand a mask word with (v >> q) for q < 8, exact for values whose FP16 bits
+  // end in 7 zeros (at most 3 significant mantissa bits: 0.0625, 0.046875, -0.75, ...)
+  if ((val & 127u) != 0) return;
   out.wmask.assign(static_cast<size_t>(out.nchunk) * 64, 0);
   for (int64_t ch = 0; ch < out.nchunk; ++ch)
     for (int lane = 0; lane < 64; ++lane) {

@@ -42,7 +42,8 @@ struct BlockedLayer {
   std::vector<int32_t> gidx;   // nchunk*32
   std::vector<uint16_t> wt;    // nchunk*1024 FP16 bit patterns (v_mfma_f32_16x16x32_f16 A order)
   std::vector<float> wt32;     // nchunk*1024 FP32 values (v_mfma_f32_16x16x4f32 A order)
-  // Mask image: when every nonzero of the tile image is one FP16 value (wmask_val), the image
+  // Mask image: when every nonzero of the tile image is one FP16 value (wmask_val) whose low 7
+  // bits are zero, the image
   // is a bit per element: per chunk and lane one 16-bit word, bit 4m + e/2 for the even
   // elements e of fragment m (m = 0, 1), bit 8 + 4m + e/2 for the odd ones (128 bytes a
   // chunk instead of 2 KB); the kernels expand it to the same fragments in registers.

@@ -483,13 +483,15 @@ __device__ __forceinline__ void expand_bits(uint32_t wbits, uint4 (&o)[4]) {
 // The chunk's two W fragments from its mask-image word (the lane's 16 bits, see format.hpp):
 // dword k of fragment m holds elements 2k (low half) and 2k+1 (high half), each the FP16 value
 // v where its bit is set and 0 elsewhere.  The odd elements' bits move up by 8 so that bit
-// 4m + k holds element 2k and bit 16 + 4m + k element 2k + 1: one shift, mask and 24-bit
-// multiply per dword (x * v with x = lo + hi * 2^16 and v < 2^16 cannot carry between halves).
+// q = 4m + k holds element 2k and bit 16 + q element 2k + 1; then dword q is
+// (sp & (0x10001 << q)) * (v >> q): one AND and one 24-bit multiply, exact because a mask
+// image exists only for values v with their low 7 bits zero (v >> q * 2^q = v for q < 8), and
+// x * (v >> q) with x = (lo + hi * 2^16) 2^q cannot carry between the halves (v < 2^16).
 __device__ __forceinline__ void expand_mask(uint32_t bits, uint32_t v, half8_t& w0, half8_t& w1) {
   const uint32_t sp = (bits & 0xFFu) | ((bits & 0xFF00u) << 8);
   uint32_t d[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) d[q] = __umul24((sp >> q) & 0x10001u, v);
+  for (int q = 0; q < 8; ++q) d[q] = __umul24(sp & (0x10001u << q), v >> q);
   w0 = __builtin_bit_cast(half8_t, u32x4_t{d[0], d[1], d[2], d[3]});
   w1 = __builtin_bit_cast(half8_t, u32x4_t{d[4], d[5], d[6], d[7]});
 }
@@ -907,10 +909,15 @@ __device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsC
   half8_t w0 = k.w0, w1 = k.w1;
   if constexpr (MASK) expand_mask(k.wm, wval, w0, w1);
   half8_t xfrag[4];
+  // 32-bit LDS offsets (row ids < 2^16, so one 24-bit multiply each)
+  typedef __attribute__((address_space(3))) const _Float16 lds_half_t;
+  lds_half_t* xl = (lds_half_t*)lds_x;
+  const uint32_t olo = __umul24(static_cast<uint32_t>(k.rlo), kLdsLd) + 16 * (j & 3);
+  const uint32_t ohi = __umul24(static_cast<uint32_t>(k.rhi), kLdsLd) + 16 * (j & 3);
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
-    const _Float16* plo = lds_x + k.rlo * kLdsLd + 16 * (j & 3) + 4 * tt;
-    const _Float16* phi = lds_x + k.rhi * kLdsLd + 16 * (j & 3) + 4 * tt;
+    lds_half_t* plo = xl + olo + 4 * tt;
+    lds_half_t* phi = xl + ohi + 4 * tt;
     const fp16x4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)plo);
     const fp16x4_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4f16((__attribute__((address_space(3))) fp16x4_t*)phi);
     xfrag[tt] = __builtin_shufflevector(__builtin_bit_cast(half4_t, lo), __builtin_bit_cast(half4_t, hi), 0, 1, 2, 3,
@@ -925,10 +932,10 @@ __device__ __forceinline__ void lds_chunk_mfma(float4_t (&acc)[4][2], const LdsC
 
 // Epilogue of the LDS-staged MFMA kernels for block column c of a 64-sample tile:
 // acc[tt][m][r] of lane j + 16g is output 16m + j of sample 16g + 4tt + r.  Bias is in acc.
-// fast: the tile's 64 samples go to 64 consecutive positions from q0, a multiple of 8 (stores
-// through the wave's staging area ostage, [16][kOutLd], as 8 lanes x 16 B per output row);
-// otherwise each value goes to its sample's position (q_own of the owning lane, -1: none).
-// Then the liveness / unsaturated / guard-band flags of the lane's sample.
+// Outputs go through the wave's staging area ostage ([16][kOutLd]).  fast: the tile's 64
+// samples go to 64 consecutive positions from q0, a multiple of 8 (8 lanes x 16 B per output
+// row); otherwise one store per output row, each lane's value to its sample's position (q_own,
+// -1: none).  Then the liveness / unsaturated / guard-band flags of the lane's sample.
 __device__ __forceinline__ void lds_epilogue(const StepArgs& a, const DevLayer& w, const float4_t (&acc)[4][2], int c,
                                              int q_own, int q0, bool fast, _Float16* ostage,
                                              _Float16* __restrict__ Y, int64_t ldy) {
@@ -946,25 +953,26 @@ __device__ __forceinline__ void lds_epilogue(const StepArgs& a, const DevLayer& 
   uint32_t fl = (ba >> j) & 1u;
   if (want_unsat) fl |= ((flag_bits<1, false>(acc, ok0, ok1, 0.0f) >> j) & 1u) << 1;
   if (want_band) fl |= ((flag_bits<2, false>(acc, ok0, ok1, a.band) >> j) & 1u) << 2;
-  if (fast) {
-    // FP16 outputs through the wave's staging area, 16 rows at a time, stored as 8 lanes x
-    // 16 B per row
+  // FP16 outputs through the wave's staging area ([16][kOutLd]: row = output 16m + j of the
+  // block column, column = the tile's sample), 16 rows at a time
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      uint32_t pk[8];
+  for (int m = 0; m < 2; ++m) {
+    uint32_t pk[8];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int tt = e / 2, r = 2 * (e % 2);
-        const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r], 0.0f, 32.0f));
-        const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r + 1], 0.0f, 32.0f));
-        pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
-                (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
-      }
-      uint4* o = reinterpret_cast<uint4*>(ostage + j * kOutLd + 16 * g);
-      o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-      o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int e = 0; e < 8; ++e) {
+      const int tt = e / 2, r = 2 * (e % 2);
+      const _Float16 h0 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r], 0.0f, 32.0f));
+      const _Float16 h1 = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][m][r + 1], 0.0f, 32.0f));
+      pk[e] = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h0)) |
+              (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h1)) << 16);
+    }
+    uint4* o = reinterpret_cast<uint4*>(ostage + j * kOutLd + 16 * g);
+    o[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+    o[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (fast) {
+      // 64 consecutive positions from q0: 8 lanes x 16 B per output row
       _Float16* ybase = Y + q0 + c8;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
@@ -972,23 +980,17 @@ __device__ __forceinline__ void lds_epilogue(const StepArgs& a, const DevLayer& 
         const uint4 v = *reinterpret_cast<const uint4*>(ostage + (r8 + 8 * k) * kOutLd + c8);
         if (full || slot < w.n_out) store_stream(ybase + static_cast<int64_t>(slot) * ldy, v);
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    }
-  } else {
-    // each value to its sample's position: sample 16g + 4tt + r is owned by that lane
+    } else if (q_own >= 0) {
+      // compacted positions: one output row per store, lane L its sample's value at q_own
+      // (positions increase with the lane, so a store covers one or two 128-byte lines)
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int qs = __shfl(q_own, 16 * g + 4 * tt + r, 64);
-        if (qs >= 0) {
-          if (cov0 >= 0)
-            Y[static_cast<int64_t>(cov0) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][0][r], 0.0f, 32.0f));
-          if (cov1 >= 0)
-            Y[static_cast<int64_t>(cov1) * ldy + qs] = static_cast<_Float16>(__builtin_amdgcn_fmed3f(acc[tt][1][r], 0.0f, 32.0f));
-        }
+      for (int rr = 0; rr < 16; ++rr) {
+        const int slot = c * 32 + 16 * m + rr;
+        if (full || slot < w.n_out) Y[static_cast<int64_t>(slot) * ldy + q_own] = ostage[rr * kOutLd + lane];
       }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   if (q_own >= 0) {
     if (fl & 1u) a.alive_next[q_own] = 1;
@@ -1099,7 +1101,7 @@ __device__ __forceinline__ void mfma_lds_body(const StepArgs& a, const DevLayer&
 // Band-streaming MFMA SpMM (plan order 6) for random layers whose 64-sample input tile does
 // not fit LDS.  Gathering a block column's rows from L2 re-reads every input row once per
 // block column that uses it (~28 times at N = 4096); here a work item is (64-sample tile,
-// group of 16 block columns = 512 outputs): the item's 8 waves keep the accumulators of two
+// group of 32 block columns = 1024 outputs): the item's 8 waves keep the accumulators of four
 // block columns each in registers and stream the tile's input rows through LDS in bands of
 // kBandRows rows (two LDS buffers), so each input row moves from L2 once per group.  The band
 // image splits every block column's K chunks at band boundaries (band-relative row ids), so
@@ -1113,7 +1115,7 @@ __device__ __forceinline__ void mfma_lds_body(const StepArgs& a, const DevLayer&
 // L2.  Epilogue and compacted stores as in the LDS-resident kernel.
 // --------------------------------------------------------------------------
 constexpr int kBandWaves = 8;  // 2 per SIMD: one work-group per CU
-constexpr int kBandBcPerWave = 2;
+constexpr int kBandBcPerWave = 4;
 constexpr int kBandThreads = kBandWaves * 64;
 constexpr int kBandGroup = kBandWaves * kBandBcPerWave;            // block columns per item
 constexpr int kBandPieces = kBandRows * 8 / kBandThreads;          // 16-byte pieces per thread and band
@@ -1148,7 +1150,7 @@ __device__ __forceinline__ void band_store(const uint4 (&v)[kBandPieces], _Float
 
 template <bool MASK>
 struct BandCfg {
-  static constexpr int kAhead = MASK ? 6 : 3;  // K chunks in flight per block column
+  static constexpr int kAhead = MASK ? 3 : 1;  // K chunks in flight per block column
 };
 
 // Loads of the first kAhead chunks of [lo, hi) (clamped; unconditional, so the number of loads
@@ -1215,7 +1217,7 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
   if (steps == 0) return;  // uniform per work-group
   auto tile_of = [&](int s) { return xcd + kXcds * ((wl + (s / nb) * wgs) / ngroups); };
   auto group_of = [&](int s) { return (wl + (s / nb) * wgs) % ngroups; };
-  // chunk range of the wave's block column q (0, 1) of group G in band b
+  // chunk range of the wave's block column q (0 .. kBandBcPerWave - 1) of group G in band b
   auto range = [&](int G, int q, int b, int& lo, int& hi) {
     const int c = G * kBandGroup + wave + kBandWaves * q;
     lo = c < nblk ? bptr[c * (nb + 1) + b] : 0;
@@ -1223,17 +1225,18 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
   };
 
   uint4 v[kBandPieces];
-  LdsChunk kc0[A], kc1[A];
-  int lo0, hi0, lo1, hi1;
+  LdsChunk kc[kBandBcPerWave][A];
+  int lo[kBandBcPerWave], hi[kBandBcPerWave];
   // step 0: its band into buffer 0, its first chunks in flight
   band_load(v, X, ldx, tile_of(0), 0, x_rows, tid);
   band_store(v, buf0, tid);
-  range(group_of(0), 0, 0, lo0, hi0);
-  range(group_of(0), 1, 0, lo1, hi1);
-  band_prefetch<MASK>(kc0, wb, lo0, hi0, lane, j, g);
-  band_prefetch<MASK>(kc1, wb, lo1, hi1, lane, j, g);
+#pragma unroll
+  for (int q = 0; q < kBandBcPerWave; ++q) {
+    range(group_of(0), q, 0, lo[q], hi[q]);
+    band_prefetch<MASK>(kc[q], wb, lo[q], hi[q], lane, j, g);
+  }
   __syncthreads();
-  float4_t acc0[4][2], acc1[4][2];
+  float4_t acc[kBandBcPerWave][4][2];
   for (int s = 0; s < steps; ++s) {
     const int b = s % nb;
     const int t = tile_of(s), G = group_of(s);
@@ -1241,29 +1244,34 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
     if (has_next) band_load(v, X, ldx, tile_of(s + 1), (s + 1) % nb, x_rows, tid);
     if (b == 0) {
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int q = 0; q < kBandBcPerWave; ++q)
 #pragma unroll
-        for (int m = 0; m < 2; ++m) acc0[tt][m] = acc1[tt][m] = bias4;
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+          for (int m = 0; m < 2; ++m) acc[q][tt][m] = bias4;
     }
     const _Float16* buf = (s & 1) ? buf1 : buf0;
-    band_chunks<MASK>(acc0, kc0, wb, lo0, hi0, buf, lane, j, g, wval);
-    band_chunks<MASK>(acc1, kc1, wb, lo1, hi1, buf, lane, j, g, wval);
+#pragma unroll
+    for (int q = 0; q < kBandBcPerWave; ++q) band_chunks<MASK>(acc[q], kc[q], wb, lo[q], hi[q], buf, lane, j, g, wval);
     if (b == nb - 1) {
       // this lane's sample (lane L owns sample 64t + L) and its output position
       const int p_own = t * kWaveSamples + lane;
       const int q_own = p_own < n ? (ident ? p_own : a.qmap[p_own]) : -1;
       const int q0 = __builtin_amdgcn_readfirstlane(q_own);
       const bool fast = __builtin_amdgcn_ballot_w64(!(q0 >= 0 && (q0 & 7) == 0 && q_own == q0 + lane)) == 0;
-      const int c0 = G * kBandGroup + wave, c1 = c0 + kBandWaves;
-      if (c0 < nblk) lds_epilogue(a, w, acc0, c0, q_own, q0, fast, ostage, Y, ldy);
-      if (c1 < nblk) lds_epilogue(a, w, acc1, c1, q_own, q0, fast, ostage, Y, ldy);
+#pragma unroll
+      for (int q = 0; q < kBandBcPerWave; ++q) {
+        const int c = G * kBandGroup + wave + kBandWaves * q;
+        if (c < nblk) lds_epilogue(a, w, acc[q], c, q_own, q0, fast, ostage, Y, ldy);
+      }
     }
     if (has_next) {
       const int bn = (s + 1) % nb, Gn = group_of(s + 1);
-      range(Gn, 0, bn, lo0, hi0);
-      range(Gn, 1, bn, lo1, hi1);
-      band_prefetch<MASK>(kc0, wb, lo0, hi0, lane, j, g);
-      band_prefetch<MASK>(kc1, wb, lo1, hi1, lane, j, g);
+#pragma unroll
+      for (int q = 0; q < kBandBcPerWave; ++q) {
+        range(Gn, q, bn, lo[q], hi[q]);
+        band_prefetch<MASK>(kc[q], wb, lo[q], hi[q], lane, j, g);
+      }
       band_store(v, (s & 1) ? buf0 : buf1, tid);
     }
     __syncthreads();
