@@ -1105,12 +1105,11 @@ __device__ __forceinline__ void mfma_lds_body(const StepArgs& a, const DevLayer&
 // block columns each in registers and stream the tile's input rows through LDS in bands of
 // kBandRows rows (two LDS buffers), so each input row moves from L2 once per group.  The band
 // image splits every block column's K chunks at band boundaries (band-relative row ids), so
-// a chunk reads one band.  Per step (item, band): the next band's rows are loaded into
-// registers, the current band's chunks run from LDS (X fragments by ds_read_b64_tr_b16 at the
-// rows' LDS addresses, same MFMA chain as the other kernels), the loaded rows are written to
-// the other buffer, one barrier.  The W fragments and row ids of a step's first chunks are
-// loaded at the end of the step before (ahead of the next band's rows, so waiting for them
-// never waits for the band).  Tiles are partitioned over the XCDs (blockIdx % 8), and the
+// a chunk reads one band.  Per step (item, band): the current band's chunks run from LDS (X
+// fragments by ds_read_b64_tr_b16 at the rows' LDS addresses, same MFMA chain as the other
+// kernels), the next band's rows (loaded into registers a step earlier) are written to the
+// other buffer, one barrier.  The W fragments and row ids of a step's first chunks are loaded
+// at the end of the step before.  Tiles are partitioned over the XCDs (blockIdx % 8), and the
 // groups of a tile follow each other, so a tile's rows are read from HBM once into its XCD's
 // L2.  Epilogue and compacted stores as in the LDS-resident kernel.
 // --------------------------------------------------------------------------
@@ -1224,10 +1223,14 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
     hi = c < nblk ? bptr[c * (nb + 1) + b + 1] : 0;
   };
 
+  // Three-stage pipeline over the steps: band s + 1 travels in registers during step s (loaded
+  // at the end of step s - 1) and goes to LDS at the end of step s.  Vector loads complete in
+  // issue order, so a wait for a chunk load issued inside step s also waits for the band loads
+  // issued before it: those were issued a whole step earlier and have landed.
   uint4 v[kBandPieces];
   LdsChunk kc[kBandBcPerWave][A];
   int lo[kBandBcPerWave], hi[kBandBcPerWave];
-  // step 0: its band into buffer 0, its first chunks in flight
+  // step 0: its band into buffer 0, its first chunks in flight, then step 1's band
   band_load(v, X, ldx, tile_of(0), 0, x_rows, tid);
   band_store(v, buf0, tid);
 #pragma unroll
@@ -1235,13 +1238,12 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
     range(group_of(0), q, 0, lo[q], hi[q]);
     band_prefetch<MASK>(kc[q], wb, lo[q], hi[q], lane, j, g);
   }
+  if (steps > 1) band_load(v, X, ldx, tile_of(1), 1 % nb, x_rows, tid);
   __syncthreads();
   float4_t acc[kBandBcPerWave][4][2];
   for (int s = 0; s < steps; ++s) {
     const int b = s % nb;
     const int t = tile_of(s), G = group_of(s);
-    const bool has_next = s + 1 < steps;
-    if (has_next) band_load(v, X, ldx, tile_of(s + 1), (s + 1) % nb, x_rows, tid);
     if (b == 0) {
 #pragma unroll
       for (int q = 0; q < kBandBcPerWave; ++q)
@@ -1265,7 +1267,9 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
         if (c < nblk) lds_epilogue(a, w, acc[q], c, q_own, q0, fast, ostage, Y, ldy);
       }
     }
-    if (has_next) {
+    if (s + 1 < steps) {
+      // step s + 1: its first chunks in flight, its band (in v) to the buffer step s - 1 read,
+      // then step s + 2's band into v
       const int bn = (s + 1) % nb, Gn = group_of(s + 1);
 #pragma unroll
       for (int q = 0; q < kBandBcPerWave; ++q) {
@@ -1273,6 +1277,7 @@ __device__ __forceinline__ void mfma_band_body(const StepArgs& a, const DevLayer
         band_prefetch<MASK>(kc[q], wb, lo[q], hi[q], lane, j, g);
       }
       band_store(v, (s & 1) ? buf0 : buf1, tid);
+      if (s + 2 < steps) band_load(v, X, ldx, tile_of(s + 2), (s + 2) % nb, x_rows, tid);
     }
     __syncthreads();
   }
