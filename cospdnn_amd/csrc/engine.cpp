@@ -249,7 +249,7 @@ struct RunCfg {
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
 constexpr double kTileBudget = 0.4;  // MFMA tile images: at most this share of the HBM per engine
 // Band images (order 6) for layers whose tile image wastes more than this (random rows) and
-// whose input has at most kBandMaxN rows: past that a group of 512 outputs uses too few of a
+// whose input has at most kBandMaxN rows: past that a group of 1024 outputs uses too few of a
 // tile's rows for streaming all of them through LDS to beat gathering the used ones.
 constexpr double kBandMinWaste = 4.0;
 constexpr int kBandMaxN = 16384;

@@ -90,7 +90,7 @@ typedef struct {
                             (each XCD's share of the tile image stays in its L2), 5 a whole
                             64-sample input tile in LDS per work-group (layers with at most
                             1024 input rows; order 1 otherwise), 6 the input tile streamed
-                            through LDS in bands of 512 rows while a work-group accumulates 16
+                            through LDS in bands of 512 rows while a work-group accumulates 32
                             block columns (random layers of at most 16384 neurons, which have
                             a band image; order 1 otherwise) */
   int mfma_groups_per_cu;/* MFMA SpMM grid: resident work-groups per CU (1..16) */

@@ -203,6 +203,17 @@ def test_item_orders_and_grids_agree(engine, kind):
                 out = cd.run([net], o, dump_cap_rows=S)[0]
                 assert out.mfma_layers == L
                 assert np.array_equal(out.categories, np.flatnonzero(alive)), (gather, bits, wmask, order, groups)
+                # the kernel each layer reports: the order's kernel (band and LDS-resident kernels
+                # on the random N = 1024 net, which has band images and fits LDS; the banded
+                # N = 2048 net falls back to the gather kernel), its mask variant when the
+                # weights are read as a mask
+                if order == 6 and kind == cd.GEN_RANDOM:
+                    want_k = {cd.KERNEL_MFMA_BAND_MASK} if wmask else {cd.KERNEL_MFMA_BAND, cd.KERNEL_MFMA}
+                elif order == 5 and kind == cd.GEN_RANDOM:
+                    want_k = {cd.KERNEL_MFMA_LDS_MASK if wmask else cd.KERNEL_MFMA_LDS}
+                else:
+                    want_k = {cd.KERNEL_MFMA_MASK if wmask else cd.KERNEL_MFMA}
+                assert set(out.layer_kernel.tolist()) <= want_k, (order, wmask, out.layer_kernel)
                 got = np.zeros((S, N), np.float32)
                 got[out.dump_ids] = out.dump_vals
                 if ref is None:
