@@ -100,6 +100,7 @@ EXPORTS = [
     "cospdnn_net_generate_layers", "cospdnn_net_set_input_dense", "cospdnn_net_set_input_coo",
     "cospdnn_net_generate_input", "cospdnn_net_upload", "cospdnn_run", "cospdnn_run_noalloc",
     "cospdnn_generate_layer", "cospdnn_generate_input_rows", "cospdnn_layer_stats", "cospdnn_layer_tiles_decode",
+    "cospdnn_layer_image_decode",
     "cospdnn_ref_bcsc_to_csc", "cospdnn_similarity_order",
     "cospdnn_net_set_plan", "cospdnn_num_candidates", "cospdnn_candidate", "cospdnn_net_layer_features",
     "cospdnn_tune_exhaustive", "cospdnn_tune_keep", "cospdnn_mlp_create", "cospdnn_mlp_destroy", "cospdnn_mlp_num_params",
@@ -167,6 +168,8 @@ def lib():
                                           ctypes.POINTER(ctypes.c_int)]
         L.cospdnn_layer_tiles_decode.argtypes = [ctypes.c_int, _c_i32p, _c_i32p, _c_f32p, ctypes.c_int64, _c_i32p,
                                                  _c_i32p, _c_f32p, _c_i64p]
+        L.cospdnn_layer_image_decode.argtypes = [ctypes.c_int, _c_i32p, _c_i32p, _c_f32p, ctypes.c_int, ctypes.c_int64,
+                                                 _c_i32p, _c_i32p, _c_f32p, _c_i64p]
         L.cospdnn_ref_bcsc_to_csc.argtypes = [ctypes.c_int, _c_f32p, ctypes.c_int64, _c_i32p, ctypes.c_int64,
                                               _c_i32p, ctypes.c_int64, _c_i32p, ctypes.c_int64, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_int64, _c_i32p, _c_i32p, _c_f32p]
@@ -434,6 +437,26 @@ def layer_tiles_decode(N, colptr, rows, vals):
     _check(lib().cospdnn_layer_tiles_decode(N, _p(cp, _c_i32p), _p(rw, _c_i32p), _p(vl, _c_f32p), n.value,
                                             _p(ins, _c_i32p), _p(outs, _c_i32p), _p(v, _c_f32p), ctypes.byref(n)),
            "cospdnn_layer_tiles_decode")
+    return ins, outs, v
+
+
+IMAGE_TILES, IMAGE_MASK, IMAGE_BAND, IMAGE_BAND_MASK = 0, 1, 2, 3
+
+
+def layer_image_decode(N, colptr, rows, vals, image):
+    """(input, output, value) triples of one of a layer's weight images (IMAGE_*), decoded as
+    the kernels read them; raises CospdnnError when the layer has no such image."""
+    cp = np.ascontiguousarray(colptr, dtype=np.int32)
+    rw = np.ascontiguousarray(rows, dtype=np.int32)
+    vl = np.ascontiguousarray(vals, dtype=np.float32)
+    n = ctypes.c_int64()
+    args = (N, _p(cp, _c_i32p), _p(rw, _c_i32p), _p(vl, _c_f32p), int(image))
+    _check(lib().cospdnn_layer_image_decode(*args, 0, None, None, None, ctypes.byref(n)), "cospdnn_layer_image_decode")
+    ins = np.zeros(n.value, dtype=np.int32)
+    outs = np.zeros(n.value, dtype=np.int32)
+    v = np.zeros(n.value, dtype=np.float32)
+    _check(lib().cospdnn_layer_image_decode(*args, n.value, _p(ins, _c_i32p), _p(outs, _c_i32p), _p(v, _c_f32p),
+                                            ctypes.byref(n)), "cospdnn_layer_image_decode")
     return ins, outs, v
 
 

@@ -1932,6 +1932,68 @@ int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows
   return COSPDNN_OK;
 }
 
+int cospdnn_layer_image_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int image,
+                               int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out) {
+  CscLayer c;
+  if (!csc_from_ptrs(N, colptr, rows, vals, c) || !n_out || image < 0 || image > 3)
+    return fail(COSPDNN_ERR_ARG, "bad argument");
+  BlockedLayer bl;
+  if (!build_layer(c, N, 1e30, bl)) return fail(COSPDNN_ERR_FORMAT, "malformed CSC");
+  const bool band = image >= 2, mask = image == 1 || image == 3;
+  if (band) build_band_image(bl, N);
+  if ((band && !bl.has_band) || (mask && !bl.has_mask))
+    return fail(COSPDNN_ERR_STATE, "the layer has no such image");
+  const std::vector<int32_t>& gidx = band ? bl.bgidx : bl.gidx;
+  const std::vector<uint16_t>& wt = band ? bl.bwt : bl.wt;
+  const std::vector<uint16_t>& wm = band ? bl.bmask : bl.wmask;
+  int64_t k = 0;
+  // one chunk: its fragments as the kernels read them (mask words through the kernels'
+  // expansion: dword q = (sp & (0x10001 << q)) * (v >> q), sp = the word with its high byte
+  // moved up by 8), row ids offset by the chunk's band
+  auto chunk = [&](int blk, int64_t ch, int row0) {
+    for (int m = 0; m < 2; ++m)
+      for (int lane = 0; lane < 64; ++lane) {
+        uint16_t frag[8];
+        if (mask) {
+          const uint32_t bits = wm[static_cast<size_t>(ch) * 64 + lane];
+          const uint32_t sp = (bits & 0xFFu) | ((bits & 0xFF00u) << 8);
+          for (int kq = 0; kq < 4; ++kq) {
+            const int q = 4 * m + kq;
+            const uint64_t x = static_cast<uint64_t>(sp & (0x10001u << q)) * (bl.wmask_val >> q);
+            const uint32_t d = static_cast<uint32_t>(x);  // the 24-bit multiply's low 32 bits
+            frag[2 * kq] = static_cast<uint16_t>(d & 0xFFFFu);
+            frag[2 * kq + 1] = static_cast<uint16_t>(d >> 16);
+          }
+        } else {
+          for (int e = 0; e < 8; ++e) frag[e] = wt[((static_cast<size_t>(ch) * 2 + m) * 64 + lane) * 8 + e];
+        }
+        for (int e = 0; e < 8; ++e) {
+          if ((frag[e] & 0x7FFFu) == 0) continue;
+          const int o = m * 16 + (lane & 15);
+          const int kk = 8 * (lane >> 4) + e;
+          if (k < cap && in_ids && out_ids && values) {
+            in_ids[k] = row0 + gidx[static_cast<size_t>(ch) * kChunkK + kk];
+            out_ids[k] = bl.out_idx[static_cast<size_t>(blk) * kBlockOut + o];
+            values[k] = half_bits_to_float(frag[e]);
+          }
+          ++k;
+        }
+      }
+  };
+  for (int blk = 0; blk < bl.nblk; ++blk) {
+    if (!band) {
+      for (int ch = bl.kptr[blk]; ch < bl.kptr[blk + 1]; ++ch) chunk(blk, ch, 0);
+      continue;
+    }
+    for (int b = 0; b < bl.bands; ++b)
+      for (int ch = bl.bptr[static_cast<size_t>(blk) * (bl.bands + 1) + b];
+           ch < bl.bptr[static_cast<size_t>(blk) * (bl.bands + 1) + b + 1]; ++ch)
+        chunk(blk, ch, b * kBandRows);
+  }
+  *n_out = k;
+  return COSPDNN_OK;
+}
+
 int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const int32_t* row_idx, int64_t n_row_idx,
                             const int32_t* ptr, int64_t n_ptr, const int32_t* fuse, int64_t n_fuse, int blocksize_c,
                             int blocksize_r, int64_t cap, int32_t* colptr, int32_t* rows, float* vals) {

@@ -221,6 +221,12 @@ int cospdnn_layer_stats(int N, const int32_t* colptr, const int32_t* rows, const
    returns the triple count in *n_out (capacity cap); used to pin the tile layout */
 int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals,
                                int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out);
+/* the same for the layer's other weight images: image 0 the FP16 tile image, 1 its mask image
+   (expanded with the kernels' arithmetic), 2 the band image, 3 the band image's mask words;
+   COSPDNN_ERR_STATE when the layer has no such image (mask: not one FP16 value with its low
+   7 bits zero; band: built at upload for every layer, here on request) */
+int cospdnn_layer_image_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int image,
+                               int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out);
 /* Similarity order of one layer (host): the output order that puts outputs reading the same
    inputs into the same group of `group` consecutive outputs; order[i] = the output at position i.
    overlap = 1: greedy by shared inputs (each group seeded with the lowest-numbered free output,

@@ -53,6 +53,40 @@ def test_tile_image_decodes_to_the_csc(case):
     assert decoded(N, cp, rw, vl) == triples(cp, rw, vl)
 
 
+def decoded_image(N, cp, rw, vl, image):
+    ins, outs, v = cd.layer_image_decode(N, cp, rw, vl, image)
+    d = {}
+    for i, o, x in zip(ins, outs, v):
+        assert (int(i), int(o)) not in d, "an entry appears twice in the image"
+        d[(int(i), int(o))] = float(x)
+    return d
+
+
+@pytest.mark.parametrize("case", ["random1024", "random4096", "random2000", "banded4096"])
+def test_mask_and_band_images_decode_to_the_csc(case):
+    """The mask image (through the kernels' two-instruction expansion), the band image (rows
+    split at 512-row bands, band-relative ids) and the band image's mask words all decode to
+    the layer's nonzeros, each exactly once."""
+    N = {"random1024": 1024, "random4096": 4096, "random2000": 2000, "banded4096": 4096}[case]
+    kind = cd.GEN_BANDED if case.startswith("banded") else cd.GEN_RANDOM
+    cp, rw, vl = cd.generate_layer(4, kind, N, 32, 0)
+    want = triples(cp, rw, vl)
+    for image in (cd.IMAGE_TILES, cd.IMAGE_MASK, cd.IMAGE_BAND, cd.IMAGE_BAND_MASK):
+        assert decoded_image(N, cp, rw, vl, image) == want, image
+
+
+@pytest.mark.parametrize("val", [0.0625, 0.046875, -0.75, 1.0, 2.0 ** -14])
+def test_mask_expansion_is_exact_for_seven_trailing_zero_bits(val):
+    """Values whose FP16 bits end in 7 zeros get a mask image, and its expansion
+    ((sp & (0x10001 << q)) * (v >> q)) reproduces them exactly; 0.3, whose bits do not, gets
+    none (the FP16 tiles serve it)."""
+    cp, rw, _ = cd.generate_layer(5, cd.GEN_RANDOM, 256, 16, 0)
+    vl = np.full(len(rw), val, np.float32)
+    assert decoded_image(256, cp, rw, vl, cd.IMAGE_MASK) == triples(cp, rw, vl)
+    with pytest.raises(cd.CospdnnError):
+        cd.layer_image_decode(256, cp, rw, np.full(len(rw), 0.3, np.float32), cd.IMAGE_MASK)
+
+
 def test_tile_waste():
     cp, rw, vl = cd.generate_layer(3, cd.GEN_BANDED, 16384, 32, 0)
     st = cd.layer_stats(16384, cp, rw, vl)
