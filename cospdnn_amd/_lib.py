@@ -49,6 +49,7 @@ class Opts(ctypes.Structure):
         ("weight_mask", ctypes.c_int),
         ("batch_cu_share", ctypes.c_int),
         ("weight_budget_mb", ctypes.c_int),
+        ("spmm_slab", ctypes.c_int),
     ]
 
 
@@ -78,11 +79,11 @@ class Result(ctypes.Structure):
     ]
 
 
-PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT, PLAN_ROWS = 0, 1, 2, 3, 4
+PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT, PLAN_ROWS, PLAN_SLAB = 0, 1, 2, 3, 4, 5
 # SpMM kernels reported per layer (RunOutput.layer_kernel)
 (KERNEL_SIMT, KERNEL_SIMT_ROWS, KERNEL_MFMA, KERNEL_MFMA_MASK, KERNEL_MFMA_LDS, KERNEL_MFMA_LDS_MASK, KERNEL_MFMA_BAND,
- KERNEL_MFMA_BAND_MASK, KERNEL_WAVE_ROWS, KERNEL_MFMA32) = range(10)
-TUNE_FEATURES = 14
+ KERNEL_MFMA_BAND_MASK, KERNEL_WAVE_ROWS, KERNEL_MFMA32, KERNEL_SLAB) = range(11)
+TUNE_FEATURES = 15
 
 
 class LayerPlan(ctypes.Structure):

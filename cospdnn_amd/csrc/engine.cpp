@@ -84,7 +84,8 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
     CALL_TRY(dalloc(rb.ret_count, sizeof(int32_t)));
     HIP_TRY(hipMemsetAsync(rb.flags.p, 0, rb.flags.bytes, st));
   }
-  const size_t elems = static_cast<size_t>(N) * rb.ld;
+  // slab_rows(N) rows: the sample-blocked layout rounds each plane's rows up to a multiple of 8
+  const size_t elems = static_cast<size_t>(slab_rows(N)) * rb.ld;
   if (precision == COSPDNN_FP16) {
     for (int s = 0; s < 2; ++s)
       if (!rb.act16[s].p) {
@@ -213,6 +214,17 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
     ls->view.one_chunk = bl.nchunk == bl.nblk ? 1 : 0;
     ls->view.full_blocks = std::all_of(bl.out_idx.begin(), bl.out_idx.end(), [](int32_t o) { return o >= 0; }) ? 1 : 0;
   }
+  if (bl.has_slab) {
+    CALL_TRY(dupload(ls->sgoff, bl.sgoff));
+    CALL_TRY(dupload(ls->sid, bl.sid));
+    ls->view.sgoff = ls->sgoff.as<int32_t>();
+    ls->view.sid = ls->sid.as<uint16_t>();
+    if (!bl.slab_uniform) {
+      CALL_TRY(dupload(ls->sval, bl.sval));
+      ls->view.sval = ls->sval.as<_Float16>();
+    }
+    ls->view.svalue = bl.slab_value;
+  }
   net->nnz_total += bl.nnz;
   net->weights_fp16_exact = net->weights_fp16_exact && bl.fp16_exact;
   net->dl[l] = std::move(ls);
@@ -244,6 +256,7 @@ struct RunCfg {
   int gather = 0;            // FP16 pass: MFMA layers after a compacting scan gather by output position
   int bits = 0;              // FP16 pass: an MFMA layer 0 reads the bit-packed copy of a binary input
   int wmask = 0;             // FP16 pass: MFMA layers read the mask image where the layer has one
+  int slab = 0;              // FP16 pass: layers with a slab image run the sample-slab kernel
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -269,9 +282,13 @@ int grid_for(const cospdnn_engine* e, int bpc, int64_t S_run, int nblk) {
 // Layer l's SpMM in a pass with options cfg: the options' choice, overridden by the network's
 // per-layer plan (FP16).
 struct LayerLaunch {
-  bool mfma = false, gather = false, rows = false;
+  bool mfma = false, gather = false, rows = false, slab = false;
   int order = 1, groups = 1;
 };
+
+// Rows of the buffer layer l reads: the input store's neurons for layer 0, the previous
+// layer's output slots after it.
+int input_rows(const cospdnn_net* net, int l) { return l == 0 ? net->N : net->dl[l - 1]->view.n_out; }
 
 LayerLaunch layer_launch(const cospdnn_net* net, const RunCfg& cfg, int l) {
   const bool fp16 = cfg.precision == COSPDNN_FP16;
@@ -279,12 +296,21 @@ LayerLaunch layer_launch(const cospdnn_net* net, const RunCfg& cfg, int l) {
   const LayerStore& ls = *net->dl[l];
   // a streamed image holding mask words serves only passes that read the mask
   const bool tiles = ls.has_tiles && !(ls.streamed && ls.stream_mask && !cfg.wmask);
+  // the sample-slab kernel needs the slab image and a slab that fits LDS
+  const bool slab_ok = fp16 && ls.view.sid != nullptr && slab_samples(input_rows(net, l)) > 0;
   x.mfma = fp16 && tiles && cfg.use_mfma;
+  x.slab = slab_ok && cfg.slab;
+  if (x.slab) x.mfma = false;
   x.order = cfg.order;
   x.groups = cfg.groups_per_cu;
   bool compact = false;
   if (fp16 && !net->plan.empty()) {
     const cospdnn_layer_plan& pl = net->plan[l];
+    if (pl.kernel != COSPDNN_PLAN_AUTO) x.slab = false;
+    if (pl.kernel == COSPDNN_PLAN_SLAB) {
+      x.mfma = false;
+      x.slab = slab_ok;
+    }
     if (pl.kernel == COSPDNN_PLAN_SIMT) x.mfma = false;
     if (pl.kernel == COSPDNN_PLAN_ROWS) {
       x.mfma = false;
@@ -313,6 +339,7 @@ int pick_kernel(const cospdnn_net* net, const RunCfg& cfg, int l, int64_t S_run)
   const bool few = S_run <= kFewRows;
   const LayerLaunch lx = layer_launch(net, cfg, l);
   const LayerStore& ls = *net->dl[l];
+  if (lx.slab) return COSPDNN_KERNEL_SLAB;
   if (lx.mfma) {
     // the mask image where the pass reads it: resident, or streamed in mask words
     const bool mask = cfg.wmask && (ls.streamed ? ls.stream_mask : ls.view.wmask != nullptr);
@@ -404,7 +431,7 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     sa.qmap = rb.qmap.as<int32_t>();
     // the output-to-input position map, for a layer in gather mode
     const LayerLaunch lx = l < L ? layer_launch(net, cfg, l) : LayerLaunch();
-    sa.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
+    sa.qsrc = lx.gather || lx.slab ? rb.qsrc.as<int32_t>() : nullptr;
     sa.flag_undecided = (l == D) ? 1 : 0;
     sa.flag_ids = rb.flag_ids.as<int32_t>();
     sa.flag_count = rb.flag_count.as<int32_t>();
@@ -462,8 +489,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     a.counts = rb.counts.as<int32_t>();
     a.layer = l;
     a.qmap = rb.qmap.as<int32_t>();
-    a.qsrc = lx.gather ? rb.qsrc.as<int32_t>() : nullptr;
-    if (l == 0 && lx.mfma && cfg.bits && net->bits.p && input == net->store.as<_Float16>()) {
+    a.qsrc = lx.gather || lx.slab ? rb.qsrc.as<int32_t>() : nullptr;
+    if (l == 0 && (lx.mfma || lx.slab) && cfg.bits && net->bits.p && input == net->store.as<_Float16>()) {
       a.xbits = net->bits.as<uint32_t>();
       a.xbits_rows = net->N;
     }
@@ -490,6 +517,16 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     const Dtype t_in = l == 0 ? Dtype::F16 : (fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32));
     const Dtype t_out = fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32);
     switch (kern) {
+      case COSPDNN_KERNEL_SLAB: {
+        // consecutive slab layers pass the activations sample-blocked by the slab size; the
+        // buffers any other kernel reads or writes (and the last layer's output) stay neuron-major
+        a.x_rows = input_rows(net, l);
+        a.x_layout = l >= 1 && pick_kernel(net, cfg, l - 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
+        a.y_layout = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
+        const int64_t slabs = (S_run + slab_samples(a.x_rows) - 1) / slab_samples(a.x_rows);
+        HIP_TRY(launch_spmm_slab(a, view, capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(slabs, e->cu_count)))), st));
+        break;
+      }
       case COSPDNN_KERNEL_MFMA_BAND:
       case COSPDNN_KERNEL_MFMA_BAND_MASK: {
         a.order = 1;
@@ -794,6 +831,7 @@ int run_config(cospdnn_net* net, const cospdnn_opts& o, RunCfg& cfg) {
   cfg.gather = o.precision == COSPDNN_FP16 ? o.spmm_gather : 0;
   cfg.bits = o.precision == COSPDNN_FP16 ? o.input_bits : 0;
   cfg.wmask = o.precision == COSPDNN_FP16 ? o.weight_mask : 0;
+  cfg.slab = o.precision == COSPDNN_FP16 ? o.spmm_slab : 0;
   return COSPDNN_OK;
 }
 
@@ -825,6 +863,7 @@ bool graph_matches(const GraphCache& gc, const RunCfg& cfg, int64_t plan_version
          gc.band_from == cfg.band_from && gc.use_mfma == cfg.use_mfma && gc.mfma32 == cfg.mfma32 &&
          gc.order == cfg.order && gc.groups == cfg.groups_per_cu && gc.plan_version == plan_version &&
          gc.keep_ppm == cfg.keep_ppm && gc.gather == cfg.gather && gc.bits == cfg.bits && gc.wmask == cfg.wmask &&
+         gc.slab == cfg.slab &&
          gc.grid_div == cfg.grid_div &&
          gc.dev_guard == x.dev_guard && gc.guard_layer32 == x.guard_layer32 && gc.guard_tail == x.guard_tail &&
          gc.gen_main == x.gen_main && gc.gen_rec == x.gen_rec && gc.gen_rep == x.gen_rep;
@@ -849,6 +888,7 @@ void graph_remember(GraphCache& gc, const RunCfg& cfg, int64_t plan_version, con
   gc.gather = cfg.gather;
   gc.bits = cfg.bits;
   gc.wmask = cfg.wmask;
+  gc.slab = cfg.slab;
   gc.grid_div = cfg.grid_div;
   gc.plan_version = plan_version;
 }
@@ -1132,7 +1172,8 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
     bytes += 2.0 * lv * N * ab + static_cast<double>(net->dl[l]->nnz) * ab;
     const int k = pick_kernel(net, cfg, l, S);
     if (res->layer_kernel) res->layer_kernel[l] = k;
-    const bool m = k != COSPDNN_KERNEL_SIMT && k != COSPDNN_KERNEL_SIMT_ROWS && k != COSPDNN_KERNEL_WAVE_ROWS;
+    const bool m = k != COSPDNN_KERNEL_SIMT && k != COSPDNN_KERNEL_SIMT_ROWS && k != COSPDNN_KERNEL_WAVE_ROWS &&
+                   k != COSPDNN_KERNEL_SLAB;
     mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;
@@ -1254,7 +1295,7 @@ size_t carve_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool ne
   take(rb.work, sizeof(int32_t));
   take(rb.ret_ids, sizeof(int32_t) * ld);
   take(rb.ret_count, sizeof(int32_t));
-  const size_t elems = static_cast<size_t>(N) * ld;
+  const size_t elems = static_cast<size_t>(slab_rows(N)) * ld;  // as ensure_bufs
   DevBuf* act = precision == COSPDNN_FP16 ? rb.act16 : (precision == COSPDNN_FP32 ? rb.act32 : rb.act64);
   const size_t esz = precision == COSPDNN_FP16 ? 2 : (precision == COSPDNN_FP32 ? 4 : 8);
   take(act[0], elems * esz);
@@ -1473,6 +1514,8 @@ const char* cospdnn_kernel_name(int kernel) {
     case COSPDNN_KERNEL_MFMA_BAND_MASK: return "spmm_mfma_band_mask_kernel";
     case COSPDNN_KERNEL_WAVE_ROWS: return "spmm_wave_rows_kernel";
     case COSPDNN_KERNEL_MFMA32: return "spmm_mfma32_f32in_kernel";  // layer 0: spmm_mfma32_f16in_kernel
+    // spmm_slab_kernel_<T>, or spmm_slab_uni_kernel_<T> for a layer of one weight value
+    case COSPDNN_KERNEL_SLAB: return "spmm_slab_kernel";
     default: return nullptr;
   }
 }
@@ -1501,6 +1544,7 @@ void cospdnn_default_opts(cospdnn_opts* o) {
   o->weight_mask = 1;
   o->weight_budget_mb = 0;
   o->batch_cu_share = 0;
+  o->spmm_slab = 0;
 }
 
 int cospdnn_engine_create(int device, cospdnn_engine** out) {
@@ -1756,6 +1800,7 @@ int cospdnn_net_upload(cospdnn_net* net, const cospdnn_opts* opts) {
           if (l >= 1) remap_inputs(b, l > l0 ? bl[l - 1 - l0].out_idx : carry, net->N);
           if (b.has_tiles && !streamed[l - l0] && b.tile_waste > kBandMinWaste && net->N <= kBandMaxN)
             build_band_image(b, net->N);
+          build_slab_image(b, net->N);
         });
       for (auto& t : th) t.join();
       for (int l = l0; l < l1; ++l) {

@@ -73,6 +73,7 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 struct LayerStore {
   DevBuf out_idx, optr, orow, oval, kptr, gidx, wt, wt32, wmask;
   DevBuf bptr, bgidx, bwt, bmask;  // band image (order 6)
+  DevBuf sgoff, sid, sval;  // slab image (sample-slab kernel)
   DevBuf out_logical;  // N int32: the neuron id of each activation row this layer writes
   DevLayer view;
   bool has_tiles = false;
@@ -130,7 +131,7 @@ using cospdnn_detail::RunBufs;
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0, gather = 0, bits = 0, wmask = 0, grid_div = 1;
+  int precision = -1, guard_layer = 0, band_from = 0, use_mfma = 0, mfma32 = 0, order = 0, groups = 0, keep_ppm = 0, gather = 0, bits = 0, wmask = 0, slab = 0, grid_div = 1;
   int64_t plan_version = -1;
   // the device-side guard pipeline captured with the main pass (0: main pass only)
   int dev_guard = 0, guard_layer32 = 0, guard_tail = 0;

@@ -335,6 +335,61 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out) 
   return true;
 }
 
+void build_slab_image(BlockedLayer& bl, int in_rows) {
+  bl.has_slab = false;
+  bl.sgoff.clear();
+  bl.sid.clear();
+  bl.sval.clear();
+  const int T = slab_samples(in_rows);
+  if (T == 0 || !bl.fp16_exact || bl.N < 1) return;
+  const int bs = slab_batch_slots(T), m_per = slab_lane_slots(T), opw = bs / m_per;
+  const int nout = bl.N;  // slots 0..N-1 hold the outputs (block order), padding slots follow
+  const int nb = (nout + bs - 1) / bs;
+  auto slot_of = [&](int s) { return s < nout ? s : nout - 1; };  // slots past the end repeat the last
+  auto len_of = [&](int s) { return bl.optr[slot_of(s) + 1] - bl.optr[slot_of(s)]; };
+  bool uniform = true;
+  const float v0 = bl.oval.empty() ? 0.0f : bl.oval[0];
+  for (float v : bl.oval)
+    if (v != v0) {
+      uniform = false;
+      break;
+    }
+  bl.sgoff.assign(nb + 1, 0);
+  bool pads = false;
+  for (int b = 0; b < nb; ++b) {
+    int most = 0;
+    for (int i = 0; i < bs; ++i) most = std::max(most, len_of(b * bs + i));
+    const int oct = std::max(1, (most + 7) / 8);
+    for (int i = 0; i < bs; ++i) pads = pads || len_of(b * bs + i) != 8 * oct;
+    bl.sgoff[b + 1] = bl.sgoff[b] + oct;
+  }
+  // the zero row (slab_rows(in_rows)) has no 16-bit id when the input has more than 65528 rows:
+  // pads then read row 0 at value 0
+  const int zrow = slab_rows(in_rows);
+  const bool zero_row = zrow <= 65535;
+  if (pads && !zero_row) uniform = false;
+  const uint16_t pad_id = zero_row ? static_cast<uint16_t>(zrow) : 0;
+  const size_t entries = static_cast<size_t>(bl.sgoff[nb]) * bs * 8;
+  bl.sid.assign(entries, pad_id);
+  if (!uniform) bl.sval.assign(entries, 0);  // FP16 zero
+  for (int b = 0; b < nb; ++b)
+    for (int i = 0; i < bs; ++i) {
+      // batch slot i = m * opw + oi: lane oi's m-th slot, stored at lane position oi * m_per + m
+      const int m = i / opw, oi = i % opw;
+      const int slot = slot_of(b * bs + i);
+      const int32_t q0 = bl.optr[slot], q1 = bl.optr[slot + 1];
+      for (int32_t q = q0; q < q1; ++q) {
+        const int k = q - q0;
+        const size_t at = (static_cast<size_t>(bl.sgoff[b] + k / 8) * bs + oi * m_per + m) * 8 + k % 8;
+        bl.sid[at] = static_cast<uint16_t>(bl.orow[q]);
+        if (!uniform) bl.sval[at] = float_to_half_bits(bl.oval[q]);
+      }
+    }
+  bl.slab_uniform = uniform;
+  bl.slab_value = uniform ? v0 : 0.0f;
+  bl.has_slab = true;
+}
+
 void build_band_image(BlockedLayer& bl, int in_rows) {
   const int nb = std::max(1, (in_rows + kBandRows - 1) / kBandRows);
   bl.bands = nb;

@@ -62,6 +62,15 @@ struct BlockedLayer {
   std::vector<int32_t> bgidx;
   std::vector<uint16_t> bwt;
   std::vector<uint16_t> bmask;  // when has_mask
+  // Slab image (sample-slab kernel, layout in kernels.hpp DevLayer::sid): the SIMT image's lists
+  // per batch of slab_batch_slots(T) output slots in octets of 8 rows, interleaved by lane, as
+  // 16-bit rows (and FP16 values unless every weight is slab_value).
+  bool has_slab = false;
+  bool slab_uniform = false;
+  float slab_value = 0.0f;
+  std::vector<int32_t> sgoff;  // batches+1, offsets in octets
+  std::vector<uint16_t> sid;   // sgoff[batches] * slab_batch_slots(T) * 8
+  std::vector<uint16_t> sval;  // the same size (FP16 bits) unless slab_uniform
   bool has_tiles32 = false;    // wt32 built (tile_waste <= kMaxWasteFp32Tiles)
   int64_t nchunk = 0;
   double tile_waste = 0.0;     // MFMA MACs per useful MAC (nchunk*32*32 / nnz)
@@ -92,6 +101,13 @@ bool build_layer(const CscLayer& w, int N, double max_waste, BlockedLayer& out);
 // the layer's input, in_rows of them).  Fragments follow the values' FP16 bits; mask words are
 // built when the tile image has a mask (one value throughout).
 void build_band_image(BlockedLayer& bl, int in_rows);
+
+// The slab image of a layer from its SIMT image (rows already in the activation-row order of
+// the layer's input, in_rows of them): built when slab_samples(in_rows) > 0 and every weight is
+// FP16-exact.  Every batch holds at least one octet; shorter lists are padded with the zero
+// row (slab_rows(in_rows)) at value 0, or with row 0 at value 0 past 65528 input rows (no
+// 16-bit id for the zero row; the image then carries values even for one weight value).
+void build_slab_image(BlockedLayer& bl, int in_rows);
 
 // True when an all-32 input row gives an all-32 output row whatever the summation order
 // in FP32 or FP64: every weight is nonnegative and, for every column j (empty ones

@@ -1294,6 +1294,545 @@ __global__ __launch_bounds__(kBandThreads, 1) void spmm_mfma_band_mask_kernel(St
 }
 
 // --------------------------------------------------------------------------
+// Sample-slab SpMM (FP16 pass, plan kind SLAB) for layers whose MFMA images waste most of
+// their work: on a random layer a block column's 32 outputs gather ~32 chunks of 32 rows
+// with about one nonzero per row, so the MFMA kernels multiply ~32 zeros per useful product
+// and re-read every input row once per block column (or per band).  Here a work-group holds
+// a slab of T consecutive input positions in LDS, all input rows of them (T = slab_samples:
+// 64 up to 1023 rows, 16 at 4096, 4 at 16384, 1 at 65536), and computes every output of
+// those samples from it.  Each lane takes one output slot and min(T, 8) of the slab's
+// samples: it reads the slot's input rows from the slab image (16-bit row ids, 8 per 16-byte
+// load), gathers each row's samples from LDS in one read and accumulates them in FP32 with
+// v_fma_mix_f32, one instruction per sample and nonzero: the layer's MACs and no more.
+// Every input value moves from memory once, the flags of a sample are the OR over all of its
+// outputs (reduced in the work-group), and the epilogue fuses bias, clamp and FP16 store.
+// In the sample-blocked layout (kLayoutSlab) a slab is one contiguous block, loaded into
+// registers a slab ahead while the current one is computed, and a wave's stores cover
+// contiguous runs; the neuron-major layout and the bit-packed layer-0 input are read (and
+// neuron-major written) too, so the kernel can sit next to the MFMA kernels in a plan.
+// Stores go to the samples' compacted positions (qmap); a slab whose samples keep T
+// consecutive positions from a multiple of T stores whole vectors, any other one value by
+// value.
+// --------------------------------------------------------------------------
+constexpr int kSlabThreads = 512;  // 8 waves, 2 per SIMD (256 VGPRs for deep pipelines); one work-group per CU
+constexpr int kSlabWaves = kSlabThreads / 64;
+
+template <int T>
+struct SlabShape {
+  static constexpr int kP = T < 4 ? T : 4;       // samples per plane
+  static constexpr int kSpl = T < 16 ? T : 16;   // samples per lane
+  static constexpr int kLpo = T / kSpl;          // lanes per output slot
+  static constexpr int kOpw = 64 / kLpo;         // output slots per wave at a time
+  static constexpr int kReads = kSpl / kP;       // LDS reads per row id and lane (one per plane)
+  static constexpr int kPlanes = T / kP;
+  static constexpr int kM = slab_lane_slots(T);  // output slots per lane
+  static constexpr int kBatch = slab_batch_slots(T);  // slots of a wave's batch (kOpw * kM)
+  // LDS bytes per plane (compile-time, so that plane offsets fold into the reads' immediates)
+  static constexpr int kPb = slab_plane_cap(T);
+};
+
+typedef __attribute__((address_space(3))) _Float16 lds_half_t;
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+// plain stores into LDS (vector types of the HIP headers cannot be assigned in address space 3)
+__device__ __forceinline__ void lds_put16(lds_half_t* p, uint4 v) {
+  *reinterpret_cast<__attribute__((address_space(3))) u32x4_t*>(p) = u32x4_t{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void lds_put8(lds_half_t* p, uint32_t lo, uint32_t hi) {
+  *reinterpret_cast<__attribute__((address_space(3))) u32x2_t*>(p) = u32x2_t{lo, hi};
+}
+__device__ __forceinline__ void lds_put4(lds_half_t* p, uint32_t v) {
+  *reinterpret_cast<__attribute__((address_space(3))) uint32_t*>(p) = v;
+}
+__device__ __forceinline__ void lds_put2(lds_half_t* p, uint16_t v) {
+  *reinterpret_cast<__attribute__((address_space(3))) uint16_t*>(p) = v;
+}
+
+// P consecutive FP16 values of one plane row from LDS (byte offset off): one 8-byte read for P = 4
+template <int P>
+__device__ __forceinline__ void slab_read(const lds_half_t* base, uint32_t off, _Float16* x) {
+  const __attribute__((address_space(3))) char* p = reinterpret_cast<const __attribute__((address_space(3))) char*>(base) + off;
+  if constexpr (P == 4) {
+    const half4_t v = *reinterpret_cast<const __attribute__((address_space(3))) half4_t*>(p);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x[t] = v[t];
+  } else if constexpr (P == 2) {
+    const half2_t v = *reinterpret_cast<const __attribute__((address_space(3))) half2_t*>(p);
+    x[0] = v[0];
+    x[1] = v[1];
+  } else {
+    x[0] = *reinterpret_cast<const __attribute__((address_space(3))) _Float16*>(p);
+  }
+}
+
+// P FP16 values (packed in dwords, low half first) to global memory at p
+template <int P>
+__device__ __forceinline__ void slab_store(_Float16* p, const uint32_t* d) {
+  if constexpr (P == 4)
+    *reinterpret_cast<uint2*>(p) = make_uint2(d[0], d[1]);
+  else if constexpr (P == 2)
+    *reinterpret_cast<uint32_t*>(p) = d[0];
+  else
+    *reinterpret_cast<uint16_t*>(p) = static_cast<uint16_t>(d[0]);
+}
+
+// Element index of input position p, row r in a sample-blocked buffer of R rows per plane.
+template <int T>
+__device__ __forceinline__ int64_t slab_index(int p, int r, int R) {
+  constexpr int P = SlabShape<T>::kP;
+  return static_cast<int64_t>(p / T) * T * R + static_cast<int64_t>((p % T) / P) * R * P + static_cast<int64_t>(r) * P + p % P;
+}
+
+// Slab sl (positions sl*T ..: input positions = output positions, no compaction) into the LDS
+// planes straight from memory: the bit-packed layer-0 input or T values of every neuron-major
+// row (through registers).  Plane k at half k * PB/2, row r at + r * P.
+template <int T>
+__device__ __forceinline__ void slab_load_rows(const StepArgs& a, lds_half_t* lds, int sl, int x_rows, int64_t ldx,
+                                               int tid) {
+  constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
+  const int p0 = sl * T;
+  if (a.xbits != nullptr) {
+    // 64-sample tile t, row r: words (t * xbits_rows + r) * 2 + h, bit i of word h = sample 64t + 32h + i
+    const int t = p0 >> 6, o = p0 & 63;
+    for (int r = tid; r < x_rows; r += kSlabThreads) {
+      const uint32_t* wp = a.xbits + (static_cast<int64_t>(t) * a.xbits_rows + r) * 2;
+      uint32_t bits[2];
+      if constexpr (T == 64) {
+        bits[0] = wp[0];
+        bits[1] = wp[1];
+      } else {
+        bits[0] = wp[o >> 5] >> (o & 31);  // the slab's T bits (T divides 32 and o)
+        bits[1] = 0;
+      }
+#pragma unroll
+      for (int k = 0; k < T / P; ++k) {
+        // plane k: samples kP .. kP+P-1, dword j holds samples kP+2j (low half) and kP+2j+1
+        uint32_t d[2] = {0u, 0u};
+#pragma unroll
+        for (int j = 0; j < (P + 1) / 2; ++j) {
+          const int sp = k * P + 2 * j;
+          const uint32_t bb = (bits[sp >> 5] >> (sp & 31)) & 3u;
+          d[j] = (bb & 1u) * 0x3C00u | (bb >> 1) * 0x3C000000u;
+        }
+        lds_half_t* dst = lds + k * PH + r * P;
+        if constexpr (P == 4)
+          lds_put8(dst, d[0], d[1]);
+        else if constexpr (P == 2)
+          lds_put4(dst, d[0]);
+        else
+          lds_put2(dst, static_cast<uint16_t>(d[0]));
+      }
+    }
+  } else {
+    const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+    if constexpr (T >= 8) {
+      // 16-byte pieces of a row (8 samples = two planes)
+      constexpr int ppr = T / 8;
+      const int pieces = x_rows * ppr;
+      for (int i = tid; i < pieces; i += kSlabThreads) {
+        const int r = i / ppr, part = i % ppr;
+        const uint4 v = *reinterpret_cast<const uint4*>(X + static_cast<int64_t>(r) * ldx + p0 + 8 * part);
+        lds_put8(lds + (2 * part) * PH + r * 4, v.x, v.y);
+        lds_put8(lds + (2 * part + 1) * PH + r * 4, v.z, v.w);
+      }
+    } else {
+      for (int r = tid; r < x_rows; r += kSlabThreads) {
+        const _Float16* src = X + static_cast<int64_t>(r) * ldx + p0;
+        if constexpr (T == 4) {
+          const uint2 v = *reinterpret_cast<const uint2*>(src);
+          lds_put8(lds + r * 4, v.x, v.y);
+        } else if constexpr (T == 2) {
+          lds_put4(lds + r * 2, *reinterpret_cast<const uint32_t*>(src));
+        } else {
+          lds_put2(lds + r, *reinterpret_cast<const uint16_t*>(src));
+        }
+      }
+    }
+  }
+}
+
+// A sample-blocked slab (T / P planes of slab_rows(x_rows) rows, contiguous in HBM) straight into
+// the LDS planes, 16 bytes a lane (global_load_lds: the lanes of a wave fill 1 KB, pieces
+// k0 = w * 64 + k * kSlabThreads on), no registers held.  The caller waits for vmcnt(0) before
+// the barrier that publishes it.
+template <int T>
+__device__ __forceinline__ void slab_load_blocked(const StepArgs& a, lds_half_t* lds, int sl, int x_rows, int wave,
+                                                  int lane) {
+  constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
+  const int R = slab_rows(x_rows);
+  const int pp = R * P / 8;  // 16-byte pieces per plane (R is a multiple of 8)
+  const _Float16* src = static_cast<const _Float16*>(a.x) + static_cast<int64_t>(sl) * T * R;
+#pragma unroll 1
+  for (int k = 0; k < T / P; ++k) {
+    const _Float16* sp = src + static_cast<int64_t>(k) * R * P;
+    lds_half_t* dp = lds + k * PH;
+    for (int k0 = wave * 64; k0 < pp; k0 += kSlabThreads) {
+      const int i = k0 + lane;
+      if (i < pp) __builtin_amdgcn_global_load_lds(sp + 8 * i, (lds_ptr_t)(dp + 8 * k0), 16, 0, 0);
+    }
+  }
+}
+
+// The slab of output positions q0 = sl*T .. after a compacting scan: sample t is input position
+// psrc[t] (-1: past the live rows, loaded as zeros), gathered value by value from the input
+// (sample-blocked, neuron-major or bit-packed).  Thread (t, r): t = tid % T, rows r = tid / T +
+// k * (kSlabThreads / T); neighbouring threads read neighbouring positions of one row.
+template <int T>
+__device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* lds, const int* psrc, int x_rows,
+                                                 int64_t ldx, bool x_slab, int tid) {
+  constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
+  constexpr int kRowsPer = kSlabThreads / T;
+  const int t = tid % T;
+  const int p = psrc[t];
+  const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+  const int R = slab_rows(x_rows);
+  lds_half_t* dst = lds + (t / P) * PH + t % P;
+  for (int r0 = tid / T; r0 < x_rows; r0 += 8 * kRowsPer) {
+    uint16_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u * kRowsPer;
+      v[u] = 0;
+      if (p >= 0 && r < x_rows) {
+        if (a.xbits != nullptr) {
+          const uint32_t wd = a.xbits[(static_cast<int64_t>(p >> 6) * a.xbits_rows + r) * 2 + ((p >> 5) & 1)];
+          v[u] = ((wd >> (p & 31)) & 1u) ? 0x3C00u : 0u;
+        } else if (x_slab) {
+          v[u] = __builtin_bit_cast(uint16_t, X[slab_index<T>(p, r, R)]);
+        } else {
+          v[u] = __builtin_bit_cast(uint16_t, X[static_cast<int64_t>(r) * ldx + p]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u * kRowsPer;
+      if (r < x_rows) lds_put2(dst + r * P, v[u]);
+    }
+  }
+}
+
+// OR of the lanes' flag bits per slab sample: lane (slot, group g) holds bit b for sample
+// g * SPL + b; the result's bit s is set when any lane's bit for sample s is.
+template <int T>
+__device__ __forceinline__ uint64_t slab_wave_bits(uint32_t f) {
+  constexpr int SPL = SlabShape<T>::kSpl, LPO = SlabShape<T>::kLpo;
+  // lanes l with l % LPO == 0
+  constexpr uint64_t kLanes0 = LPO == 1 ? ~0ull : (LPO == 2 ? 0x5555555555555555ull : 0x1111111111111111ull);
+  uint64_t out = 0;
+#pragma unroll
+  for (int b = 0; b < SPL; ++b) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(((f >> b) & 1u) != 0u);
+#pragma unroll
+    for (int g = 0; g < LPO; ++g) out |= ((m & (kLanes0 << g)) != 0ull ? 1ull : 0ull) << (g * SPL + b);
+  }
+  return out;
+}
+
+// A wave's position in the slab image: batch b (b * B >= nout: past the end), octet o of its nk,
+// and the lane's entry offset of octet 0 (elements of sid / sval).
+struct SlabCur {
+  int b, o, nk;
+  int64_t e;
+};
+template <int T>
+__device__ __forceinline__ void slab_cur_batch(SlabCur& c, int b, int nout, int oi, cint_ptr_t sgoff) {
+  constexpr int B = SlabShape<T>::kBatch, M = SlabShape<T>::kM;
+  c.b = b;
+  c.o = 0;
+  c.nk = 0;
+  c.e = 0;
+  if (b * B < nout) {
+    const int k0 = sgoff[b];
+    c.nk = sgoff[b + 1] - k0;
+    c.e = (static_cast<int64_t>(k0) * B + oi * M) * 8;
+  }
+}
+template <int T>
+__device__ __forceinline__ void slab_cur_next(SlabCur& c, int nout, int oi, cint_ptr_t sgoff) {
+  if (c.b * SlabShape<T>::kBatch >= nout) return;
+  if (c.o + 1 < c.nk) {
+    ++c.o;
+    return;
+  }
+  slab_cur_batch<T>(c, c.b + kSlabWaves, nout, oi, sgoff);
+}
+// One octet of a lane's M slots: 8 row ids (and values given, 8 FP16 values) of each.
+template <int M>
+struct SlabOct {
+  uint4 id[M], val[M];
+};
+// Unconditional (a cursor past the end reads octet 0 of the image, never used), so that every
+// step issues the same loads.
+template <int T, bool UNI>
+__device__ __forceinline__ SlabOct<SlabShape<T>::kM> slab_oct_load(const DevLayer& w, const SlabCur& c) {
+  constexpr int B = SlabShape<T>::kBatch, M = SlabShape<T>::kM;
+  SlabOct<M> k;
+  const int64_t at = c.e + static_cast<int64_t>(c.o) * (B * 8);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    k.id[m] = *reinterpret_cast<const uint4*>(w.sid + at + 8 * m);
+    if constexpr (!UNI)
+      k.val[m] = *reinterpret_cast<const uint4*>(w.sval + at + 8 * m);
+    else
+      k.val[m] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  return k;
+}
+
+template <int T, bool UNI, bool BAND>
+__device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, lds_half_t* lds) {
+  using Sh = SlabShape<T>;
+  constexpr int P = Sh::kP, SPL = Sh::kSpl, LPO = Sh::kLpo, OPW = Sh::kOpw, NR = Sh::kReads, PB = Sh::kPb, M = Sh::kM;
+  constexpr int ND = (SPL + 1) / 2;       // dwords of a lane's SPL FP16 outputs
+  __shared__ int psrc[T];                 // input position of each slab sample (-1: none)
+  __shared__ uint32_t sflag[2][3][2];     // per slab parity and flag kind: T bits
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  const bool ident = n_out == n;
+  const int x_rows = a.x_rows;
+  const int nout = w.n_out;
+  const int ry = slab_rows(nout);            // rows per plane of a sample-blocked output slab
+  // slabs are output positions; without compaction they are the input positions too
+  const int nsl = (n_out + T - 1) / T;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int oi = lane / LPO, g = lane % LPO;
+  const int64_t ldx = x_ld(a), ldy = y_ld(a);
+  _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const bool y_slab = a.y_layout == kLayoutSlab;
+  const bool x_slab = a.x_layout == kLayoutSlab && a.layer > 0 && a.xbits == nullptr;
+  const cint_ptr_t sgoff = (cint_ptr_t)w.sgoff;
+  // BAND: the guard-band flags are wanted (a.band_next set); a kernel of its own, since the
+  // band test per output is as costly as the rest of the epilogue
+  const bool want_unsat = a.unsat_next != nullptr;
+  constexpr bool want_band = BAND;
+  // FP16 1.0 that the compiler cannot fold (in the uniform case sval is null), so that the
+  // uniform case keeps the f16 x f16 + f32 form (v_fma_mix_f32) instead of a conversion and
+  // an add
+  const _Float16 one_h = w.sval == nullptr ? static_cast<_Float16>(1.0f) : static_cast<_Float16>(2.0f);
+  const float bias = a.bias, wuni = w.svalue;
+  // LDS byte offset of the lane's first plane (its samples are planes g * NR ..)
+  const uint32_t lane_base = static_cast<uint32_t>(g * NR * PB);
+  // the zero row of every plane (pads of shorter lists)
+  const int zr = slab_rows(x_rows);
+  for (int i = tid; i < T; i += kSlabThreads) lds[(i / P) * (PB / 2) + zr * P + i % P] = static_cast<_Float16>(0.0f);
+  auto load_slab = [&](int sl) {
+    if (ident) {
+      if (x_slab)
+        slab_load_blocked<T>(a, lds, sl, x_rows, wave, lane);
+      else
+        slab_load_rows<T>(a, lds, sl, x_rows, ldx, tid);
+    } else {
+      slab_load_gather<T>(a, lds, psrc, x_rows, ldx, x_slab, tid);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  };
+  auto set_src = [&](int sl) {
+    if (tid < T) {
+      const int q = sl * T + tid;
+      psrc[tid] = q < n_out ? (ident ? q : a.qsrc[q]) : -1;
+    }
+  };
+  int sl = blockIdx.x;
+  set_src(sl);
+  __syncthreads();  // psrc of the first slab
+  if (sl < nsl) load_slab(sl);
+  for (int it = 0; sl < nsl; ++it) {
+    const int par = it & 1;
+    if (tid < 6) sflag[par][tid >> 1][tid & 1] = 0u;
+    __syncthreads();  // the slab and cleared flag words are in place
+    const int next = sl + static_cast<int>(gridDim.x);
+    const int q0 = sl * T;
+    // per lane: packed FP16 max (alive: > 0) and min (unsaturated: < 32) of its samples over
+    // its slots, reduced to flag bits once per slab
+    uint32_t fmx[ND], fmn[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      fmx[j] = 0u;           // two FP16 zeros
+      fmn[j] = 0x50005000u;  // two FP16 32.0
+    }
+    uint32_t fb = 0;
+    // The wave's steps, one octet (8 nonzeros of each of the lane's M slots, 128 FMAs a lane) per
+    // step: batches b = wave, wave + kSlabWaves, ... (every batch has at least one octet),
+    // octets in order.  Row ids (and values) sit in a ring of kSlabAhead entries: step k
+    // processes entry k % kSlabAhead, then reloads it for step k + kSlabAhead; the loop is
+    // unrolled over the ring (no register copies that would wait early).  The epilogue's stores
+    // make the next waits drain every load (the compiler's counter model), once per batch.
+    constexpr int kSlabAhead = 3;
+    SlabCur cur, ahead;
+    slab_cur_batch<T>(cur, wave, nout, oi, sgoff);
+    ahead = cur;
+    SlabOct<M> ring[kSlabAhead];
+#pragma unroll
+    for (int u = 0; u < kSlabAhead; ++u) {
+      ring[u] = slab_oct_load<T, UNI>(w, ahead);
+      slab_cur_next<T>(ahead, nout, oi, sgoff);
+    }
+    float acc[M][SPL];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+      for (int t = 0; t < SPL; ++t) acc[m][t] = 0.0f;
+    auto step = [&](SlabOct<M>& slot) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const uint32_t dw[4] = {slot.id[m].x, slot.id[m].y, slot.id[m].z, slot.id[m].w};
+        const uint32_t wv[4] = {slot.val[m].x, slot.val[m].y, slot.val[m].z, slot.val[m].w};
+        // the slot's 8 rows' reads first, then the FMAs as they land
+        _Float16 x[8][SPL];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t r = (dw[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+          const uint32_t off = lane_base + r * (P * 2);
+          // plane offsets fold into the reads' immediate offsets (below 64 KB)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
+            constexpr int kFold = 65535 / PB + 1;  // planes one base covers
+            const uint32_t base = off + static_cast<uint32_t>((q / kFold) * kFold * PB);
+            slab_read<P>(lds + (q % kFold) * (PB / 2), base, &x[e][q * P]);
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const _Float16 wh =
+              UNI ? one_h : __builtin_bit_cast(_Float16, static_cast<uint16_t>(wv[e >> 1] >> (16 * (e & 1))));
+#pragma unroll
+          for (int t = 0; t < SPL; ++t)
+            acc[m][t] = __builtin_fmaf(static_cast<float>(x[e][t]), static_cast<float>(wh), acc[m][t]);
+        }
+      }
+      if (cur.o + 1 == cur.nk) {
+        // the batch's last octet: epilogue of the lane's M slots (bias, clamp, FP16, aligned store)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int s = cur.b * Sh::kBatch + m * OPW + oi;
+          if (s < nout) {
+            uint32_t d[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) d[j] = 0u;
+#pragma unroll
+            for (int t = 0; t < SPL; ++t) {
+              const float z = UNI ? __builtin_fmaf(acc[m][t], wuni, bias) : acc[m][t] + bias;
+              if (want_band) fb |= (fabsf(z) < a.band || fabsf(z - 32.0f) < a.band ? 1u : 0u) << t;
+              const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
+              d[t >> 1] |= static_cast<uint32_t>(__builtin_bit_cast(uint16_t, yv)) << (16 * (t & 1));
+            }
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+              const half2_t v = __builtin_bit_cast(half2_t, d[j]);
+              fmx[j] = __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(half2_t, fmx[j]), v));
+              fmn[j] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(half2_t, fmn[j]), v));
+            }
+            if (y_slab) {
+              // slab q0 / T of the output, planes g * NR .., row s
+              _Float16* yb = Y + static_cast<int64_t>(q0) * ry + static_cast<int64_t>(g * NR) * ry * P +
+                             static_cast<int64_t>(s) * P;
+#pragma unroll
+              for (int q = 0; q < NR; ++q) slab_store<P>(yb + static_cast<int64_t>(q) * ry * P, d + q * ((P + 1) / 2));
+            } else {
+              _Float16* yp = Y + static_cast<int64_t>(s) * ldy + q0 + g * SPL;
+#pragma unroll
+              for (int q = 0; q < NR; ++q) slab_store<P>(yp + q * P, d + q * ((P + 1) / 2));
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < SPL; ++t) acc[m][t] = 0.0f;
+        }
+        // Drain the stores here, once per batch: the compiler's wait counting treats loads and
+        // stores in flight together as unordered, and with a store possibly pending every later
+        // wait for a row-id load would drain everything, the loads just issued included.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      }
+      slot = slab_oct_load<T, UNI>(w, ahead);
+      slab_cur_next<T>(ahead, nout, oi, sgoff);
+      slab_cur_next<T>(cur, nout, oi, sgoff);
+    };
+    // the wave's step count (octets over its batches), so that the main loop runs whole rounds
+    // of the ring with no exit test between steps (the compiler's waits then count exactly)
+    int nsteps = 0;
+    for (int b = wave; b * Sh::kBatch < nout; b += kSlabWaves) nsteps += sgoff[b + 1] - sgoff[b];
+    int k = 0;
+    for (; k + kSlabAhead <= nsteps; k += kSlabAhead) {
+#pragma unroll
+      for (int u = 0; u < kSlabAhead; ++u) step(ring[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kSlabAhead - 1; ++u)
+      if (k + u < nsteps) step(ring[u]);
+    // flags of the slab's samples: per-sample bits from the lane's max / min, OR over the wave,
+    // then over the work-group in LDS
+    uint32_t fa = 0, fu = 0;
+#pragma unroll
+    for (int t = 0; t < SPL; ++t) {
+      const uint32_t hx = (fmx[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+      const uint32_t hn = (fmn[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+      fa |= (hx != 0u ? 1u : 0u) << t;        // max > 0 (values are in [0, 32])
+      fu |= (hn < 0x5000u ? 1u : 0u) << t;    // min < 32
+    }
+    const uint64_t wa = slab_wave_bits<T>(fa);
+    const uint64_t wu = want_unsat ? slab_wave_bits<T>(fu) : 0ull;
+    const uint64_t wb = want_band ? slab_wave_bits<T>(fb) : 0ull;
+    if (lane == 0) {
+      if (wa != 0ull) {
+        atomicOr(&sflag[par][0][0], static_cast<uint32_t>(wa));
+        if (T > 32) atomicOr(&sflag[par][0][1], static_cast<uint32_t>(wa >> 32));
+      }
+      if (wu != 0ull) {
+        atomicOr(&sflag[par][1][0], static_cast<uint32_t>(wu));
+        if (T > 32) atomicOr(&sflag[par][1][1], static_cast<uint32_t>(wu >> 32));
+      }
+      if (wb != 0ull) {
+        atomicOr(&sflag[par][2][0], static_cast<uint32_t>(wb));
+        if (T > 32) atomicOr(&sflag[par][2][1], static_cast<uint32_t>(wb >> 32));
+      }
+    }
+    __syncthreads();  // every read of this slab and every flag word are done
+    if (tid < T) {
+      const int q = q0 + tid;
+      if (q < n_out) {
+        if ((sflag[par][0][tid >> 5] >> (tid & 31)) & 1u) a.alive_next[q] = 1;
+        if (want_unsat && ((sflag[par][1][tid >> 5] >> (tid & 31)) & 1u)) a.unsat_next[q] = 1;
+        if (want_band && ((sflag[par][2][tid >> 5] >> (tid & 31)) & 1u)) a.band_next[q] = 1;
+      }
+    }
+    if (next < nsl) {
+      if (!ident) {
+        set_src(next);
+        __syncthreads();  // psrc of the next slab (gathered loads read it)
+      }
+      load_slab(next);
+    }
+    sl = next;
+  }
+}
+
+#define SLAB_KERNEL(T_)                                                                                 \
+  __global__ __launch_bounds__(kSlabThreads, 1) void spmm_slab_kernel_##T_(StepArgs a, DevLayer w) {     \
+    extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
+    slab_body<T_, false, false>(a, w, (lds_half_t*)slab_lds);                                           \
+  }                                                                                                     \
+  __global__ __launch_bounds__(kSlabThreads, 1) void spmm_slab_uni_kernel_##T_(StepArgs a, DevLayer w) { \
+    extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
+    slab_body<T_, true, false>(a, w, (lds_half_t*)slab_lds);                                            \
+  }                                                                                                     \
+  __global__ __launch_bounds__(kSlabThreads, 1) void spmm_slab_band_kernel_##T_(StepArgs a, DevLayer w) { \
+    extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
+    if (w.sval == nullptr)                                                                              \
+      slab_body<T_, true, true>(a, w, (lds_half_t*)slab_lds);                                           \
+    else                                                                                                \
+      slab_body<T_, false, true>(a, w, (lds_half_t*)slab_lds);                                          \
+  }
+SLAB_KERNEL(64)
+SLAB_KERNEL(32)
+SLAB_KERNEL(16)
+SLAB_KERNEL(8)
+SLAB_KERNEL(4)
+SLAB_KERNEL(2)
+SLAB_KERNEL(1)
+#undef SLAB_KERNEL
+
+// --------------------------------------------------------------------------
 // SIMT gather SpMM: thread = sample, work item = (tile of 256 samples, block of
 // 32 outputs); each output sums its CSC inputs read straight from global memory.
 // --------------------------------------------------------------------------
@@ -1781,6 +2320,45 @@ hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid,
     hipLaunchKernelGGL(spmm_mfma_band_kernel, dim3(grid), dim3(kBandThreads), spmm_mfma_band_bytes(), st, a, w);
   return hipGetLastError();
 }
+int spmm_slab_bytes(int t) {
+  switch (t) {
+    case 64: return SlabShape<64>::kPlanes * SlabShape<64>::kPb;
+    case 32: return SlabShape<32>::kPlanes * SlabShape<32>::kPb;
+    case 16: return SlabShape<16>::kPlanes * SlabShape<16>::kPb;
+    case 8: return SlabShape<8>::kPlanes * SlabShape<8>::kPb;
+    case 4: return SlabShape<4>::kPlanes * SlabShape<4>::kPb;
+    case 2: return SlabShape<2>::kPlanes * SlabShape<2>::kPb;
+    default: return SlabShape<1>::kPlanes * SlabShape<1>::kPb;
+  }
+}
+hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
+  const int t = slab_samples(a.x_rows);
+  if (t == 0 || w.sid == nullptr || w.sgoff == nullptr || grid < 1) return hipErrorInvalidValue;
+  const bool uni = w.sval == nullptr;
+  const size_t bytes = static_cast<size_t>(spmm_slab_bytes(t));
+#define SLAB_LAUNCH(T_)                                                                                       \
+  case T_:                                                                                                    \
+    if (a.band_next != nullptr)                                                                               \
+      hipLaunchKernelGGL(spmm_slab_band_kernel_##T_, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);        \
+    else if (uni)                                                                                             \
+      hipLaunchKernelGGL(spmm_slab_uni_kernel_##T_, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);         \
+    else                                                                                                      \
+      hipLaunchKernelGGL(spmm_slab_kernel_##T_, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);             \
+    break;
+  switch (t) {
+    SLAB_LAUNCH(64)
+    SLAB_LAUNCH(32)
+    SLAB_LAUNCH(16)
+    SLAB_LAUNCH(8)
+    SLAB_LAUNCH(4)
+    SLAB_LAUNCH(2)
+    SLAB_LAUNCH(1)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef SLAB_LAUNCH
+  return hipGetLastError();
+}
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   hipLaunchKernelGGL(spmm_wave_rows_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
@@ -1862,8 +2440,20 @@ hipError_t configure_kernels() {
   const hipError_t e3 = hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_band_kernel),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, band_bytes);
   if (e3 != hipSuccess) return e3;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_band_mask_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, band_bytes);
+  const hipError_t e4 = hipFuncSetAttribute(reinterpret_cast<const void*>(spmm_mfma_band_mask_kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, band_bytes);
+  if (e4 != hipSuccess) return e4;
+#define SLAB_KERNELS(T_)                                                                      \
+  reinterpret_cast<const void*>(spmm_slab_kernel_##T_), reinterpret_cast<const void*>(spmm_slab_uni_kernel_##T_), \
+      reinterpret_cast<const void*>(spmm_slab_band_kernel_##T_)
+  const void* slab_kernels[] = {SLAB_KERNELS(64), SLAB_KERNELS(32), SLAB_KERNELS(16), SLAB_KERNELS(8),
+                                SLAB_KERNELS(4),  SLAB_KERNELS(2),  SLAB_KERNELS(1)};
+#undef SLAB_KERNELS
+  for (const void* k : slab_kernels) {
+    const hipError_t e5 = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMaxBytes);
+    if (e5 != hipSuccess) return e5;
+  }
+  return hipSuccess;
 }
 
 int spmm_mfma_resident_per_cu() {

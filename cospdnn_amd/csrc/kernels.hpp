@@ -48,7 +48,57 @@ struct DevLayer {
   const float* wt32 = nullptr;       // FP32 tile image (MFMA FP32 kernel)
   int one_chunk = 0;                 // every block column has exactly one K chunk (kptr[c] = c)
   int full_blocks = 0;               // every out_idx slot holds an output (no padding slots)
+  // slab image (null: none; the sample-slab kernel), for T = slab_samples(input rows): the output
+  // slots in batches of B = slab_batch_slots(T) (a wave's unit of work: lane oi of the wave takes
+  // slots oi, oi + B/M, .. of a batch, M = slab_lane_slots(T)); per batch the slots' input rows
+  // (16-bit activation rows of the layer's input) in octets of 8: batch b's octets are
+  // sgoff[b] .. sgoff[b + 1] - 1, and within octet o the lane's M slots follow each other, slot
+  // i = m * B/M + oi of the batch at sid[((o * B) + oi * M + m) * 8 ..].  A slot with fewer rows
+  // than its batch's octets hold is padded with the zero row (row slab_rows(x_rows), all zero
+  // in the kernel's LDS) at value 0.  sval: FP16 values in the same layout, or null when every
+  // weight equals svalue (then pads point at the zero row).  Slots n_out .. repeat slot n_out-1.
+  const int32_t* sgoff = nullptr;
+  const uint16_t* sid = nullptr;
+  const _Float16* sval = nullptr;
+  float svalue = 0.0f;
 };
+
+// LDS of the sample-slab kernel's slab (at most this many bytes).
+constexpr int kSlabMaxBytes = 131072 + 256;
+// Slab layout.  A slab holds T consecutive sample positions of every row, in planes of
+// P = min(T, 4) samples: plane k holds samples kP..kP+P-1 of all rows, P values a row, so that
+// a row's values for one plane are one 8-byte (P = 4) LDS read.  In HBM (the sample-blocked
+// activation layout) plane k of slab b holds slab_rows(rows) rows; in LDS every plane has one
+// more, the zero row slab_rows(x_rows) (all zero: pads of shorter weight lists), and starts on
+// a 16-byte boundary.
+__host__ __device__ constexpr int slab_plane(int t) { return t < 4 ? t : 4; }
+__host__ __device__ constexpr int slab_rows(int rows) { return (rows + 7) / 8 * 8; }
+__host__ __device__ constexpr int slab_lds_plane_bytes(int x_rows, int t) {
+  return ((slab_rows(x_rows) + 1) * slab_plane(t) * 2 + 15) / 16 * 16;
+}
+// LDS bytes per plane the kernel reserves for T samples (a compile-time constant per T, so that
+// plane offsets fold into the LDS reads' immediate offsets)
+__host__ __device__ constexpr int slab_plane_cap(int t) { return kSlabMaxBytes / (t / slab_plane(t)) / 16 * 16; }
+// A lane of the sample-slab kernel computes slab_lane_samples(T) samples of each of
+// slab_lane_slots(T) output slots (16 FP32 sums a lane and 128 FMAs per 8-row step down to T = 4;
+// 4 slots below); slab_batch_slots(T) slots make a wave's batch.
+__host__ __device__ constexpr int slab_lane_samples(int t) { return t < 16 ? t : 16; }
+__host__ __device__ constexpr int slab_lane_slots(int t) { return 16 / slab_lane_samples(t) < 4 ? 16 / slab_lane_samples(t) : 4; }
+__host__ __device__ constexpr int slab_batch_slots(int t) { return 64 / (t / slab_lane_samples(t)) * slab_lane_slots(t); }
+// Samples per slab for a layer with x_rows input rows: the largest power of two up to 64 whose
+// planes (with the zero row) fit slab_plane_cap; 0 when none does (no slab kernel for the layer).
+__host__ __device__ inline int slab_samples(int x_rows) {
+  if (x_rows < 1 || x_rows > 65536) return 0;
+  for (int t = 64; t >= 1; t >>= 1)
+    if (slab_lds_plane_bytes(x_rows, t) <= slab_plane_cap(t)) return t;
+  return 0;
+}
+// Activation layouts (sample-slab kernel): neuron-major [rows][ld] (position p of row r at
+// r * ld + p), or sample-blocked by the slab size T (position p of row r in slab p / T, plane
+// (p % T) / P, at ((p / T) * T * R + ((p % T) / P) * R * P) + r * P + p % P with R = slab_rows(rows),
+// so that one slab is one contiguous block of planes).  Buffers hold slab_rows(N) * ld values.
+constexpr int kLayoutRows = 0;
+constexpr int kLayoutSlab = 1;
 
 // Activation buffers are compacted layer by layer: the buffer a layer >= 1 reads holds
 // counts[layer] rows of samples at leading dimension act_ld(counts[layer]), the row count
@@ -82,7 +132,9 @@ struct StepArgs {
   float band = 0.0f;
   int32_t* work = nullptr;           // work-item counter of the MFMA kernel (zeroed by the scan)
   int order = 1;                     // MFMA kernel item order (0..4); row-gather kernel work split
-  int x_rows = 0;                    // LDS-resident MFMA kernel: rows of the input buffer (<= 1024)
+  int x_rows = 0;                    // LDS-resident MFMA and sample-slab kernels: rows of the input buffer
+  int x_layout = kLayoutRows;        // sample-slab kernel: layout of x (layers >= 1; layer 0 reads
+  int y_layout = kLayoutRows;        // the input store or its bit-packed copy) and of y
 };
 
 struct ScanArgs {
@@ -159,6 +211,9 @@ hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, 
 // Band-streaming MFMA kernel (band image present, a.x_rows input rows; w.bmask used when set,
 // w.bwt otherwise); one work-group of 512 threads per CU, any grid of at least 8 work-groups.
 hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 1024 threads,
+// one work-group per CU); any grid.
+hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,

@@ -40,12 +40,12 @@ const cospdnn_layer_plan kCandidates[] = {
     {COSPDNN_PLAN_MFMA, 3, 4},         {COSPDNN_PLAN_MFMA, 4, 4},         {COSPDNN_PLAN_MFMA_COMPACT, 1, 4},
     {COSPDNN_PLAN_MFMA_COMPACT, 2, 2}, {COSPDNN_PLAN_MFMA_COMPACT, 2, 4}, {COSPDNN_PLAN_MFMA_COMPACT, 3, 4},
     {COSPDNN_PLAN_MFMA_COMPACT, 4, 4}, {COSPDNN_PLAN_MFMA, 5, 1},         {COSPDNN_PLAN_ROWS, 0, 5},
-    {COSPDNN_PLAN_ROWS, 1, 5},         {COSPDNN_PLAN_MFMA, 6, 1},
+    {COSPDNN_PLAN_ROWS, 1, 5},         {COSPDNN_PLAN_MFMA, 6, 1},         {COSPDNN_PLAN_SLAB, 0, 1},
 };
 constexpr int kNumCandidates = static_cast<int>(sizeof(kCandidates) / sizeof(kCandidates[0]));
 
 int check_plan(const cospdnn_layer_plan& p) {
-  if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_ROWS) return fail(COSPDNN_ERR_ARG, "plan: kernel");
+  if (p.kernel < COSPDNN_PLAN_AUTO || p.kernel > COSPDNN_PLAN_SLAB) return fail(COSPDNN_ERR_ARG, "plan: kernel");
   const bool mfma = p.kernel == COSPDNN_PLAN_MFMA || p.kernel == COSPDNN_PLAN_MFMA_COMPACT;
   if (mfma && (p.order < 0 || p.order > 6)) return fail(COSPDNN_ERR_ARG, "plan: order");
   if (p.kernel == COSPDNN_PLAN_ROWS && (p.order < 0 || p.order > 1)) return fail(COSPDNN_ERR_ARG, "plan: rows split");
@@ -72,7 +72,10 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[5] = ls.view.one_chunk ? 1.0f : 0.0f;
   f[6] = mfma ? 1.0f : 0.0f;
   const bool rows_k = c.kernel == COSPDNN_PLAN_ROWS;
-  f[7] = mfma || rows_k ? 0.0f : 1.0f;
+  // the sample-slab kernel where the layer has a slab image (it falls back to SIMT otherwise)
+  const int slab_t = slab_samples(l == 0 ? net->N : net->dl[l - 1]->view.n_out);
+  const bool slab = c.kernel == COSPDNN_PLAN_SLAB && ls.view.sid != nullptr && slab_t > 0;
+  f[7] = mfma || rows_k || slab ? 0.0f : 1.0f;
   f[8] = mfma ? static_cast<float>(c.order) / 5.0f : 0.0f;
   f[9] = mfma ? static_cast<float>(c.groups_per_cu) / 4.0f : 0.0f;
   f[10] = static_cast<float>(std::log2(rows * static_cast<double>(ls.nnz) + 1.0) / 48.0);
@@ -84,6 +87,8 @@ void layer_features(const cospdnn_net* net, int l, int64_t live, int64_t rows_pr
   f[12] = mfma && c.kernel == COSPDNN_PLAN_MFMA && !lds_resident ? 1.0f : 0.0f;
   // the row-gather kernel and its work split (0: all groups at once, 1: groups per XCD)
   f[13] = rows_k ? 0.5f * static_cast<float>(1 + c.order) : 0.0f;
+  // the sample-slab kernel and its slab size (samples per work-group: 64 .. 1)
+  f[14] = slab ? static_cast<float>(1.0 + std::log2(static_cast<double>(slab_t))) / 7.0f : 0.0f;
 }
 
 int mlp_reserve(cospdnn_mlp* m, int n) {

@@ -56,7 +56,10 @@ enum {
   COSPDNN_KERNEL_MFMA_BAND = 6,       /* FP16 MFMA streaming 512-row bands of the input tile (order 6) */
   COSPDNN_KERNEL_MFMA_BAND_MASK = 7,
   COSPDNN_KERNEL_WAVE_ROWS = 8,       /* row-gather kernel (COSPDNN_PLAN_ROWS) */
-  COSPDNN_KERNEL_MFMA32 = 9           /* FP32 MFMA over the FP32 tile image */
+  COSPDNN_KERNEL_MFMA32 = 9,          /* FP32 MFMA over the FP32 tile image */
+  COSPDNN_KERNEL_SLAB = 10            /* FP16 sample-slab kernel: T whole input samples in LDS, every
+                                         output of them by FP32 FMAs over the slab image
+                                         (spmm_slab[_uni]_kernel_<T>) */
 };
 
 typedef struct cospdnn_engine cospdnn_engine;
@@ -127,6 +130,14 @@ typedef struct {
                             the layers past it keep their images in pinned host memory and the
                             FP16 pass streams them into two device slots, one layer ahead of
                             its SpMM, on a copy stream (eager and in captured graphs) */
+  int spmm_slab;         /* FP16: 1 = every layer with a slab image (FP16-exact weights, at most
+                            65536 input rows) runs the sample-slab kernel: a work-group keeps T
+                            whole input samples in LDS (T*rows*2 bytes <= 128 KB: T = 64 up to
+                            1023 rows, 16 at 4096, 4 at 16384, 1 at 65536) and computes all of
+                            their outputs with one FP32 FMA per nonzero and sample, no tile
+                            waste; consecutive slab layers pass activations in a sample-blocked
+                            layout.  0 (default) = the MFMA / SIMT choice above; a plan can pick
+                            it per layer (COSPDNN_PLAN_SLAB) */
 } cospdnn_opts;
 
 typedef struct {
@@ -250,13 +261,16 @@ int cospdnn_ref_bcsc_to_csc(int N, const float* values, int64_t n_values, const 
    positions and the surviving samples are stored compacted, see spmm_gather).
    COSPDNN_PLAN_ROWS is the row-gather kernel (a wave per 64-sample group and output slots,
    CSC lists, no tile image); its order is the work split: 0 all sample groups at once
-   (few live rows), 1 sample groups partitioned over the XCDs and streamed (L2 reuse). */
+   (few live rows), 1 sample groups partitioned over the XCDs and streamed (L2 reuse).
+   COSPDNN_PLAN_SLAB is the sample-slab kernel (see spmm_slab; order and groups unused); it
+   falls back to SIMT on a layer without a slab image. */
 enum {
   COSPDNN_PLAN_AUTO = 0,
   COSPDNN_PLAN_MFMA = 1,
   COSPDNN_PLAN_SIMT = 2,
   COSPDNN_PLAN_MFMA_COMPACT = 3,
-  COSPDNN_PLAN_ROWS = 4
+  COSPDNN_PLAN_ROWS = 4,
+  COSPDNN_PLAN_SLAB = 5
 };
 typedef struct {
   int kernel;        /* COSPDNN_PLAN_*; MFMA falls back to SIMT on a layer without a tile image */
@@ -270,7 +284,7 @@ int cospdnn_num_candidates(void);
 int cospdnn_candidate(int k, cospdnn_layer_plan* out);
 /* feature vector of (layer, candidate, rows entering the layer, rows entering the layer
    before it; pass live_rows again for layer 0) for the cost model */
-#define COSPDNN_TUNE_FEATURES 14
+#define COSPDNN_TUNE_FEATURES 15
 int cospdnn_net_layer_features(cospdnn_net* net, int layer, int64_t live_rows, int64_t rows_prev,
                                const cospdnn_layer_plan* cand, float* feat);
 /* Exhaustive tuning: runs the FP16 forward once per candidate applied to every layer

@@ -98,7 +98,7 @@ def test_plans_keep_categories_exact(engine):
     with pytest.raises(cd.CospdnnError):
         net.set_plan([(cd.PLAN_MFMA, 7, 4)] * L)  # bad order
     with pytest.raises(cd.CospdnnError):
-        net.set_plan([(5, 1, 4)] * L)  # bad kernel
+        net.set_plan([(6, 1, 4)] * L)  # bad kernel
     net.set_plan(None)
     net.close()
 
