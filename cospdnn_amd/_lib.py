@@ -441,7 +441,7 @@ def layer_tiles_decode(N, colptr, rows, vals):
     return ins, outs, v
 
 
-IMAGE_TILES, IMAGE_MASK, IMAGE_BAND, IMAGE_BAND_MASK = 0, 1, 2, 3
+IMAGE_TILES, IMAGE_MASK, IMAGE_BAND, IMAGE_BAND_MASK, IMAGE_SLAB = 0, 1, 2, 3, 4
 
 
 def layer_image_decode(N, colptr, rows, vals, image):

@@ -1980,10 +1980,38 @@ int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows
 int cospdnn_layer_image_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int image,
                                int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out) {
   CscLayer c;
-  if (!csc_from_ptrs(N, colptr, rows, vals, c) || !n_out || image < 0 || image > 3)
+  if (!csc_from_ptrs(N, colptr, rows, vals, c) || !n_out || image < 0 || image > 4)
     return fail(COSPDNN_ERR_ARG, "bad argument");
   BlockedLayer bl;
   if (!build_layer(c, N, 1e30, bl)) return fail(COSPDNN_ERR_FORMAT, "malformed CSC");
+  if (image == 4) {
+    // the slab image as the sample-slab kernel walks it: batch b, octet o, lane oi's slot m at
+    // lane position oi * M + m, entry e; pads (the zero row, or value 0) and the slots past the
+    // layer's outputs skipped
+    build_slab_image(bl, N);
+    if (!bl.has_slab) return fail(COSPDNN_ERR_STATE, "the layer has no such image");
+    const int T = slab_samples(N), bs = slab_batch_slots(T), mper = slab_lane_slots(T), opw = bs / mper;
+    const int zrow = slab_rows(N);
+    int64_t k = 0;
+    for (size_t b = 0; b + 1 < bl.sgoff.size(); ++b)
+      for (int o = bl.sgoff[b]; o < bl.sgoff[b + 1]; ++o)
+        for (int lp = 0; lp < bs; ++lp)
+          for (int e = 0; e < 8; ++e) {
+            const size_t at = (static_cast<size_t>(o) * bs + lp) * 8 + e;
+            const int slot = static_cast<int>(b) * bs + (lp % mper) * opw + lp / mper;
+            const int id = bl.sid[at];
+            const float v = bl.slab_uniform ? bl.slab_value : half_bits_to_float(bl.sval[at]);
+            if (slot >= N || id == zrow || (!bl.slab_uniform && v == 0.0f)) continue;
+            if (k < cap && in_ids && out_ids && values) {
+              in_ids[k] = id;
+              out_ids[k] = bl.out_idx[slot];
+              values[k] = v;
+            }
+            ++k;
+          }
+    *n_out = k;
+    return COSPDNN_OK;
+  }
   const bool band = image >= 2, mask = image == 1 || image == 3;
   if (band) build_band_image(bl, N);
   if ((band && !bl.has_band) || (mask && !bl.has_mask))

@@ -233,7 +233,8 @@ int cospdnn_layer_stats(int N, const int32_t* colptr, const int32_t* rows, const
 int cospdnn_layer_tiles_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals,
                                int64_t cap, int32_t* in_ids, int32_t* out_ids, float* values, int64_t* n_out);
 /* the same for the layer's other weight images: image 0 the FP16 tile image, 1 its mask image
-   (expanded with the kernels' arithmetic), 2 the band image, 3 the band image's mask words;
+   (expanded with the kernels' arithmetic), 2 the band image, 3 the band image's mask words,
+   4 the slab image (16-bit rows in the sample-slab kernel's batch order, pads skipped);
    COSPDNN_ERR_STATE when the layer has no such image (mask: not one FP16 value with its low
    7 bits zero; band: built at upload for every layer, here on request) */
 int cospdnn_layer_image_decode(int N, const int32_t* colptr, const int32_t* rows, const float* vals, int image,

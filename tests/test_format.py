@@ -75,6 +75,29 @@ def test_mask_and_band_images_decode_to_the_csc(case):
         assert decoded_image(N, cp, rw, vl, image) == want, image
 
 
+@pytest.mark.parametrize("case", ["random1024", "random4096", "random16384", "banded4096", "mixed80", "mixed3000",
+                                  "uneven_one_value"])
+def test_slab_image_decodes_to_the_csc(case):
+    """The slab image (16-bit rows in batch order, one value or FP16 values, lists padded with
+    the zero row) decodes to the layer's nonzeros, each exactly once, for slab sizes 64 (N <= 1023
+    rows ... 1024), 16 (N = 3000, 4096) and 4 (N = 16384); uneven lists of one value pad with
+    the zero row and still need no value array."""
+    rng = np.random.default_rng(11)
+    if case.startswith("random"):
+        N = int(case[6:])
+        cp, rw, vl = cd.generate_layer(6, cd.GEN_RANDOM, N, 32, 1)
+    elif case == "banded4096":
+        N, (cp, rw, vl) = 4096, cd.generate_layer(6, cd.GEN_BANDED, 4096, 32, 1)
+    elif case == "uneven_one_value":
+        N = 1500
+        cp, rw, _ = mixed_layer(rng, N, 40)
+        vl = np.full(len(rw), 0.0625, np.float32)
+    else:
+        N = int(case[5:])
+        cp, rw, vl = mixed_layer(rng, N, 40)
+    assert decoded_image(N, cp, rw, vl, cd.IMAGE_SLAB) == triples(cp, rw, vl)
+
+
 @pytest.mark.parametrize("val", [0.0625, 0.046875, -0.75, 1.0, 2.0 ** -14])
 def test_mask_expansion_is_exact_for_seven_trailing_zero_bits(val):
     """Values whose FP16 bits end in 7 zeros get a mask image, and its expansion
