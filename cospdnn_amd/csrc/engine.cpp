@@ -523,8 +523,9 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         a.x_rows = input_rows(net, l);
         a.x_layout = l >= 1 && pick_kernel(net, cfg, l - 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
         a.y_layout = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
-        const int64_t slabs = (S_run + slab_samples(a.x_rows) - 1) / slab_samples(a.x_rows);
-        HIP_TRY(launch_spmm_slab(a, view, capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(slabs, e->cu_count)))), st));
+        // one work-group per CU; with fewer slabs than CUs the kernel splits each slab's outputs
+        // over several work-groups (it knows the live row count, the host only its bound)
+        HIP_TRY(launch_spmm_slab(a, view, capg(e->cu_count), st));
         break;
       }
       case COSPDNN_KERNEL_MFMA_BAND:

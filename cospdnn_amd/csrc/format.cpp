@@ -1,6 +1,7 @@
 #include "format.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -372,19 +373,71 @@ void build_slab_image(BlockedLayer& bl, int in_rows) {
   const size_t entries = static_cast<size_t>(bl.sgoff[nb]) * bs * 8;
   bl.sid.assign(entries, pad_id);
   if (!uniform) bl.sval.assign(entries, 0);  // FP16 zero
-  for (int b = 0; b < nb; ++b)
-    for (int i = 0; i < bs; ++i) {
-      // batch slot i = m * opw + oi: lane oi's m-th slot, stored at lane position oi * m_per + m
-      const int m = i / opw, oi = i % opw;
-      const int slot = slot_of(b * bs + i);
-      const int32_t q0 = bl.optr[slot], q1 = bl.optr[slot + 1];
-      for (int32_t q = q0; q < q1; ++q) {
-        const int k = q - q0;
-        const size_t at = (static_cast<size_t>(bl.sgoff[b] + k / 8) * bs + oi * m_per + m) * 8 + k % 8;
-        bl.sid[at] = static_cast<uint16_t>(bl.orow[q]);
-        if (!uniform) bl.sval[at] = float_to_half_bits(bl.oval[q]);
+  // Entry order within a slot is free (it only changes the FP32 summation order).  With one lane
+  // per slot (T <= 16) every 8-byte LDS read of a step takes one row per lane, at byte
+  // row * 8 (+ a plane offset common to the wave), and a half-wave of 32 lanes is served
+  // without bank conflicts when its rows differ mod 32.  So within each half-wave of a batch the
+  // slots' entries are dealt to the step positions by a matching of slots to residues mod 32 at
+  // every position (Kuhn's augmenting paths), leftovers taking any free entry.
+  const bool spread = T <= 16;
+  struct Ent {
+    uint16_t id, val;
+  };
+  std::vector<std::vector<Ent>> lst(32);
+  std::vector<std::vector<char>> used(32);
+  std::vector<int> owner(32), pick(32);
+  std::vector<char> seen(32);
+  std::function<bool(int)> augment = [&](int i) {
+    for (size_t e = 0; e < lst[i].size(); ++e) {
+      if (used[i][e]) continue;
+      const int r = lst[i][e].id & 31;
+      if (seen[r]) continue;
+      seen[r] = 1;
+      if (owner[r] < 0 || augment(owner[r])) {
+        owner[r] = i;
+        pick[i] = static_cast<int>(e);
+        return true;
       }
     }
+    return false;
+  };
+  for (int b = 0; b < nb; ++b) {
+    const int oct = bl.sgoff[b + 1] - bl.sgoff[b];
+    const int npos = 8 * oct;
+    for (int m = 0; m < m_per; ++m)
+      for (int h = 0; h < opw; h += 32) {
+        const int lanes = std::min(32, opw - h);
+        // the half-wave's slots: lane oi = h + j, batch slot i = m * opw + oi
+        for (int j = 0; j < lanes; ++j) {
+          const int slot = slot_of(b * bs + m * opw + h + j);
+          lst[j].clear();
+          for (int32_t q = bl.optr[slot]; q < bl.optr[slot + 1]; ++q)
+            lst[j].push_back({static_cast<uint16_t>(bl.orow[q]), uniform ? uint16_t(0) : float_to_half_bits(bl.oval[q])});
+          while (static_cast<int>(lst[j].size()) < npos) lst[j].push_back({pad_id, 0});
+          used[j].assign(lst[j].size(), 0);
+        }
+        for (int p = 0; p < npos; ++p) {
+          std::fill(owner.begin(), owner.end(), -1);
+          std::fill(pick.begin(), pick.end(), -1);
+          if (spread)
+            for (int j = 0; j < lanes; ++j) {
+              std::fill(seen.begin(), seen.end(), 0);
+              augment(j);
+            }
+          for (int j = 0; j < lanes; ++j) {
+            int e = pick[j];
+            if (e < 0)
+              for (e = 0; used[j][e]; ++e) {
+              }
+            used[j][e] = 1;
+            const int oi = h + j;
+            const size_t at = (static_cast<size_t>(bl.sgoff[b] + p / 8) * bs + oi * m_per + m) * 8 + p % 8;
+            bl.sid[at] = lst[j][e].id;
+            if (!uniform) bl.sval[at] = lst[j][e].val;
+          }
+        }
+      }
+  }
   bl.slab_uniform = uniform;
   bl.slab_value = uniform ? v0 : 0.0f;
   bl.has_slab = true;

@@ -1487,10 +1487,12 @@ __device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* 
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
   const int R = slab_rows(x_rows);
   lds_half_t* dst = lds + (t / P) * PH + t % P;
-  for (int r0 = tid / T; r0 < x_rows; r0 += 8 * kRowsPer) {
-    uint16_t v[8];
+  // 32 loads in flight per thread: the gather is latency-bound (2-byte loads)
+  constexpr int kU = 32;
+  for (int r0 = tid / T; r0 < x_rows; r0 += kU * kRowsPer) {
+    uint16_t v[kU];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kU; ++u) {
       const int r = r0 + u * kRowsPer;
       v[u] = 0;
       if (p >= 0 && r < x_rows) {
@@ -1505,7 +1507,7 @@ __device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* 
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < kU; ++u) {
       const int r = r0 + u * kRowsPer;
       if (r < x_rows) lds_put2(dst + r * P, v[u]);
     }
@@ -1549,13 +1551,13 @@ __device__ __forceinline__ void slab_cur_batch(SlabCur& c, int b, int nout, int 
   }
 }
 template <int T>
-__device__ __forceinline__ void slab_cur_next(SlabCur& c, int nout, int oi, cint_ptr_t sgoff) {
+__device__ __forceinline__ void slab_cur_next(SlabCur& c, int nout, int oi, cint_ptr_t sgoff, int bstride) {
   if (c.b * SlabShape<T>::kBatch >= nout) return;
   if (c.o + 1 < c.nk) {
     ++c.o;
     return;
   }
-  slab_cur_batch<T>(c, c.b + kSlabWaves, nout, oi, sgoff);
+  slab_cur_batch<T>(c, c.b + bstride, nout, oi, sgoff);
 }
 // One octet of a lane's M slots: 8 row ids (and values given, 8 FP16 values) of each.
 template <int M>
@@ -1593,8 +1595,14 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
   const int x_rows = a.x_rows;
   const int nout = w.n_out;
   const int ry = slab_rows(nout);            // rows per plane of a sample-blocked output slab
-  // slabs are output positions; without compaction they are the input positions too
+  // slabs are output positions; without compaction they are the input positions too.  A work
+  // item is (slab, part): with fewer slabs than work-groups, `parts` work-groups share a slab,
+  // each loading it and computing every parts-th batch of the wave-batches (flag bytes are only
+  // ever set to 1, so work-groups sharing a sample write them without atomics)
   const int nsl = (n_out + T - 1) / T;
+  const int nbw = (nout + Sh::kBatch * kSlabWaves - 1) / (Sh::kBatch * kSlabWaves);  // batches per wave
+  const int parts = max(1, min((2 * static_cast<int>(gridDim.x) + nsl - 1) / max(nsl, 1), nbw));
+  const int nitems = nsl * parts;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int oi = lane / LPO, g = lane % LPO;
@@ -1634,16 +1642,19 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
       psrc[tid] = q < n_out ? (ident ? q : a.qsrc[q]) : -1;
     }
   };
-  int sl = blockIdx.x;
-  set_src(sl);
+  int item = blockIdx.x;
+  set_src(item / parts);
   __syncthreads();  // psrc of the first slab
-  if (sl < nsl) load_slab(sl);
-  for (int it = 0; sl < nsl; ++it) {
+  if (item < nitems) load_slab(item / parts);
+  for (int it = 0; item < nitems; ++it) {
     const int par = it & 1;
+    const int sl = item / parts, part = item % parts;
     if (tid < 6) sflag[par][tid >> 1][tid & 1] = 0u;
     __syncthreads();  // the slab and cleared flag words are in place
-    const int next = sl + static_cast<int>(gridDim.x);
+    const int next = item + static_cast<int>(gridDim.x);
     const int q0 = sl * T;
+    // this part's batches of the wave: b0, b0 + bstride, ..
+    const int b0 = part * kSlabWaves + wave, bstride = kSlabWaves * parts;
     // per lane: packed FP16 max (alive: > 0) and min (unsaturated: < 32) of its samples over
     // its slots, reduced to flag bits once per slab
     uint32_t fmx[ND], fmn[ND];
@@ -1661,13 +1672,13 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
     // make the next waits drain every load (the compiler's counter model), once per batch.
     constexpr int kSlabAhead = 3;
     SlabCur cur, ahead;
-    slab_cur_batch<T>(cur, wave, nout, oi, sgoff);
+    slab_cur_batch<T>(cur, b0, nout, oi, sgoff);
     ahead = cur;
     SlabOct<M> ring[kSlabAhead];
 #pragma unroll
     for (int u = 0; u < kSlabAhead; ++u) {
       ring[u] = slab_oct_load<T, UNI>(w, ahead);
-      slab_cur_next<T>(ahead, nout, oi, sgoff);
+      slab_cur_next<T>(ahead, nout, oi, sgoff, bstride);
     }
     float acc[M][SPL];
 #pragma unroll
@@ -1745,13 +1756,13 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       }
       slot = slab_oct_load<T, UNI>(w, ahead);
-      slab_cur_next<T>(ahead, nout, oi, sgoff);
-      slab_cur_next<T>(cur, nout, oi, sgoff);
+      slab_cur_next<T>(ahead, nout, oi, sgoff, bstride);
+      slab_cur_next<T>(cur, nout, oi, sgoff, bstride);
     };
     // the wave's step count (octets over its batches), so that the main loop runs whole rounds
     // of the ring with no exit test between steps (the compiler's waits then count exactly)
     int nsteps = 0;
-    for (int b = wave; b * Sh::kBatch < nout; b += kSlabWaves) nsteps += sgoff[b + 1] - sgoff[b];
+    for (int b = b0; b * Sh::kBatch < nout; b += bstride) nsteps += sgoff[b + 1] - sgoff[b];
     int k = 0;
     for (; k + kSlabAhead <= nsteps; k += kSlabAhead) {
 #pragma unroll
@@ -1796,14 +1807,14 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
         if (want_band && ((sflag[par][2][tid >> 5] >> (tid & 31)) & 1u)) a.band_next[q] = 1;
       }
     }
-    if (next < nsl) {
+    if (next < nitems) {
       if (!ident) {
-        set_src(next);
+        set_src(next / parts);
         __syncthreads();  // psrc of the next slab (gathered loads read it)
       }
-      load_slab(next);
+      load_slab(next / parts);
     }
-    sl = next;
+    item = next;
   }
 }
 
