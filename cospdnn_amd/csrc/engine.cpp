@@ -64,6 +64,7 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
       rb.act32[s].reset();
       rb.act64[s].reset();
     }
+    rb.act16x.reset();
     rb.in16.reset();
     rb.cap = cap;
     rb.L = L;
@@ -92,6 +93,12 @@ int ensure_bufs(RunBufs& rb, int64_t cap, int L, int N, int precision, bool need
         CALL_TRY(dalloc(rb.act16[s], elems * 2));
         HIP_TRY(hipMemsetAsync(rb.act16[s].p, 0, rb.act16[s].bytes, st));
       }
+    // a third buffer for the layout conversions next to slab layers of few samples per slab
+    const int t = slab_samples(N);
+    if (t > 0 && t <= 4 && !rb.act16x.p) {
+      CALL_TRY(dalloc(rb.act16x, elems * 2));
+      HIP_TRY(hipMemsetAsync(rb.act16x.p, 0, rb.act16x.bytes, st));
+    }
   } else if (precision == COSPDNN_FP32) {
     for (int s = 0; s < 2; ++s)
       if (!rb.act32[s].p) {
@@ -521,11 +528,51 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         // consecutive slab layers pass the activations sample-blocked by the slab size; the
         // buffers any other kernel reads or writes (and the last layer's output) stay neuron-major
         a.x_rows = input_rows(net, l);
-        a.x_layout = l >= 1 && pick_kernel(net, cfg, l - 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
-        a.y_layout = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
+        const bool prev_slab = l >= 1 && pick_kernel(net, cfg, l - 1, S_run) == COSPDNN_KERNEL_SLAB;
+        const bool next_slab = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB;
+        a.x_layout = prev_slab ? kLayoutSlab : kLayoutRows;
+        a.y_layout = next_slab ? kLayoutSlab : kLayoutRows;
+        // With 4 samples per slab or fewer the kernel would read (write) a few bytes per row and
+        // slab of a neuron-major buffer: the layout is converted beside it instead, through the
+        // third buffer (the input then, when both convert, the output goes to the input's own
+        // buffer, free once converted).
+        const int t = slab_samples(a.x_rows);
+        const bool conv_in = fp16 && l >= 1 && !prev_slab && t <= 4 && rb.act16x.p;
+        const bool conv_out = fp16 && !next_slab && t <= 4 && rb.act16x.p;
+        const int64_t tiles = ((S_run + 63) / 64) * ((std::max(a.x_rows, view.n_out) + 63) / 64);
+        const int cgrid = capg(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tiles, 4 * e->cu_count))));
+        void* final_y = a.y;
+        if (conv_in) {
+          ConvArgs c;
+          c.src = static_cast<const _Float16*>(a.x);
+          c.dst = rb.act16x.as<_Float16>();
+          c.rows = a.x_rows;
+          c.counts = rb.counts.as<int32_t>();
+          c.count_idx = l;
+          c.t = t;
+          c.to_slab = 1;
+          HIP_TRY(launch_convert_layout(c, cgrid, st));
+          a.x = rb.act16x.p;
+          a.x_layout = kLayoutSlab;
+        }
+        if (conv_out) {
+          a.y = conv_in ? side_ptr(cur) : rb.act16x.p;
+          a.y_layout = kLayoutSlab;
+        }
         // one work-group per CU; with fewer slabs than CUs the kernel splits each slab's outputs
         // over several work-groups (it knows the live row count, the host only its bound)
         HIP_TRY(launch_spmm_slab(a, view, capg(e->cu_count), st));
+        if (conv_out) {
+          ConvArgs c;
+          c.src = static_cast<const _Float16*>(a.y);
+          c.dst = static_cast<_Float16*>(final_y);
+          c.rows = view.n_out;
+          c.counts = rb.counts.as<int32_t>();
+          c.count_idx = l + 1;
+          c.t = t;
+          c.to_slab = 0;
+          HIP_TRY(launch_convert_layout(c, cgrid, st));
+        }
         break;
       }
       case COSPDNN_KERNEL_MFMA_BAND:

@@ -102,6 +102,7 @@ struct RunBufs {
   int L = 0;
   int64_t gen = 0;  // bumped whenever the buffers are reallocated (captured graphs hold their addresses)
   DevBuf act16[2], act32[2], act64[2];
+  DevBuf act16x;  // FP16 networks of at most 4 samples per slab: layout conversions next to slab layers
   DevBuf flags;  // 6 arrays of ld bytes: side s, kind k at (s*3+k)*ld
   DevBuf ids;    // 2 x ld int32
   DevBuf qmap;   // ld int32: output position of each input position (-1: removed)

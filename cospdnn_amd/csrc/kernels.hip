@@ -1844,6 +1844,84 @@ SLAB_KERNEL(1)
 #undef SLAB_KERNEL
 
 // --------------------------------------------------------------------------
+// Layout conversion between the neuron-major and the sample-blocked activation layouts, for a
+// sample-slab layer next to a layer of another kernel: with few samples per slab (T <= 4) the
+// slab kernel would read (or write) 2 to 8 bytes per row and slab in the neuron-major layout,
+// one scattered access each; here a 64-row x 64-position tile moves through LDS, read and
+// written as whole runs (128-byte row segments, and 64-row runs of P-sample pieces).
+// --------------------------------------------------------------------------
+constexpr int kConvLd = 64 + 8;  // LDS tile row stride (halves)
+
+template <int T>
+__device__ __forceinline__ void convert_tile(const ConvArgs& c, int n, int64_t ld, int r0, int p0,
+                                             _Float16 (*tile)[kConvLd]) {
+  constexpr int P = T < 4 ? T : 4;
+  const int R = slab_rows(c.rows);
+  const int tid = threadIdx.x;
+  const _Float16* __restrict__ src = c.src;
+  _Float16* __restrict__ dst = c.dst;
+  if (c.to_slab) {
+    // neuron-major rows (16-byte pieces) -> tile
+    for (int i = tid; i < 64 * 8; i += 256) {
+      const int r = i >> 3, j = i & 7;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (r0 + r < c.rows) v = *reinterpret_cast<const uint4*>(src + static_cast<int64_t>(r0 + r) * ld + p0 + 8 * j);
+      *reinterpret_cast<uint4*>(&tile[r][8 * j]) = v;
+    }
+    __syncthreads();
+    // tile -> P-sample pieces of the blocked layout, consecutive threads on consecutive rows
+    for (int i = tid; i < 64 * (64 / P); i += 256) {
+      const int r = i & 63, j = i >> 6;
+      if (r0 + r >= c.rows) continue;
+      const int p = p0 + j * P;
+      _Float16* d = dst + static_cast<int64_t>(p / T) * T * R + static_cast<int64_t>((p % T) / P) * R * P +
+                    static_cast<int64_t>(r0 + r) * P;
+      if constexpr (P == 4) {
+        *reinterpret_cast<uint2*>(d) = *reinterpret_cast<const uint2*>(&tile[r][j * P]);
+      } else if constexpr (P == 2) {
+        *reinterpret_cast<uint32_t*>(d) = *reinterpret_cast<const uint32_t*>(&tile[r][j * P]);
+      } else {
+        *d = tile[r][j];
+      }
+    }
+  } else {
+    for (int i = tid; i < 64 * (64 / P); i += 256) {
+      const int r = i & 63, j = i >> 6;
+      const int p = p0 + j * P;
+      const _Float16* q = src + static_cast<int64_t>(p / T) * T * R + static_cast<int64_t>((p % T) / P) * R * P +
+                          static_cast<int64_t>(r0 + r) * P;
+      if (r0 + r >= c.rows) continue;
+      if constexpr (P == 4) {
+        *reinterpret_cast<uint2*>(&tile[r][j * P]) = *reinterpret_cast<const uint2*>(q);
+      } else if constexpr (P == 2) {
+        *reinterpret_cast<uint32_t*>(&tile[r][j * P]) = *reinterpret_cast<const uint32_t*>(q);
+      } else {
+        tile[r][j] = *q;
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < 64 * 8; i += 256) {
+      const int r = i >> 3, j = i & 7;
+      if (r0 + r < c.rows)
+        *reinterpret_cast<uint4*>(dst + static_cast<int64_t>(r0 + r) * ld + p0 + 8 * j) =
+            *reinterpret_cast<const uint4*>(&tile[r][8 * j]);
+    }
+  }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void convert_layout_kernel(ConvArgs c) {
+  __shared__ __attribute__((aligned(16))) _Float16 tile[64][kConvLd];
+  const int n = __builtin_amdgcn_readfirstlane(c.counts[c.count_idx]);
+  const int64_t ld = act_ld(n);
+  const int ptiles = (n + 63) / 64, rtiles = (c.rows + 63) / 64;
+  for (int t = blockIdx.x; t < ptiles * rtiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's reads of `tile` are done
+    convert_tile<T>(c, n, ld, (t % rtiles) * 64, (t / rtiles) * 64, tile);
+  }
+}
+
+// --------------------------------------------------------------------------
 // SIMT gather SpMM: thread = sample, work item = (tile of 256 samples, block of
 // 32 outputs); each output sums its CSC inputs read straight from global memory.
 // --------------------------------------------------------------------------
@@ -2329,6 +2407,15 @@ hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid,
     hipLaunchKernelGGL(spmm_mfma_band_mask_kernel, dim3(grid), dim3(kBandThreads), spmm_mfma_band_bytes(), st, a, w);
   else
     hipLaunchKernelGGL(spmm_mfma_band_kernel, dim3(grid), dim3(kBandThreads), spmm_mfma_band_bytes(), st, a, w);
+  return hipGetLastError();
+}
+hipError_t launch_convert_layout(const ConvArgs& c, int grid, hipStream_t st) {
+  switch (c.t) {
+    case 4: hipLaunchKernelGGL(convert_layout_kernel<4>, dim3(grid), dim3(256), 0, st, c); break;
+    case 2: hipLaunchKernelGGL(convert_layout_kernel<2>, dim3(grid), dim3(256), 0, st, c); break;
+    case 1: hipLaunchKernelGGL(convert_layout_kernel<1>, dim3(grid), dim3(256), 0, st, c); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 int spmm_slab_bytes(int t) {

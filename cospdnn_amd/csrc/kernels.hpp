@@ -214,6 +214,19 @@ hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid,
 // Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 1024 threads,
 // one work-group per CU); any grid.
 hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// Activations of `rows` rows at positions [0, counts[count_idx]) (leading dimension act_ld of that
+// count in the neuron-major layout) converted between the neuron-major layout and the
+// sample-blocked layout of slab size t (4, 2 or 1); to_slab: neuron-major -> blocked.
+struct ConvArgs {
+  const _Float16* src = nullptr;
+  _Float16* dst = nullptr;
+  int rows = 0;
+  const int32_t* counts = nullptr;
+  int count_idx = 0;
+  int t = 0;
+  int to_slab = 1;
+};
+hipError_t launch_convert_layout(const ConvArgs& c, int grid, hipStream_t st);
 hipError_t launch_spmm_mfma32(const StepArgs& a, const DevLayer& w, Dtype tin, int grid, hipStream_t st);
 hipError_t launch_input_alive(const _Float16* store, int64_t ld, int N, int64_t S, uint8_t* alive, hipStream_t st);
 hipError_t launch_gather_rows(const _Float16* src, int64_t lds, const int32_t* ids, const int32_t* count,

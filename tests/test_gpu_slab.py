@@ -79,12 +79,15 @@ def test_slab_generated(engine, case):
     net.close()
 
 
-def test_slab_layout_changes_and_plans(engine):
+@pytest.mark.parametrize("N,L,S,bias,seed", [(4096, 24, 5000, -0.35, 111), (16384, 12, 1500, -0.4, 112),
+                                          (32768, 8, 600, -0.45, 113), (65536, 7, 300, -0.45, 114)],
+                         ids=["T16", "T4", "T2", "T1"])
+def test_slab_layout_changes_and_plans(engine, N, L, S, bias, seed):
     """Slab layers next to MFMA layers (activations change between the sample-blocked and the
-    neuron-major layout at every switch), kept layouts and graph replay: categories equal to the
-    oracle's, the activations after layer 5 within 0.05 of FP64 (the plans switch layouts
-    before it)."""
-    N, L, S, bias, seed = 4096, 24, 5000, -0.35, 111
+    neuron-major layout at every switch, in the slab kernel itself at 16 samples per slab and
+    through the layout conversion kernel at 4 and fewer), kept layouts and graph replay:
+    categories equal to the oracle's, the activations after layer 5 within 0.05 of FP64 (the
+    plans switch layouts before it)."""
     d = 5
     alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, cd.GEN_RANDOM, seed), 0, S)
     want = np.flatnonzero(alive)
@@ -98,6 +101,7 @@ def test_slab_layout_changes_and_plans(engine):
     plans = {
         "all slab": [slab] * L,
         "alternating": [slab if l % 2 == 0 else mfma for l in range(L)],
+        "alternating from mfma": [slab if l % 2 == 1 else mfma for l in range(L)],
         "runs of three": [slab if (l // 3) % 2 else mfma for l in range(L)],
         "slab first and last": [slab if l in (0, L - 1) else mfma for l in range(L)],
     }
