@@ -374,12 +374,15 @@ void build_slab_image(BlockedLayer& bl, int in_rows) {
   bl.sid.assign(entries, pad_id);
   if (!uniform) bl.sval.assign(entries, 0);  // FP16 zero
   // Entry order within a slot is free (it only changes the FP32 summation order).  With one lane
-  // per slot (T <= 16) every 8-byte LDS read of a step takes one row per lane, at byte
-  // row * 8 (+ a plane offset common to the wave), and a half-wave of 32 lanes is served
-  // without bank conflicts when its rows differ mod 32.  So within each half-wave of a batch the
-  // slots' entries are dealt to the step positions by a matching of slots to residues mod 32 at
-  // every position (Kuhn's augmenting paths), leftovers taking any free entry.
+  // per slot (T <= 16) every LDS read of a step takes one row per lane, at byte row * 2P (+ a
+  // plane offset common to the wave; P = slab_plane(T) samples of 2 bytes), and a half-wave of 32
+  // lanes is served without bank conflicts when its rows fall in distinct 4-byte banks mod 32:
+  // rows distinct mod 32 for P = 4 and 2, rows / 2 distinct mod 32 for P = 1 (two rows share a
+  // bank word there).  So within each half-wave of a batch the slots' entries are dealt to the
+  // step positions by a matching of slots to these residues at every position (Kuhn's
+  // augmenting paths), leftovers taking any free entry.
   const bool spread = T <= 16;
+  const int rshift = slab_plane(T) == 1 ? 1 : 0;
   struct Ent {
     uint16_t id, val;
   };
@@ -390,7 +393,7 @@ void build_slab_image(BlockedLayer& bl, int in_rows) {
   std::function<bool(int)> augment = [&](int i) {
     for (size_t e = 0; e < lst[i].size(); ++e) {
       if (used[i][e]) continue;
-      const int r = lst[i][e].id & 31;
+      const int r = (lst[i][e].id >> rshift) & 31;
       if (seen[r]) continue;
       seen[r] = 1;
       if (owner[r] < 0 || augment(owner[r])) {

@@ -1300,14 +1300,14 @@ __global__ __launch_bounds__(kBandThreads, 1) void spmm_mfma_band_mask_kernel(St
 // and re-read every input row once per block column (or per band).  Here a work-group holds
 // a slab of T consecutive input positions in LDS, all input rows of them (T = slab_samples:
 // 64 up to 1023 rows, 16 at 4096, 4 at 16384, 1 at 65536), and computes every output of
-// those samples from it.  Each lane takes one output slot and min(T, 8) of the slab's
-// samples: it reads the slot's input rows from the slab image (16-bit row ids, 8 per 16-byte
-// load), gathers each row's samples from LDS in one read and accumulates them in FP32 with
-// v_fma_mix_f32, one instruction per sample and nonzero: the layer's MACs and no more.
-// Every input value moves from memory once, the flags of a sample are the OR over all of its
-// outputs (reduced in the work-group), and the epilogue fuses bias, clamp and FP16 store.
-// In the sample-blocked layout (kLayoutSlab) a slab is one contiguous block, loaded into
-// registers a slab ahead while the current one is computed, and a wave's stores cover
+// those samples from it.  Each lane takes slab_lane_slots(T) output slots and min(T, 16) of the
+// slab's samples: it reads the slots' input rows from the slab image (16-bit row ids, 8 per
+// 16-byte load), gathers each row's samples from LDS (one read per plane of up to 4 samples)
+// and accumulates them in FP32 with v_fma_mix_f32, one instruction per sample and nonzero: the
+// layer's MACs and no more.  Every input value moves from memory once per slab, the flags of a
+// sample are the OR over all of its outputs (reduced in the work-group), and the epilogue fuses
+// bias, clamp and FP16 store.  In the sample-blocked layout (kLayoutSlab) a slab is one
+// contiguous block, copied straight into LDS (global_load_lds), and a wave's stores cover
 // contiguous runs; the neuron-major layout and the bit-packed layer-0 input are read (and
 // neuron-major written) too, so the kernel can sit next to the MFMA kernels in a plan.
 // Stores go to the samples' compacted positions (qmap); a slab whose samples keep T
@@ -1363,6 +1363,39 @@ __device__ __forceinline__ void slab_read(const lds_half_t* base, uint32_t off, 
   } else {
     x[0] = *reinterpret_cast<const __attribute__((address_space(3))) _Float16*>(p);
   }
+}
+
+// LDS byte offset row * 2P of the 16-bit row id in half `hi` of dw (P = 1, 2 or 4 samples per
+// plane row): one SDWA shift, which takes the word and scales it in a single instruction (the
+// compiler's form is an extract, a shift and a mask)
+template <int P>
+__device__ __forceinline__ uint32_t slab_row_off(uint32_t dw, int hi) {
+  uint32_t o;
+  if (hi) {
+    if constexpr (P == 1)
+      asm("v_lshlrev_b32_sdwa %0, 1, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1" : "=v"(o) : "v"(dw));
+    else if constexpr (P == 2)
+      asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1" : "=v"(o) : "v"(dw));
+    else
+      asm("v_lshlrev_b32_sdwa %0, 3, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_1" : "=v"(o) : "v"(dw));
+  } else {
+    if constexpr (P == 1)
+      asm("v_lshlrev_b32_sdwa %0, 1, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0" : "=v"(o) : "v"(dw));
+    else if constexpr (P == 2)
+      asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0" : "=v"(o) : "v"(dw));
+    else
+      asm("v_lshlrev_b32_sdwa %0, 3, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0" : "=v"(o) : "v"(dw));
+  }
+  return o;
+}
+
+// P consecutive FP16 values of one plane row from LDS byte address addr.  The slab kernels hold no
+// kernel-static LDS (checked at set-up), so their dynamic LDS, the slab, starts at address 0 and
+// row offsets are addresses as they are (through the dynamic array's symbol every address would
+// take an add of its link-time value).
+template <int P>
+__device__ __forceinline__ void slab_read_at(uint32_t addr, _Float16* x) {
+  slab_read<P>((const lds_half_t*)(uintptr_t)0u, addr, x);
 }
 
 // P FP16 values (packed in dwords, low half first) to global memory at p
@@ -1531,33 +1564,33 @@ __device__ __forceinline__ uint64_t slab_wave_bits(uint32_t f) {
   return out;
 }
 
-// A wave's position in the slab image: batch b (b * B >= nout: past the end), octet o of its nk,
+// A wave's position in the slab image: batch b (b >= bend: past the end), octet o of its nk,
 // and the lane's entry offset of octet 0 (elements of sid / sval).
 struct SlabCur {
   int b, o, nk;
   int64_t e;
 };
 template <int T>
-__device__ __forceinline__ void slab_cur_batch(SlabCur& c, int b, int nout, int oi, cint_ptr_t sgoff) {
+__device__ __forceinline__ void slab_cur_batch(SlabCur& c, int b, int bend, int oi, cint_ptr_t sgoff) {
   constexpr int B = SlabShape<T>::kBatch, M = SlabShape<T>::kM;
   c.b = b;
   c.o = 0;
   c.nk = 0;
   c.e = 0;
-  if (b * B < nout) {
+  if (b < bend) {
     const int k0 = sgoff[b];
     c.nk = sgoff[b + 1] - k0;
     c.e = (static_cast<int64_t>(k0) * B + oi * M) * 8;
   }
 }
 template <int T>
-__device__ __forceinline__ void slab_cur_next(SlabCur& c, int nout, int oi, cint_ptr_t sgoff, int bstride) {
-  if (c.b * SlabShape<T>::kBatch >= nout) return;
+__device__ __forceinline__ void slab_cur_next(SlabCur& c, int bend, int oi, cint_ptr_t sgoff, int bstride) {
+  if (c.b >= bend) return;
   if (c.o + 1 < c.nk) {
     ++c.o;
     return;
   }
-  slab_cur_batch<T>(c, c.b + bstride, nout, oi, sgoff);
+  slab_cur_batch<T>(c, c.b + bstride, bend, oi, sgoff);
 }
 // One octet of a lane's M slots: 8 row ids (and values given, 8 FP16 values) of each.
 template <int M>
@@ -1587,22 +1620,30 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
   using Sh = SlabShape<T>;
   constexpr int P = Sh::kP, SPL = Sh::kSpl, LPO = Sh::kLpo, OPW = Sh::kOpw, NR = Sh::kReads, PB = Sh::kPb, M = Sh::kM;
   constexpr int ND = (SPL + 1) / 2;       // dwords of a lane's SPL FP16 outputs
-  __shared__ int psrc[T];                 // input position of each slab sample (-1: none)
-  __shared__ uint32_t sflag[2][3][2];     // per slab parity and flag kind: T bits
+  // bookkeeping behind the planes (kernel-static LDS would move the slab off address 0, and the
+  // row offsets would then need an add each): input position of each slab sample (-1: none),
+  // and per slab parity and flag kind the T flag bits
+  int* const psrc = reinterpret_cast<int*>((_Float16*)lds + kSlabMaxBytes / 2);
+  uint32_t(*const sflag)[3][2] = reinterpret_cast<uint32_t(*)[3][2]>(psrc + 64);
   const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
   const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
   const bool ident = n_out == n;
   const int x_rows = a.x_rows;
   const int nout = w.n_out;
   const int ry = slab_rows(nout);            // rows per plane of a sample-blocked output slab
-  // slabs are output positions; without compaction they are the input positions too.  A work
-  // item is (slab, part): with fewer slabs than work-groups, `parts` work-groups share a slab,
-  // each loading it and computing every parts-th batch of the wave-batches (flag bytes are only
-  // ever set to 1, so work-groups sharing a sample write them without atomics)
+  // Slabs are output positions; without compaction they are the input positions too.  The work
+  // is nsl * nb units (slab, batch of output slots) in slab-major order, split into one contiguous
+  // range per work-group, balanced to a unit: a work-group loads each slab of its range once,
+  // and its waves take the range's batches of that slab in turn (lo + wave, lo + wave + 8, ..).
+  // With fewer slabs than work-groups several work-groups share a slab; a slab at a range
+  // boundary is shared by two (flag bytes are only ever set to 1, so work-groups sharing a
+  // sample write them without atomics).
   const int nsl = (n_out + T - 1) / T;
-  const int nbw = (nout + Sh::kBatch * kSlabWaves - 1) / (Sh::kBatch * kSlabWaves);  // batches per wave
-  const int parts = max(1, min((2 * static_cast<int>(gridDim.x) + nsl - 1) / max(nsl, 1), nbw));
-  const int nitems = nsl * parts;
+  const int nb = (nout + Sh::kBatch - 1) / Sh::kBatch;
+  const int64_t units = static_cast<int64_t>(nsl) * nb;
+  const int64_t u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
+  const int sl_first = static_cast<int>(u0 / nb);
+  const int sl_last = u1 > u0 ? static_cast<int>((u1 - 1) / nb) : sl_first - 1;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int oi = lane / LPO, g = lane % LPO;
@@ -1631,6 +1672,10 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
         slab_load_blocked<T>(a, lds, sl, x_rows, wave, lane);
       else
         slab_load_rows<T>(a, lds, sl, x_rows, ldx, tid);
+    } else if (T == 1 && x_slab) {
+      // one sample per slab: the sample's rows are one contiguous block of the sample-blocked
+      // input (slab psrc[0] of it), loaded whole instead of value by value
+      slab_load_blocked<T>(a, lds, __builtin_amdgcn_readfirstlane(psrc[0]), x_rows, wave, lane);
     } else {
       slab_load_gather<T>(a, lds, psrc, x_rows, ldx, x_slab, tid);
     }
@@ -1642,19 +1687,19 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
       psrc[tid] = q < n_out ? (ident ? q : a.qsrc[q]) : -1;
     }
   };
-  int item = blockIdx.x;
-  set_src(item / parts);
+  set_src(sl_first);
   __syncthreads();  // psrc of the first slab
-  if (item < nitems) load_slab(item / parts);
-  for (int it = 0; item < nitems; ++it) {
+  if (sl_first <= sl_last) load_slab(sl_first);
+  for (int sl = sl_first, it = 0; sl <= sl_last; ++sl, ++it) {
     const int par = it & 1;
-    const int sl = item / parts, part = item % parts;
     if (tid < 6) sflag[par][tid >> 1][tid & 1] = 0u;
     __syncthreads();  // the slab and cleared flag words are in place
-    const int next = item + static_cast<int>(gridDim.x);
     const int q0 = sl * T;
-    // this part's batches of the wave: b0, b0 + bstride, ..
-    const int b0 = part * kSlabWaves + wave, bstride = kSlabWaves * parts;
+    // the range's batches of this slab, [blo, bhi); the wave's: b0, b0 + bstride, ..
+    const int64_t ub = static_cast<int64_t>(sl) * nb;
+    const int blo = static_cast<int>((u0 > ub ? u0 : ub) - ub);
+    const int bhi = static_cast<int>((u1 < ub + nb ? u1 : ub + nb) - ub);
+    const int b0 = blo + wave, bstride = kSlabWaves;
     // per lane: packed FP16 max (alive: > 0) and min (unsaturated: < 32) of its samples over
     // its slots, reduced to flag bits once per slab
     uint32_t fmx[ND], fmn[ND];
@@ -1672,13 +1717,13 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
     // make the next waits drain every load (the compiler's counter model), once per batch.
     constexpr int kSlabAhead = 3;
     SlabCur cur, ahead;
-    slab_cur_batch<T>(cur, b0, nout, oi, sgoff);
+    slab_cur_batch<T>(cur, b0, bhi, oi, sgoff);
     ahead = cur;
     SlabOct<M> ring[kSlabAhead];
 #pragma unroll
     for (int u = 0; u < kSlabAhead; ++u) {
       ring[u] = slab_oct_load<T, UNI>(w, ahead);
-      slab_cur_next<T>(ahead, nout, oi, sgoff, bstride);
+      slab_cur_next<T>(ahead, bhi, oi, sgoff, bstride);
     }
     float acc[M][SPL];
 #pragma unroll
@@ -1686,31 +1731,31 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
 #pragma unroll
       for (int t = 0; t < SPL; ++t) acc[m][t] = 0.0f;
     auto step = [&](SlabOct<M>& slot) {
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
+      // per slot its 8 rows' LDS reads, then the FMAs as they land; the reads of slot m + 1 are
+      // issued before the FMAs of slot m, so that a wave has reads in flight while it computes
+      _Float16 x[M][8][SPL];
+      auto reads = [&](int m) {
         const uint32_t dw[4] = {slot.id[m].x, slot.id[m].y, slot.id[m].z, slot.id[m].w};
-        const uint32_t wv[4] = {slot.val[m].x, slot.val[m].y, slot.val[m].z, slot.val[m].w};
-        // the slot's 8 rows' reads first, then the FMAs as they land
-        _Float16 x[8][SPL];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const uint32_t r = (dw[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-          const uint32_t off = lane_base + r * (P * 2);
+          const uint32_t off = (LPO == 1 ? 0u : lane_base) + slab_row_off<P>(dw[e >> 1], e & 1);
           // plane offsets fold into the reads' immediate offsets (below 64 KB)
 #pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            constexpr int kFold = 65535 / PB + 1;  // planes one base covers
-            const uint32_t base = off + static_cast<uint32_t>((q / kFold) * kFold * PB);
-            slab_read<P>(lds + (q % kFold) * (PB / 2), base, &x[e][q * P]);
-          }
+          for (int q = 0; q < NR; ++q) slab_read_at<P>(off + static_cast<uint32_t>(q * PB), &x[m][e][q * P]);
         }
+      };
+      reads(0);
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (m + 1 < M) reads(m + 1);
+        const uint32_t wv[4] = {slot.val[m].x, slot.val[m].y, slot.val[m].z, slot.val[m].w};
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const _Float16 wh =
               UNI ? one_h : __builtin_bit_cast(_Float16, static_cast<uint16_t>(wv[e >> 1] >> (16 * (e & 1))));
 #pragma unroll
           for (int t = 0; t < SPL; ++t)
-            acc[m][t] = __builtin_fmaf(static_cast<float>(x[e][t]), static_cast<float>(wh), acc[m][t]);
+            acc[m][t] = __builtin_fmaf(static_cast<float>(x[m][e][t]), static_cast<float>(wh), acc[m][t]);
         }
       }
       if (cur.o + 1 == cur.nk) {
@@ -1756,13 +1801,13 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       }
       slot = slab_oct_load<T, UNI>(w, ahead);
-      slab_cur_next<T>(ahead, nout, oi, sgoff, bstride);
-      slab_cur_next<T>(cur, nout, oi, sgoff, bstride);
+      slab_cur_next<T>(ahead, bhi, oi, sgoff, bstride);
+      slab_cur_next<T>(cur, bhi, oi, sgoff, bstride);
     };
     // the wave's step count (octets over its batches), so that the main loop runs whole rounds
     // of the ring with no exit test between steps (the compiler's waits then count exactly)
     int nsteps = 0;
-    for (int b = b0; b * Sh::kBatch < nout; b += bstride) nsteps += sgoff[b + 1] - sgoff[b];
+    for (int b = b0; b < bhi; b += bstride) nsteps += sgoff[b + 1] - sgoff[b];
     int k = 0;
     for (; k + kSlabAhead <= nsteps; k += kSlabAhead) {
 #pragma unroll
@@ -1807,14 +1852,13 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
         if (want_band && ((sflag[par][2][tid >> 5] >> (tid & 31)) & 1u)) a.band_next[q] = 1;
       }
     }
-    if (next < nitems) {
+    if (sl < sl_last) {
       if (!ident) {
-        set_src(next / parts);
+        set_src(sl + 1);
         __syncthreads();  // psrc of the next slab (gathered loads read it)
       }
-      load_slab(next / parts);
+      load_slab(sl + 1);
     }
-    item = next;
   }
 }
 
@@ -2418,22 +2462,11 @@ hipError_t launch_convert_layout(const ConvArgs& c, int grid, hipStream_t st) {
   }
   return hipGetLastError();
 }
-int spmm_slab_bytes(int t) {
-  switch (t) {
-    case 64: return SlabShape<64>::kPlanes * SlabShape<64>::kPb;
-    case 32: return SlabShape<32>::kPlanes * SlabShape<32>::kPb;
-    case 16: return SlabShape<16>::kPlanes * SlabShape<16>::kPb;
-    case 8: return SlabShape<8>::kPlanes * SlabShape<8>::kPb;
-    case 4: return SlabShape<4>::kPlanes * SlabShape<4>::kPb;
-    case 2: return SlabShape<2>::kPlanes * SlabShape<2>::kPb;
-    default: return SlabShape<1>::kPlanes * SlabShape<1>::kPb;
-  }
-}
 hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   const int t = slab_samples(a.x_rows);
   if (t == 0 || w.sid == nullptr || w.sgoff == nullptr || grid < 1) return hipErrorInvalidValue;
   const bool uni = w.sval == nullptr;
-  const size_t bytes = static_cast<size_t>(spmm_slab_bytes(t));
+  constexpr size_t bytes = kSlabMaxBytes + kSlabMiscBytes;
 #define SLAB_LAUNCH(T_)                                                                                       \
   case T_:                                                                                                    \
     if (a.band_next != nullptr)                                                                               \
@@ -2548,8 +2581,14 @@ hipError_t configure_kernels() {
                                 SLAB_KERNELS(4),  SLAB_KERNELS(2),  SLAB_KERNELS(1)};
 #undef SLAB_KERNELS
   for (const void* k : slab_kernels) {
-    const hipError_t e5 = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMaxBytes);
+    const hipError_t e5 =
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kSlabMaxBytes + kSlabMiscBytes);
     if (e5 != hipSuccess) return e5;
+    // their LDS reads address the slab from LDS address 0: no kernel-static LDS may precede it
+    hipFuncAttributes attr;
+    const hipError_t e6 = hipFuncGetAttributes(&attr, k);
+    if (e6 != hipSuccess) return e6;
+    if (attr.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
   }
   return hipSuccess;
 }

@@ -63,8 +63,10 @@ struct DevLayer {
   float svalue = 0.0f;
 };
 
-// LDS of the sample-slab kernel's slab (at most this many bytes).
+// LDS of the sample-slab kernel's slab (at most this many bytes), followed by kSlabMiscBytes of
+// per-slab bookkeeping (the slab therefore starts at LDS address 0).
 constexpr int kSlabMaxBytes = 131072 + 256;
+constexpr int kSlabMiscBytes = 512;
 // Slab layout.  A slab holds T consecutive sample positions of every row, in planes of
 // P = min(T, 4) samples: plane k holds samples kP..kP+P-1 of all rows, P values a row, so that
 // a row's values for one plane are one 8-byte (P = 4) LDS read.  In HBM (the sample-blocked
@@ -211,8 +213,8 @@ hipError_t launch_spmm_mfma_lds(const StepArgs& a, const DevLayer& w, int grid, 
 // Band-streaming MFMA kernel (band image present, a.x_rows input rows; w.bmask used when set,
 // w.bwt otherwise); one work-group of 512 threads per CU, any grid of at least 8 work-groups.
 hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
-// Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 1024 threads,
-// one work-group per CU); any grid.
+// Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 512 threads,
+// one work-group per CU); any grid (the work is split into one balanced range per work-group).
 hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 // Activations of `rows` rows at positions [0, counts[count_idx]) (leading dimension act_ld of that
 // count in the neuron-major layout) converted between the neuron-major layout and the
