@@ -1547,6 +1547,53 @@ __device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* 
   }
 }
 
+// The slab of output positions q0 = sl*T .. after a compacting scan, from a sample-blocked input:
+// sample t is input position p = psrc[t], whose values are every P-th of its source plane (slab
+// p / T, plane (p % T) / P, lane p % P).  The plane is read in coalesced 16-byte pieces (8 / P
+// rows of P samples: neighbouring threads, neighbouring pieces), kU samples' pieces in flight
+// per thread, and the sample's values picked out into its LDS plane.  Rows past x_rows (up to
+// slab_rows) take whatever the input's pad rows hold; the slab image never reads them.
+template <int T>
+__device__ __forceinline__ void slab_load_gather_blocked(const StepArgs& a, lds_half_t* lds, const int* psrc,
+                                                         int x_rows, int tid) {
+  constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
+  constexpr int kRpp = 8 / P;              // rows per 16-byte piece
+  constexpr int kU = T < 8 ? T : 8;        // samples in flight per thread
+  const int R = slab_rows(x_rows);
+  const int pieces = R / kRpp;
+  const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
+#pragma unroll 1
+  for (int t0 = 0; t0 < T; t0 += kU) {
+    int src[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) src[u] = psrc[t0 + u];
+#pragma unroll 1
+    for (int c = tid; c < pieces; c += kSlabThreads) {
+      uint4 v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int p = src[u];
+        v[u] = make_uint4(0u, 0u, 0u, 0u);
+        if (p >= 0)
+          v[u] = *reinterpret_cast<const uint4*>(X + static_cast<int64_t>(p / T) * T * R +
+                                                 static_cast<int64_t>((p % T) / P) * R * P + static_cast<int64_t>(c) * 8);
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int t = t0 + u, j = src[u] >= 0 ? src[u] % P : 0;
+        const uint32_t d[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+        lds_half_t* dst = lds + (t / P) * PH + t % P + c * kRpp * P;
+#pragma unroll
+        for (int rr = 0; rr < kRpp; ++rr) {
+          const int k = rr * P + j;  // half k of the piece: row c * kRpp + rr, lane j
+          const uint32_t dw = (k >> 1) == 0 ? d[0] : (k >> 1) == 1 ? d[1] : (k >> 1) == 2 ? d[2] : d[3];
+          lds_put2(dst + rr * P, static_cast<uint16_t>(dw >> (16 * (k & 1))));
+        }
+      }
+    }
+  }
+}
+
 // OR of the lanes' flag bits per slab sample: lane (slot, group g) holds bit b for sample
 // g * SPL + b; the result's bit s is set when any lane's bit for sample s is.
 template <int T>
@@ -1676,6 +1723,8 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
       // one sample per slab: the sample's rows are one contiguous block of the sample-blocked
       // input (slab psrc[0] of it), loaded whole instead of value by value
       slab_load_blocked<T>(a, lds, __builtin_amdgcn_readfirstlane(psrc[0]), x_rows, wave, lane);
+    } else if (x_slab) {
+      slab_load_gather_blocked<T>(a, lds, psrc, x_rows, tid);
     } else {
       slab_load_gather<T>(a, lds, psrc, x_rows, ldx, x_slab, tid);
     }
