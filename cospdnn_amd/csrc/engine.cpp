@@ -1454,14 +1454,15 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
     for (size_t i = old; i < e->join_ev.size(); ++i) HIP_TRY(hipEventCreateWithFlags(&e->join_ev[i], hipEventDisableTiming));
   }
   if (!e->fork_ev) HIP_TRY(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
+  // the side holding each network's final ids, also on graph replays (which enqueue nothing)
   std::vector<int> side(n, 0);
+  for (int i = 0; i < n; ++i) side[i] = (nets[i]->L & 1) ^ 1;
   const bool graph = o.use_graph != 0 && o.layer_timing == 0;
   auto enqueue_all = [&]() -> int {
     HIP_TRY(hipEventRecord(e->fork_ev, st));
     for (int c = 0; c < chains; ++c) HIP_TRY(hipStreamWaitEvent(e->side[c], e->fork_ev, 0));
     for (int i = 0; i < n; ++i) {
       hipStream_t si = e->side[chain[i]];
-      side[i] = (nets[i]->L & 1) ^ 1;
       if (dev_guard[i])
         CALL_TRY(enqueue_main_guarded(nets[i], o, cfg[i], si, &side[i], e->side[chains + chain[i]]));
       else

@@ -90,3 +90,36 @@ def test_c5_six_networks_full_size(engine):
             assert o.mfma_layers == sp.L, (sp.name, mode)
     for n in nets:
         n.close()
+
+
+def test_graph_replay_reads_the_final_ids_when_the_last_layer_kills_rows(engine):
+    """A replayed batch graph enqueues nothing, so the side holding each network's final ids
+    must not depend on the capture: networks whose last layer kills most rows (the two id
+    lists then differ), an even and an odd layer count, captured once and replayed twice,
+    categories equal to the oracle's every time."""
+    rng = np.random.default_rng(77)
+    nets, want = [], []
+    for N, L, S in ((1024, 6, 2500), (2048, 5, 1800)):
+        layers = []
+        for l in range(L):
+            k = 8 if l == L - 1 else 32  # 8 inputs per output in the last layer: most rows die there
+            cp = np.arange(N + 1, dtype=np.int32) * k
+            rows = np.concatenate([np.sort(rng.choice(N, k, replace=False)) for _ in range(N)]).astype(np.int32)
+            layers.append((cp, rows, np.full(len(rows), 0.0625, np.float32)))
+        y0 = (rng.random((S, N)) < 0.3).astype(np.float32)
+        alive, live = orc.forward_fp64(orc.OracleNet.explicit(N, -0.3, layers, y0.astype(np.uint8)), 0, S)
+        assert 0 < live[-1] < live[-2] // 2  # the last layer kills most rows, not all
+        want.append(np.flatnonzero(alive))
+        net = cd.Net(engine, N, L, S, -0.3)
+        for l, (cp, rw, vl) in enumerate(layers):
+            net.set_layer_csc(l, cp, rw, vl)
+        net.set_input_dense(y0)
+        net.upload(cd.default_opts(max_tile_waste=64.0))
+        nets.append(net)
+    for call in ("capture", "replay", "replay again"):
+        say(f"batch graph, last layer kills rows: {call}")
+        outs = cd.run(nets, cd.default_opts(batch_concurrent=2, use_graph=1))
+        for o, w in zip(outs, want):
+            assert np.array_equal(o.categories, w), call
+    for n in nets:
+        n.close()
