@@ -397,20 +397,34 @@ def bench_batch(cd, np, args, specs):
         tuning = {"method": "exhaustive per network", "tuning_s": time.perf_counter() - t0}
         log(f"[bench] tuned in {tuning['tuning_s']:.1f}s")
     edges = float(sum(sp.edges for sp in specs))
-    seq_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=0)
-    outs_s, wall_s = timed_runs_many(cd, nets, seq_opts, args.steps, args.warmup)
+    # one after another: each network's own pass (replayed from its hipGraph, and launched
+    # eagerly); the faster is the sequential time
+    graphs = (1, 0) if args.graph else (0,)
+    seq_best = None
+    sequential = {}
+    for g in graphs:
+        seq_opts = cd.default_opts(precision=cd.FP16, use_graph=g, batch_concurrent=0)
+        outs_q, wall_q = timed_runs_many(cd, nets, seq_opts, args.steps, args.warmup)
+        sequential["graph" if g else "eager"] = wall_q * 1e3 / args.steps
+        log(f"[bench] one after another, {'graph' if g else 'eager'}: {wall_q * 1e3 / args.steps:.2f} ms")
+        if seq_best is None or wall_q < seq_best[1]:
+            seq_best = (outs_q, wall_q, g)
+    outs_s, wall_s, seq_graph = seq_best
+    seq_opts = cd.default_opts(precision=cd.FP16, use_graph=seq_graph, batch_concurrent=0)
     best = None
     concurrent = {}
-    for chains in (2, 3):
-        for share in (0, 1):
-            bat_opts = cd.default_opts(precision=cd.FP16, use_graph=1 if args.graph else 0, batch_concurrent=chains,
-                                       batch_cu_share=share)
-            outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
-            concurrent[f"{chains} chains, cu_share {share}"] = wall_c * 1e3 / args.steps
-            log(f"[bench] {chains} chains, cu_share {share}: {wall_c * 1e3 / args.steps:.2f} ms")
-            if best is None or wall_c < best[1]:
-                best = (outs_c, wall_c, chains, share)
-    outs_b, wall_b, chains, share = best
+    settings = [(chains, share, graphs[0]) for chains in (2, 3) for share in (0, 1)]
+    if args.graph:
+        settings += [(2, 0, 0), (3, 0, 0)]  # the chains launched eagerly
+    for chains, share, g in settings:
+        bat_opts = cd.default_opts(precision=cd.FP16, use_graph=g, batch_concurrent=chains, batch_cu_share=share)
+        outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
+        key = f"{chains} chains, cu_share {share}, {'graph' if g else 'eager'}"
+        concurrent[key] = wall_c * 1e3 / args.steps
+        log(f"[bench] {key}: {wall_c * 1e3 / args.steps:.2f} ms")
+        if best is None or wall_c < best[1]:
+            best = (outs_c, wall_c, chains, share, g)
+    outs_b, wall_b, chains, share, con_graph = best
     same = all(np.array_equal(a.categories, b.categories) for a, b in zip(outs_s[0], outs_b[0]))
     ms_s = wall_s * 1e3 / args.steps
     ms_b = wall_b * 1e3 / args.steps
@@ -442,11 +456,12 @@ def bench_batch(cd, np, args, specs):
         "config": {"workload": f"{args.config}: " + ", ".join(f"(N={sp.N}, L={sp.L}, samples={sp.S})" for sp in specs)
                    + ", random 32 nnz/column of 0.0625", "networks": len(specs), "parallelism": "single GPU"},
         "batch_mode": mode,
-        "sequential": {"ms_per_step": ms_s, "value": edges / (ms_s * 1e-3)},
-        "concurrent": {"ms_per_step": ms_b, "value": edges / (ms_b * 1e-3)},
+        "sequential": {"ms_per_step": ms_s, "value": edges / (ms_s * 1e-3), "graph": bool(seq_graph)},
+        "concurrent": {"ms_per_step": ms_b, "value": edges / (ms_b * 1e-3), "graph": bool(con_graph)},
         "networks_alone": per_net,
         "concurrent_chains": chains,
         "concurrent_cu_share": share,
+        "sequential_ms_by_setting": sequential,
         "concurrent_ms_by_setting": concurrent,
         "concurrent_over_sequential": ms_s / ms_b,
         "categories_equal": same,
