@@ -415,7 +415,7 @@ def bench_batch(cd, np, args, specs):
     concurrent = {}
     settings = [(chains, share, graphs[0]) for chains in (2, 3) for share in (0, 1)]
     if args.graph:
-        settings += [(2, 0, 0), (3, 0, 0)]  # the chains launched eagerly
+        settings += [(2, 0, 0), (3, 0, 0), (4, 0, 0)]  # the chains launched eagerly (4: the engine's cap)
     for chains, share, g in settings:
         bat_opts = cd.default_opts(precision=cd.FP16, use_graph=g, batch_concurrent=chains, batch_cu_share=share)
         outs_c, wall_c = timed_runs_many(cd, nets, bat_opts, args.steps, args.warmup)
