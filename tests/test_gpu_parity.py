@@ -529,8 +529,9 @@ def test_c4_shape_on_a_sample_slice(engine):
     """The largest network (N=65536, L=480: ~1e9 weights) on the first 12000 samples (188
     64-sample tiles per layer at first, 4 or fewer after layer 7), with the tile images the
     bench uses (waste ~32): the default MFMA launch, the row-gather kernel on every layer,
-    and the tuned per-layer plan (MFMA on the wide layers, the row-gather kernel on many of
-    the few-row ones), eager and graph-replayed; categories equal to the FP64 oracle's."""
+    the sample-slab kernel on every layer (one sample per slab), and the tuned per-layer plan
+    (the bench's: MFMA on layer 0, the slab kernel after it), eager and graph-replayed;
+    categories equal to the FP64 oracle's."""
     spec = cd.config_nets("c4")[0]
     S = 12000
     say(f"c4 shape N={spec.N} L={spec.L} S={S}")
@@ -546,6 +547,13 @@ def test_c4_shape_on_a_sample_slice(engine):
     net.set_plan([(cd.PLAN_ROWS, 1, 5)] * spec.L)
     out = cd.run([net], cd.default_opts(precision=cd.FP16))[0]
     assert np.array_equal(out.categories, want)
+    net.set_plan([(cd.PLAN_SLAB, 0, 1)] * spec.L)
+    for graph in (0, 1):
+        out = cd.run([net], cd.default_opts(precision=cd.FP16, use_graph=graph))[0]
+        say(f"  all slab, graph={graph}: categories={len(out.categories)} ms={out.ms_total:.2f}")
+        assert set(out.layer_kernel.tolist()) == {cd.KERNEL_SLAB}
+        assert np.array_equal(out.categories, want), graph
+        assert_live_close(out.live, live, out)
     plan, _, _ = cd.tune_exhaustive(net, reps=1)
     net.set_plan(plan)
     kinds = {}
