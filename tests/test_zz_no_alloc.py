@@ -58,8 +58,32 @@ def test_noalloc_matches_run(engine):
     o = cd.default_opts(precision=cd.FP16, spmm_order=5, keep_holes_ppm=30000)
     check_same(cd.run_noalloc(nets[0], o), cd.run([nets[0]], cd.default_opts(
         precision=cd.FP16, spmm_order=5, keep_holes_ppm=30000, guard_concurrent=0))[0], wants[0])
+    # the sample-slab kernel on every layer (64 and 32 samples per slab)
+    for net, want in zip(nets, wants):
+        say(f"slab N={net.N}")
+        check_same(cd.run_noalloc(net, cd.default_opts(precision=cd.FP16, spmm_slab=1)),
+                   cd.run([net], cd.default_opts(precision=cd.FP16, spmm_slab=1, guard_concurrent=0))[0], want)
     for n in nets:
         n.close()
+
+
+def test_noalloc_slab_layers_next_to_mfma_layers(engine):
+    """4 samples per slab (N = 16384), slab layers alternating with MFMA layers: cospdnn_run
+    converts the activation layout beside every slab layer through a third buffer, the fixed
+    array holds none, so there the slab kernel reads and writes the neuron-major layout
+    itself; the same results either way."""
+    N, L, S, bias, seed = 16384, 10, 200, -0.4, 65  # the fixed array also holds both tiers' buffers
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, cd.GEN_RANDOM, seed), 0, S)
+    net = make(engine, N, L, S, bias, cd.GEN_RANDOM, seed)
+    net.set_plan([cd.LayerPlan(cd.PLAN_SLAB, 0, 1) if l % 2 == 0 else cd.LayerPlan(cd.PLAN_MFMA, 1, 4)
+                  for l in range(L)])
+    ref = cd.run([net], cd.default_opts(precision=cd.FP16, guard_concurrent=0))[0]
+    got = cd.run_noalloc(net, cd.default_opts(precision=cd.FP16))
+    say(f"N={N} alternating slab/MFMA: categories {len(got.categories)} ms {got.ms_total:.2f} "
+        f"(cospdnn_run {ref.ms_total:.2f})")
+    assert set(got.layer_kernel.tolist()) == {cd.KERNEL_SLAB, cd.KERNEL_MFMA_MASK}
+    check_same(got, ref, np.flatnonzero(alive))
+    net.close()
 
 
 def test_noalloc_refuses_what_it_cannot_hold(engine):
