@@ -82,7 +82,7 @@ class Result(ctypes.Structure):
 PLAN_AUTO, PLAN_MFMA, PLAN_SIMT, PLAN_MFMA_COMPACT, PLAN_ROWS, PLAN_SLAB = 0, 1, 2, 3, 4, 5
 # SpMM kernels reported per layer (RunOutput.layer_kernel)
 (KERNEL_SIMT, KERNEL_SIMT_ROWS, KERNEL_MFMA, KERNEL_MFMA_MASK, KERNEL_MFMA_LDS, KERNEL_MFMA_LDS_MASK, KERNEL_MFMA_BAND,
- KERNEL_MFMA_BAND_MASK, KERNEL_WAVE_ROWS, KERNEL_MFMA32, KERNEL_SLAB) = range(11)
+ KERNEL_MFMA_BAND_MASK, KERNEL_WAVE_ROWS, KERNEL_MFMA32, KERNEL_SLAB, KERNEL_SLAB_BITS) = range(12)
 TUNE_FEATURES = 15
 
 

@@ -179,6 +179,7 @@ int upload_layer(cospdnn_net* net, int l, const BlockedLayer& bl, bool sat_stabl
   ls->fp16_exact = bl.fp16_exact;
   ls->nnz = bl.nnz;
   ls->sat_stable = sat_stable;
+  for (size_t s = 0; s + 1 < bl.optr.size(); ++s) ls->max_list = std::max(ls->max_list, bl.optr[s + 1] - bl.optr[s]);
   if (bl.has_tiles && streamed) {
     CALL_TRY(stage_streamed_image(*ls, bl));
     if (bl.has_mask) ls->view.wmask_val = bl.wmask_val;
@@ -346,7 +347,17 @@ int pick_kernel(const cospdnn_net* net, const RunCfg& cfg, int l, int64_t S_run)
   const bool few = S_run <= kFewRows;
   const LayerLaunch lx = layer_launch(net, cfg, l);
   const LayerStore& ls = *net->dl[l];
-  if (lx.slab) return COSPDNN_KERNEL_SLAB;
+  if (lx.slab) {
+    // layer 0 on the bit-packed binary input with one weight value: bit-sliced counts (6 count
+    // planes: lists of at most 63 rows; one lane per output slot: at most 16 samples per slab;
+    // no guard-band flags)
+    const bool band0 = cfg.band > 0.0f && cfg.band_from <= 0;
+    const int t = slab_samples(net->N);
+    if (l == 0 && fp16 && cfg.bits && net->bits.p && ls.view.sid && !ls.view.sval && t >= 1 && t <= 16 &&
+        ls.max_list <= 63 && !band0)
+      return COSPDNN_KERNEL_SLAB_BITS;
+    return COSPDNN_KERNEL_SLAB;
+  }
   if (lx.mfma) {
     // the mask image where the pass reads it: resident, or streamed in mask words
     const bool mask = cfg.wmask && (ls.streamed ? ls.stream_mask : ls.view.wmask != nullptr);
@@ -524,11 +535,22 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
     const Dtype t_in = l == 0 ? Dtype::F16 : (fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32));
     const Dtype t_out = fp16 ? Dtype::F16 : (fp64 ? Dtype::F64 : Dtype::F32);
     switch (kern) {
+      case COSPDNN_KERNEL_SLAB_BITS:
+        if (a.xbits != nullptr) {
+          // writes the next layer's layout itself (16 to 64 samples of a row per lane)
+          a.x_rows = input_rows(net, l);
+          a.x_layout = kLayoutRows;
+          a.y_layout = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
+          HIP_TRY(launch_spmm_bits(a, view, capg(e->cu_count), st));
+          break;
+        }
+        [[fallthrough]];  // (an input other than the network's store has no bit-packed copy)
       case COSPDNN_KERNEL_SLAB: {
         // consecutive slab layers pass the activations sample-blocked by the slab size; the
         // buffers any other kernel reads or writes (and the last layer's output) stay neuron-major
         a.x_rows = input_rows(net, l);
-        const bool prev_slab = l >= 1 && pick_kernel(net, cfg, l - 1, S_run) == COSPDNN_KERNEL_SLAB;
+        const int kprev = l >= 1 ? pick_kernel(net, cfg, l - 1, S_run) : -1;
+        const bool prev_slab = kprev == COSPDNN_KERNEL_SLAB || kprev == COSPDNN_KERNEL_SLAB_BITS;
         const bool next_slab = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB;
         a.x_layout = prev_slab ? kLayoutSlab : kLayoutRows;
         a.y_layout = next_slab ? kLayoutSlab : kLayoutRows;
@@ -1221,7 +1243,7 @@ int finish_result(cospdnn_net* net, const cospdnn_opts& o, const RunCfg& cfg, st
     const int k = pick_kernel(net, cfg, l, S);
     if (res->layer_kernel) res->layer_kernel[l] = k;
     const bool m = k != COSPDNN_KERNEL_SIMT && k != COSPDNN_KERNEL_SIMT_ROWS && k != COSPDNN_KERNEL_WAVE_ROWS &&
-                   k != COSPDNN_KERNEL_SLAB;
+                   k != COSPDNN_KERNEL_SLAB && k != COSPDNN_KERNEL_SLAB_BITS;
     mfma_layers += m ? 1 : 0;
   }
   res->alg_bytes = bytes;
@@ -1565,6 +1587,7 @@ const char* cospdnn_kernel_name(int kernel) {
     case COSPDNN_KERNEL_MFMA32: return "spmm_mfma32_f32in_kernel";  // layer 0: spmm_mfma32_f16in_kernel
     // spmm_slab_kernel_<T>, or spmm_slab_uni_kernel_<T> for a layer of one weight value
     case COSPDNN_KERNEL_SLAB: return "spmm_slab_kernel";
+    case COSPDNN_KERNEL_SLAB_BITS: return "spmm_bits_kernel";
     default: return nullptr;
   }
 }

@@ -82,6 +82,7 @@ struct LayerStore {
   bool fp16_exact = true;
   bool sat_stable = false;  // an all-32 input row gives an all-32 output row in FP32 and FP64
   int64_t nnz = 0;
+  int max_list = 0;  // the longest column (input rows of one output)
   // Streamed MFMA image (the weight budget was exceeded): kptr | gidx | mask words (or FP16
   // fragments) in pinned host memory, copied into one of the network's two device slots one
   // layer ahead of its SpMM (the reference streams every layer so, cospdnn.cu:170-210)

@@ -1937,6 +1937,321 @@ SLAB_KERNEL(1)
 #undef SLAB_KERNEL
 
 // --------------------------------------------------------------------------
+// Layer 0 of a sample-slab plan on the bit-packed binary input with one weight value w: an
+// output is y = clamp(w c + b, 0, 32), c the number of its input rows set in the sample.  A
+// work-group holds the input bits of W = 16 P samples of every row in LDS, 2 P bytes a row at
+// byte row * 2 P (the slab kernel's plane layout for the layer's slab size T, P = slab_plane(T),
+// so the slab image's entry order spreads the LDS banks here too), and a lane counts for each
+// of its output slots the W samples' set bits over the slot's rows in bit-sliced form: the 8
+// rows of an octet reduce to a 4-bit count through full adders (v_xor3 and a majority), which
+// ripples into 6 count planes (lists of at most 63 rows).  The epilogue turns the planes into
+// per-sample counts and computes w c + b in FP32 as the slab kernel does (its sum of 1.0 over
+// the set rows is the same count), so the outputs are the slab kernel's bit for bit.  One LDS
+// read serves W samples where the slab kernel reads one per sample (at T = 1) to four.
+// --------------------------------------------------------------------------
+// flag bits of an FP16 value h in [0, 32] by arithmetic (as compares, 16 to 64 samples' lane
+// masks would live in scalar registers at once): alive 1 when h != 0, unsaturated 1 below 0x5000
+__device__ __forceinline__ uint32_t bits_alive(uint32_t h) {
+  uint32_t r;
+  asm("v_min_u32 %0, %1, 1" : "=v"(r) : "v"(h));
+  return r;
+}
+__device__ __forceinline__ uint32_t bits_unsat(uint32_t h) {
+  uint32_t r;
+  asm("v_add_u32 %0, 0xffffb000, %1\n\tv_lshrrev_b32 %0, 31, %0" : "=v"(r) : "v"(h));
+  return r;
+}
+__device__ __forceinline__ void bits_fa(uint32_t a, uint32_t b, uint32_t c, uint32_t& s, uint32_t& cy) {
+  s = a ^ b ^ c;
+  cy = (a & b) | (c & (a ^ b));
+}
+
+// W input bits (2 P bytes) of row r for the slab of positions q0.. into LDS, ident layout: the
+// bit-packed input holds, per 64-sample tile t and row r, 64 bits at byte (t * rows + r) * 8
+// (bit i = sample 64 t + i); gathered: bit by bit through psrc.
+template <int T>
+__device__ __forceinline__ void bits_load(const StepArgs& a, lds_half_t* lds, int q0, const int* psrc, bool ident,
+                                          int x_rows, int tid) {
+  constexpr int P = SlabShape<T>::kP, W = 16 * P;
+  const int64_t xr = a.xbits_rows;
+  if (ident) {
+    const int t = q0 >> 6, k = (q0 & 63) / W;
+    const char* base = reinterpret_cast<const char*>(a.xbits) + static_cast<int64_t>(t) * xr * 8 + k * 2 * P;
+#pragma unroll 4
+    for (int r = tid; r < x_rows; r += kSlabThreads) {
+      const char* src = base + static_cast<int64_t>(r) * 8;
+      if constexpr (P == 4) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        lds_put8(lds + r * 4, v.x, v.y);
+      } else if constexpr (P == 2) {
+        lds_put4(lds + r * 2, *reinterpret_cast<const uint32_t*>(src));
+      } else {
+        lds_put2(lds + r, *reinterpret_cast<const uint16_t*>(src));
+      }
+    }
+  } else {
+    for (int r = tid; r < x_rows; r += kSlabThreads) {
+      uint32_t wd[2] = {0u, 0u};
+      for (int i = 0; i < W; ++i) {
+        const int p = psrc[i];
+        if (p < 0) continue;
+        const uint32_t word = a.xbits[(static_cast<int64_t>(p >> 6) * xr + r) * 2 + ((p >> 5) & 1)];
+        wd[i >> 5] |= ((word >> (p & 31)) & 1u) << (i & 31);
+      }
+      if constexpr (P == 4)
+        lds_put8(lds + r * 4, wd[0], wd[1]);
+      else if constexpr (P == 2)
+        lds_put4(lds + r * 2, wd[0]);
+      else
+        lds_put2(lds + r, static_cast<uint16_t>(wd[0]));
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int T>
+__device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, lds_half_t* lds) {
+  using Sh = SlabShape<T>;
+  static_assert(Sh::kLpo == 1, "one lane per output slot");
+  constexpr int P = Sh::kP, M = Sh::kM, W = 16 * P;
+  constexpr int NW = W > 32 ? 2 : 1;  // 32-bit words of a row's bits
+  int* const psrc = reinterpret_cast<int*>((_Float16*)lds + kSlabMaxBytes / 2);
+  uint32_t* const sflag = reinterpret_cast<uint32_t*>(psrc + 64);  // [parity][alive, unsat][word]
+  const int n = __builtin_amdgcn_readfirstlane(a.counts[a.layer]);
+  const int n_out = __builtin_amdgcn_readfirstlane(a.counts[a.layer + 1]);
+  const bool ident = n_out == n;
+  const int x_rows = a.x_rows;
+  const int nout = w.n_out;
+  const int ry = slab_rows(nout);
+  const int nsl = (n_out + W - 1) / W;
+  const int nb = (nout + Sh::kBatch - 1) / Sh::kBatch;
+  const int64_t units = static_cast<int64_t>(nsl) * nb;
+  const int64_t u0 = units * blockIdx.x / gridDim.x, u1 = units * (blockIdx.x + 1) / gridDim.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t ldy = y_ld(a);
+  _Float16* __restrict__ Y = static_cast<_Float16*>(a.y);
+  const bool y_slab = a.y_layout == kLayoutSlab;
+  const cint_ptr_t sgoff = (cint_ptr_t)w.sgoff;
+  const bool want_unsat = a.unsat_next != nullptr;
+  const float bias = a.bias, wuni = w.svalue;
+  const int zr = slab_rows(x_rows);
+  if (tid < NW) lds_put4(lds + zr * P + 2 * tid, 0u);  // the zero row (pads of shorter lists; 4 bytes at P = 1)
+  auto set_src = [&](int sl) {
+    if (tid < W) {
+      const int q = sl * W + tid;
+      psrc[tid] = q < n_out ? (ident ? q : a.qsrc[q]) : -1;
+    }
+  };
+  int sl = static_cast<int>(u0 / nb);
+  const int sl_last = u1 > u0 ? static_cast<int>((u1 - 1) / nb) : sl - 1;
+  set_src(sl);
+  __syncthreads();  // psrc of the first slab, the zero row
+  if (sl <= sl_last) bits_load<T>(a, lds, sl * W, psrc, ident, x_rows, tid);
+  for (int it = 0; sl <= sl_last; ++sl, ++it) {
+    const int par = it & 1;
+    if (tid < 4) sflag[par * 4 + tid] = 0u;
+    __syncthreads();  // the slab and cleared flag words are in place
+    const int q0 = sl * W;
+    const int64_t ub = static_cast<int64_t>(sl) * nb;
+    const int blo = static_cast<int>((u0 > ub ? u0 : ub) - ub);
+    const int bhi = static_cast<int>((u1 < ub + nb ? u1 : ub + nb) - ub);
+    const int b0 = blo + wave, bstride = kSlabWaves;
+    uint32_t fa[NW], fu[NW];
+#pragma unroll
+    for (int j = 0; j < NW; ++j) fa[j] = fu[j] = 0u;
+    constexpr int kAhead = 3;
+    SlabCur cur, ahead;
+    slab_cur_batch<T>(cur, b0, bhi, lane, sgoff);
+    ahead = cur;
+    SlabOct<M> ring[kAhead];
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+      ring[u] = slab_oct_load<T, true>(w, ahead);
+      slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+    }
+    uint32_t cnt[M][6][NW];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int j = 0; j < NW; ++j) cnt[m][k][j] = 0u;
+    auto step = [&](SlabOct<M>& slot) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const uint32_t dw[4] = {slot.id[m].x, slot.id[m].y, slot.id[m].z, slot.id[m].w};
+        uint32_t x[8][NW];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t off = slab_row_off<P>(dw[e >> 1], e & 1);
+          const __attribute__((address_space(3))) char* pr = (const __attribute__((address_space(3))) char*)(uintptr_t)off;
+          if constexpr (P == 4) {
+            const u32x2_t v = *reinterpret_cast<const __attribute__((address_space(3))) u32x2_t*>(pr);
+            x[e][0] = v[0];
+            x[e][NW - 1] = v[1];
+          } else if constexpr (P == 2) {
+            x[e][0] = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t*>(pr);
+          } else {
+            x[e][0] = *reinterpret_cast<const __attribute__((address_space(3))) uint16_t*>(pr);
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          // 8 one-bit rows -> ones, twos, fours, eights
+          uint32_t s0, c0, s1, c1, ones, c3, t0, c4, twos, c5;
+          bits_fa(x[0][j], x[1][j], x[2][j], s0, c0);
+          bits_fa(x[3][j], x[4][j], x[5][j], s1, c1);
+          const uint32_t s2 = x[6][j] ^ x[7][j], c2 = x[6][j] & x[7][j];
+          bits_fa(s0, s1, s2, ones, c3);
+          bits_fa(c0, c1, c2, t0, c4);
+          twos = t0 ^ c3;
+          c5 = t0 & c3;
+          const uint32_t fours = c4 ^ c5, eights = c4 & c5;
+          // ripple into the 6 count planes
+          uint32_t k = cnt[m][0][j] & ones;
+          cnt[m][0][j] ^= ones;
+          uint32_t sum, cy;
+          bits_fa(cnt[m][1][j], twos, k, sum, cy);
+          cnt[m][1][j] = sum;
+          bits_fa(cnt[m][2][j], fours, cy, sum, k);
+          cnt[m][2][j] = sum;
+          bits_fa(cnt[m][3][j], eights, k, sum, cy);
+          cnt[m][3][j] = sum;
+          k = cnt[m][4][j] & cy;
+          cnt[m][4][j] ^= cy;
+          cnt[m][5][j] ^= k;
+        }
+      }
+      if (cur.o + 1 == cur.nk) {
+        // the batch's last octet: per slot and sample the count, w c + b, clamp, FP16, stores
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int s = cur.b * Sh::kBatch + m * 64 + lane;
+          if (s < nout) {
+            uint32_t d[W / 2];
+#pragma unroll
+            for (int t = 0; t < W; ++t) {
+              const int j = t >> 5, b = t & 31;
+              uint32_t c = 0;
+#pragma unroll
+              for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(cnt[m][k][j], b, 1) << k;
+              const float z = __builtin_fmaf(static_cast<float>(c), wuni, bias);
+              const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
+              const uint32_t h = __builtin_bit_cast(uint16_t, yv);
+              // flags from the FP16 value, as the slab kernel takes them (y in [0, 32]): alive when
+              // its bits are nonzero, unsaturated below 0x5000 (32.0); integer ops only
+              fa[j] |= bits_alive(h) << b;
+              fu[j] |= bits_unsat(h) << b;
+              if (t & 1)
+                d[t >> 1] |= h << 16;
+              else
+                d[t >> 1] = h;
+            }
+            if (q0 + W <= n_out) {
+              if (y_slab) {
+                // groups of P samples: the group of position q0 + g P is slab (q0 + g P) / T, plane
+                // ((g P) % T) / P, so its element offset grows by P * ry per group; one offset in
+                // VGPRs stepped along (the products per group would otherwise be hoisted into
+                // scalar registers, 16 to 64 of them)
+                int64_t off = slab_index<T>(q0, s, ry);
+                const int64_t step = static_cast<int64_t>(P) * ry;
+#pragma unroll
+                for (int g = 0; g < W / P; ++g) {
+                  _Float16* yg = Y + off;
+                  if constexpr (P == 1)
+                    *reinterpret_cast<uint16_t*>(yg) = static_cast<uint16_t>(d[g >> 1] >> (16 * (g & 1)));
+                  else
+                    slab_store<P>(yg, d + g * (P / 2));
+                  off += step;
+                  asm volatile("" : "+v"(off));
+                }
+              } else {
+                uint4* yp = reinterpret_cast<uint4*>(Y + static_cast<int64_t>(s) * ldy + q0);
+#pragma unroll
+                for (int v = 0; v < W / 8; ++v) yp[v] = make_uint4(d[4 * v], d[4 * v + 1], d[4 * v + 2], d[4 * v + 3]);
+              }
+            } else {
+              // the partial last slab: value by value, recomputed from the count planes
+#pragma unroll 1
+              for (int t = 0; t < W && q0 + t < n_out; ++t) {
+                const int jt = t >> 5, bt = t & 31;
+                uint32_t c = 0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(jt ? cnt[m][k][NW - 1] : cnt[m][k][0], bt, 1) << k;
+                const float z = __builtin_fmaf(static_cast<float>(c), wuni, bias);
+                const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
+                const int64_t at = y_slab ? slab_index<T>(q0 + t, s, ry) : static_cast<int64_t>(s) * ldy + q0 + t;
+                Y[at] = yv;
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int j = 0; j < NW; ++j) cnt[m][k][j] = 0u;
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      }
+      slot = slab_oct_load<T, true>(w, ahead);
+      slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+      slab_cur_next<T>(cur, bhi, lane, sgoff, bstride);
+    };
+    int nsteps = 0;
+    for (int b = b0; b < bhi; b += bstride) nsteps += sgoff[b + 1] - sgoff[b];
+    int k = 0;
+    for (; k + kAhead <= nsteps; k += kAhead) {
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) step(ring[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kAhead - 1; ++u)
+      if (k + u < nsteps) step(ring[u]);
+    // flags: OR over the wave's lanes, then over the work-group
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      uint32_t va = fa[j], vu = fu[j];
+#pragma unroll
+      for (int dl = 32; dl >= 1; dl >>= 1) {
+        va |= __shfl_xor(va, dl);
+        vu |= __shfl_xor(vu, dl);
+      }
+      if (lane == 0) {
+        if (va != 0u) atomicOr(&sflag[par * 4 + j], va);
+        if (want_unsat && vu != 0u) atomicOr(&sflag[par * 4 + 2 + j], vu);
+      }
+    }
+    __syncthreads();  // every read of this slab and every flag word are done
+    if (tid < W) {
+      const int q = q0 + tid;
+      if (q < n_out) {
+        if ((sflag[par * 4 + (tid >> 5)] >> (tid & 31)) & 1u) a.alive_next[q] = 1;
+        if (want_unsat && ((sflag[par * 4 + 2 + (tid >> 5)] >> (tid & 31)) & 1u)) a.unsat_next[q] = 1;
+      }
+    }
+    if (sl < sl_last) {
+      if (!ident) {
+        set_src(sl + 1);
+        __syncthreads();  // psrc of the next slab (gathered loads read it)
+      }
+      bits_load<T>(a, lds, (sl + 1) * W, psrc, ident, x_rows, tid);
+    }
+  }
+}
+
+#define BITS_KERNEL(T_)                                                                                 \
+  __global__ __launch_bounds__(kSlabThreads, 1) void spmm_bits_kernel_##T_(StepArgs a, DevLayer w) {     \
+    extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
+    bits_body<T_>(a, w, (lds_half_t*)slab_lds);                                                         \
+  }
+BITS_KERNEL(16)
+BITS_KERNEL(8)
+BITS_KERNEL(4)
+BITS_KERNEL(2)
+BITS_KERNEL(1)
+#undef BITS_KERNEL
+
+// --------------------------------------------------------------------------
 // Layout conversion between the neuron-major and the sample-blocked activation layouts, for a
 // sample-slab layer next to a layer of another kernel: with few samples per slab (T <= 4) the
 // slab kernel would read (or write) 2 to 8 bytes per row and slab in the neuron-major layout,
@@ -2539,6 +2854,21 @@ hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipS
 #undef SLAB_LAUNCH
   return hipGetLastError();
 }
+hipError_t launch_spmm_bits(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
+  const int t = slab_samples(a.x_rows);
+  if (t < 1 || t > 16 || w.sid == nullptr || w.sgoff == nullptr || w.sval != nullptr || a.xbits == nullptr || grid < 1 ||
+      (a.y_layout == kLayoutSlab && slab_samples(w.n_out) != t))
+    return hipErrorInvalidValue;
+  constexpr size_t bytes = kSlabMaxBytes + kSlabMiscBytes;
+  switch (t) {
+    case 16: hipLaunchKernelGGL(spmm_bits_kernel_16, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+    case 8: hipLaunchKernelGGL(spmm_bits_kernel_8, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+    case 4: hipLaunchKernelGGL(spmm_bits_kernel_4, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+    case 2: hipLaunchKernelGGL(spmm_bits_kernel_2, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+    default: hipLaunchKernelGGL(spmm_bits_kernel_1, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+  }
+  return hipGetLastError();
+}
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
   hipLaunchKernelGGL(spmm_wave_rows_kernel, dim3(grid), dim3(256), 0, st, a, w);
   return hipGetLastError();
@@ -2627,7 +2957,12 @@ hipError_t configure_kernels() {
   reinterpret_cast<const void*>(spmm_slab_kernel_##T_), reinterpret_cast<const void*>(spmm_slab_uni_kernel_##T_), \
       reinterpret_cast<const void*>(spmm_slab_band_kernel_##T_)
   const void* slab_kernels[] = {SLAB_KERNELS(64), SLAB_KERNELS(32), SLAB_KERNELS(16), SLAB_KERNELS(8),
-                                SLAB_KERNELS(4),  SLAB_KERNELS(2),  SLAB_KERNELS(1)};
+                                SLAB_KERNELS(4),  SLAB_KERNELS(2),  SLAB_KERNELS(1),
+                                reinterpret_cast<const void*>(spmm_bits_kernel_16),
+                                reinterpret_cast<const void*>(spmm_bits_kernel_8),
+                                reinterpret_cast<const void*>(spmm_bits_kernel_4),
+                                reinterpret_cast<const void*>(spmm_bits_kernel_2),
+                                reinterpret_cast<const void*>(spmm_bits_kernel_1)};
 #undef SLAB_KERNELS
   for (const void* k : slab_kernels) {
     const hipError_t e5 =

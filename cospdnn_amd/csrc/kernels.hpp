@@ -216,6 +216,11 @@ hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid,
 // Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 512 threads,
 // one work-group per CU); any grid (the work is split into one balanced range per work-group).
 hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// Layer 0 of a slab plan on the bit-packed binary input (a.xbits) with one weight value (slab image
+// without values, slab_samples(a.x_rows) <= 16, lists of at most 63 rows): per output the count
+// of set input rows by bit-sliced adders, y written in a.y_layout (sample-blocked by the same slab
+// size, or neuron-major); 512 threads, one work-group per CU, any grid.
+hipError_t launch_spmm_bits(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
 // Activations of `rows` rows at positions [0, counts[count_idx]) (leading dimension act_ld of that
 // count in the neuron-major layout) converted between the neuron-major layout and the
 // sample-blocked layout of slab size t (4, 2 or 1); to_slab: neuron-major -> blocked.

@@ -57,9 +57,13 @@ enum {
   COSPDNN_KERNEL_MFMA_BAND_MASK = 7,
   COSPDNN_KERNEL_WAVE_ROWS = 8,       /* row-gather kernel (COSPDNN_PLAN_ROWS) */
   COSPDNN_KERNEL_MFMA32 = 9,          /* FP32 MFMA over the FP32 tile image */
-  COSPDNN_KERNEL_SLAB = 10            /* FP16 sample-slab kernel: T whole input samples in LDS, every
+  COSPDNN_KERNEL_SLAB = 10,           /* FP16 sample-slab kernel: T whole input samples in LDS, every
                                          output of them by FP32 FMAs over the slab image
                                          (spmm_slab[_uni]_kernel_<T>) */
+  COSPDNN_KERNEL_SLAB_BITS = 11       /* layer 0 of a slab plan on the bit-packed binary input with one
+                                         weight value: 16 to 64 samples' input bits in LDS, per output
+                                         the count of set input rows by bit-sliced adders
+                                         (spmm_bits_kernel_<T>) */
 };
 
 typedef struct cospdnn_engine cospdnn_engine;

@@ -32,9 +32,18 @@ def gen_net(engine, N, L, S, bias, kind, seed, max_waste=64.0):
 
 def slab_T(N):
     for t in (64, 32, 16, 8, 4, 2, 1):
-        if (N + 1) * t * 2 <= 131072 + 256 and (N * t) % 8 == 0:
+        if (((N + 7) // 8 * 8 + 1) * min(t, 4) * 2 + 15) // 16 * 16 <= (131072 + 256) // (t // min(t, 4)) // 16 * 16:
             return t
     return 0
+
+
+def expect_slab(out, N, bits):
+    """Every layer on the sample-slab kernel; layer 0 on its bit-sliced variant when it reads the
+    bit-packed input (the generated and test layer-0 weights are one value, lists below 64)."""
+    ks = out.layer_kernel.tolist()
+    k0 = cd.KERNEL_SLAB_BITS if bits and slab_T(N) <= 16 else cd.KERNEL_SLAB
+    assert ks[0] == k0, (N, bits, ks[0])
+    assert set(ks[1:]) <= {cd.KERNEL_SLAB}
 
 
 # N, L, S, bias, kind, seed: one case per slab size
@@ -65,7 +74,7 @@ def test_slab_generated(engine, case):
                      dump_cap_rows=S)[0]
         say(f"  bits={bits} categories={len(out.categories)} oracle={len(want)} flagged={out.n_flagged} "
             f"ms={out.ms_total:.2f}")
-        assert set(out.layer_kernel.tolist()) == {cd.KERNEL_SLAB}
+        expect_slab(out, N, bits)
         assert np.array_equal(out.categories, want), bits
         assert out.live[0] == S
         assert np.abs(out.live - live).max() <= out.n_flagged + out.n_replayed + max(3, S // 1000)
@@ -111,7 +120,8 @@ def test_slab_layout_changes_and_plans(engine, N, L, S, bias, seed):
             out = cd.run([net], cd.default_opts(precision=cd.FP16, keep_holes_ppm=keep, use_graph=graph))[0]
             ks = out.layer_kernel.tolist()
             for l in range(L):
-                assert (ks[l] == cd.KERNEL_SLAB) == (plan[l].kernel == cd.PLAN_SLAB), (name, l, ks[l])
+                slab_k = ks[l] in (cd.KERNEL_SLAB, cd.KERNEL_SLAB_BITS)
+                assert slab_k == (plan[l].kernel == cd.PLAN_SLAB), (name, l, ks[l])
             assert np.array_equal(out.categories, want), (name, keep, graph)
             say(f"  {name} keep={keep} graph={graph}: categories={len(out.categories)} ms={out.ms_total:.2f}")
         out = cd.run([net], cd.default_opts(precision=cd.FP16, dump_layer=d), dump_cap_rows=S)[0]
@@ -157,7 +167,7 @@ def test_slab_mixed_values_and_uneven_columns(engine, N):
     for bits in (1, 0):
         out = cd.run([net], cd.default_opts(precision=cd.FP16, spmm_slab=1, input_bits=bits, dump_layer=d),
                      dump_cap_rows=S)[0]
-        assert set(out.layer_kernel.tolist()) == {cd.KERNEL_SLAB}
+        expect_slab(out, N, bits)
         assert np.array_equal(out.categories, np.flatnonzero(alive)), bits
         if out.dump_rows:
             err = np.abs(out.dump_vals - f64[out.dump_ids]).max()
@@ -186,6 +196,60 @@ def test_slab_edge_shapes(engine):
         net.set_input_dense(y0)
         net.upload()
         out = cd.run([net], cd.default_opts(precision=cd.FP16, spmm_slab=1))[0]
-        assert set(out.layer_kernel.tolist()) == {cd.KERNEL_SLAB}
+        expect_slab(out, N, True)
         assert np.array_equal(out.categories, np.flatnonzero(alive)), (N, L, S, bias)
         net.close()
+
+
+@pytest.mark.parametrize("N,S", [(4096, 3000), (8192, 1500), (16384, 1100), (32768, 500), (65536, 300)],
+                         ids=lambda v: str(v))
+def test_bits_layer0_equals_the_slab_kernel(engine, N, S):
+    """Layer 0 on the bit-packed input by bit-sliced counts (W = 16 P samples a slab: 64 at
+    N <= 16384, 32 at 32768, 16 at 65536; S not a multiple of W, so the last slab is partial):
+    its activations equal the slab kernel's on the FP16 input bit for bit, with layer 1 on the
+    slab kernel (sample-blocked output) and on the MFMA kernel (neuron-major output);
+    categories equal to the oracle's; a layer 0 with a column of more than 63 rows stays on
+    the slab kernel."""
+    L, bias, seed = 4, -0.4 if N >= 16384 else -0.35, 131 + N // 1024
+    alive, _ = orc.forward_fp64(orc.OracleNet.generated(N, L, bias, cd.GEN_RANDOM, seed), 0, S)
+    want = np.flatnonzero(alive)
+    net = gen_net(engine, N, L, S, bias, cd.GEN_RANDOM, seed)
+    slab = cd.LayerPlan(cd.PLAN_SLAB, 0, 1)
+    mfma = cd.LayerPlan(cd.PLAN_MFMA, 1, 4)
+    for name, plan in (("slab after", [slab] * L), ("mfma after", [slab, mfma, mfma, slab])):
+        net.set_plan(plan)
+        outs = {}
+        for bits in (1, 0):
+            out = cd.run([net], cd.default_opts(precision=cd.FP16, input_bits=bits, dump_layer=0), dump_cap_rows=S)[0]
+            assert out.layer_kernel[0] == (cd.KERNEL_SLAB_BITS if bits else cd.KERNEL_SLAB), (name, bits)
+            assert np.array_equal(out.categories, want), (name, bits)
+            outs[bits] = out
+        a, b = outs[1], outs[0]
+        say(f"  N={N} {name}: categories={len(a.categories)} dumped rows={a.dump_rows} ms {a.ms_total:.2f} "
+            f"(FP16 input {b.ms_total:.2f})")
+        assert np.array_equal(a.dump_ids, b.dump_ids)
+        assert np.array_equal(a.dump_vals, b.dump_vals), name
+        assert np.array_equal(a.live, b.live)
+    net.close()
+    # a column of 80 input rows in layer 0: beyond the 6 count planes, the slab kernel instead
+    rng = np.random.default_rng(N)
+    layers = []
+    for l in range(2):
+        counts = np.full(N, 32)
+        if l == 0:
+            counts[5] = 80
+        cp = np.zeros(N + 1, np.int32)
+        cp[1:] = np.cumsum(counts)
+        rows = np.concatenate([np.sort(rng.choice(N, c, replace=False)) for c in counts]).astype(np.int32)
+        layers.append((cp, rows, np.full(len(rows), 0.0625, np.float32)))
+    y0 = (rng.random((200, N)) < 0.3).astype(np.float32)
+    alive, _ = orc.forward_fp64(orc.OracleNet.explicit(N, bias, layers, y0.astype(np.uint8)), 0, 200)
+    net = cd.Net(engine, N, 2, 200, bias)
+    for l, (cp, rw, vl) in enumerate(layers):
+        net.set_layer_csc(l, cp, rw, vl)
+    net.set_input_dense(y0)
+    net.upload()
+    out = cd.run([net], cd.default_opts(precision=cd.FP16, spmm_slab=1))[0]
+    assert out.layer_kernel[0] == cd.KERNEL_SLAB
+    assert np.array_equal(out.categories, np.flatnonzero(alive))
+    net.close()
