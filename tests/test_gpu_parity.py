@@ -551,7 +551,8 @@ def test_c4_shape_on_a_sample_slice(engine):
     for graph in (0, 1):
         out = cd.run([net], cd.default_opts(precision=cd.FP16, use_graph=graph))[0]
         say(f"  all slab, graph={graph}: categories={len(out.categories)} ms={out.ms_total:.2f}")
-        assert set(out.layer_kernel.tolist()) == {cd.KERNEL_SLAB}
+        ks = out.layer_kernel.tolist()  # layer 0 on the bit-packed input: the bit-sliced variant
+        assert ks[0] == cd.KERNEL_SLAB_BITS and set(ks[1:]) == {cd.KERNEL_SLAB}
         assert np.array_equal(out.categories, want), graph
         assert_live_close(out.live, live, out)
     plan, _, _ = cd.tune_exhaustive(net, reps=1)

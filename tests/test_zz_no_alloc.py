@@ -81,7 +81,8 @@ def test_noalloc_slab_layers_next_to_mfma_layers(engine):
     got = cd.run_noalloc(net, cd.default_opts(precision=cd.FP16))
     say(f"N={N} alternating slab/MFMA: categories {len(got.categories)} ms {got.ms_total:.2f} "
         f"(cospdnn_run {ref.ms_total:.2f})")
-    assert set(got.layer_kernel.tolist()) == {cd.KERNEL_SLAB, cd.KERNEL_MFMA_MASK}
+    # layer 0 reads the bit-packed input: the slab plan's bit-sliced variant there
+    assert set(got.layer_kernel.tolist()) == {cd.KERNEL_SLAB_BITS, cd.KERNEL_SLAB, cd.KERNEL_MFMA_MASK}
     check_same(got, ref, np.flatnonzero(alive))
     net.close()
 
