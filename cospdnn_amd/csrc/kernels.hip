@@ -2060,16 +2060,9 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
     uint32_t fa[NW], fu[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) fa[j] = fu[j] = 0u;
-    constexpr int kAhead = 3;
     SlabCur cur, ahead;
     slab_cur_batch<T>(cur, b0, bhi, lane, sgoff);
     ahead = cur;
-    SlabOct<M> ring[kAhead];
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u) {
-      ring[u] = slab_oct_load<T, true>(w, ahead);
-      slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
-    }
     uint32_t cnt[M][6][NW];
 #pragma unroll
     for (int m = 0; m < M; ++m)
@@ -2077,7 +2070,7 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
       for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int j = 0; j < NW; ++j) cnt[m][k][j] = 0u;
-    auto step = [&](SlabOct<M>& slot) {
+    auto step = [&](const SlabOct<M>& slot) {
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         const uint32_t dw[4] = {slot.id[m].x, slot.id[m].y, slot.id[m].z, slot.id[m].w};
@@ -2124,89 +2117,119 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
         }
       }
       if (cur.o + 1 == cur.nk) {
-        // the batch's last octet: per slot and sample the count, w c + b, clamp, FP16, stores
+        // the batch's last octet: per slot and sample the count, w c + b, clamp, FP16, stores.
+        // Samples go in groups of 8 through a loop that is not unrolled (code size, below).
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           const int s = cur.b * Sh::kBatch + m * 64 + lane;
           if (s < nout) {
-            uint32_t d[W / 2];
-#pragma unroll
-            for (int t = 0; t < W; ++t) {
-              const int j = t >> 5, b = t & 31;
-              uint32_t c = 0;
-#pragma unroll
-              for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(cnt[m][k][j], b, 1) << k;
-              const float z = __builtin_fmaf(static_cast<float>(c), wuni, bias);
-              const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
-              const uint32_t h = __builtin_bit_cast(uint16_t, yv);
-              // flags from the FP16 value, as the slab kernel takes them (y in [0, 32]): alive when
-              // its bits are nonzero, unsaturated below 0x5000 (32.0); integer ops only
-              fa[j] |= bits_alive(h) << b;
-              fu[j] |= bits_unsat(h) << b;
-              if (t & 1)
-                d[t >> 1] |= h << 16;
-              else
-                d[t >> 1] = h;
-            }
-            if (q0 + W <= n_out) {
-              if (y_slab) {
-                // groups of P samples: the group of position q0 + g P is slab (q0 + g P) / T, plane
-                // ((g P) % T) / P, so its element offset grows by P * ry per group; one offset in
-                // VGPRs stepped along (the products per group would otherwise be hoisted into
-                // scalar registers, 16 to 64 of them)
-                int64_t off = slab_index<T>(q0, s, ry);
-                const int64_t step = static_cast<int64_t>(P) * ry;
-#pragma unroll
-                for (int g = 0; g < W / P; ++g) {
-                  _Float16* yg = Y + off;
-                  if constexpr (P == 1)
-                    *reinterpret_cast<uint16_t*>(yg) = static_cast<uint16_t>(d[g >> 1] >> (16 * (g & 1)));
-                  else
-                    slab_store<P>(yg, d + g * (P / 2));
-                  off += step;
-                  asm volatile("" : "+v"(off));
-                }
-              } else {
-                uint4* yp = reinterpret_cast<uint4*>(Y + static_cast<int64_t>(s) * ldy + q0);
-#pragma unroll
-                for (int v = 0; v < W / 8; ++v) yp[v] = make_uint4(d[4 * v], d[4 * v + 1], d[4 * v + 2], d[4 * v + 3]);
-              }
-            } else {
-              // the partial last slab: value by value, recomputed from the count planes
+            const bool full = q0 + W <= n_out;
+            // element offset of the group's first sample; per group + 8 ry (sample-blocked: a
+            // group of 8 holds 8 / P planes-groups of P samples, P ry apart) or + 8 (neuron-major)
+            int64_t off = y_slab ? slab_index<T>(q0, s, ry) : static_cast<int64_t>(s) * ldy + q0;
+            const int64_t gstep = y_slab ? 8 * static_cast<int64_t>(ry) : 8;
 #pragma unroll 1
-              for (int t = 0; t < W && q0 + t < n_out; ++t) {
-                const int jt = t >> 5, bt = t & 31;
+            for (int g = 0; g < W / 8; ++g) {
+              const int j = (8 * g) >> 5, b0 = (8 * g) & 31;
+              uint32_t pl[6];
+#pragma unroll
+              for (int k = 0; k < 6; ++k) pl[k] = j ? cnt[m][k][NW - 1] : cnt[m][k][0];
+              uint32_t d[4] = {0u, 0u, 0u, 0u}, am = 0u, um = 0u;
+#pragma unroll
+              for (int i = 0; i < 8; ++i) {
                 uint32_t c = 0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(jt ? cnt[m][k][NW - 1] : cnt[m][k][0], bt, 1) << k;
+                for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(pl[k], b0 + i, 1) << k;
                 const float z = __builtin_fmaf(static_cast<float>(c), wuni, bias);
                 const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
-                const int64_t at = y_slab ? slab_index<T>(q0 + t, s, ry) : static_cast<int64_t>(s) * ldy + q0 + t;
-                Y[at] = yv;
+                const uint32_t h = __builtin_bit_cast(uint16_t, yv);
+                // flags from the FP16 value, as the slab kernel takes them (y in [0, 32]): alive
+                // when its bits are nonzero, unsaturated below 0x5000 (32.0); integer ops only
+                am |= bits_alive(h) << i;
+                um |= bits_unsat(h) << i;
+                d[i >> 1] |= h << (16 * (i & 1));
               }
+              if (j) {
+                fa[NW - 1] |= am << b0;
+                fu[NW - 1] |= um << b0;
+              } else {
+                fa[0] |= am << b0;
+                fu[0] |= um << b0;
+              }
+              if (full) {
+                if (y_slab) {
+#pragma unroll
+                  for (int h = 0; h < 8 / P; ++h) {
+                    _Float16* yg = Y + off + static_cast<int64_t>(h) * P * ry;
+                    if constexpr (P == 1)
+                      *reinterpret_cast<uint16_t*>(yg) = static_cast<uint16_t>(d[h >> 1] >> (16 * (h & 1)));
+                    else
+                      slab_store<P>(yg, d + h * (P / 2));
+                  }
+                } else {
+                  *reinterpret_cast<uint4*>(Y + off) = make_uint4(d[0], d[1], d[2], d[3]);
+                }
+              } else {
+                // the partial last slab: value by value
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                  const int t = 8 * g + i;
+                  if (q0 + t < n_out) {
+                    const int64_t at = y_slab ? slab_index<T>(q0 + t, s, ry) : static_cast<int64_t>(s) * ldy + q0 + t;
+                    *reinterpret_cast<uint16_t*>(Y + at) = static_cast<uint16_t>(d[i >> 1] >> (16 * (i & 1)));
+                  }
+                }
+              }
+              off += gstep;
             }
           }
 #pragma unroll
           for (int k = 0; k < 6; ++k)
 #pragma unroll
-            for (int j = 0; j < NW; ++j) cnt[m][k][j] = 0u;
+            for (int jj = 0; jj < NW; ++jj) cnt[m][k][jj] = 0u;
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       }
-      slot = slab_oct_load<T, true>(w, ahead);
-      slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
       slab_cur_next<T>(cur, bhi, lane, sgoff, bstride);
     };
     int nsteps = 0;
     for (int b = b0; b < bhi; b += bstride) nsteps += sgoff[b + 1] - sgoff[b];
-    int k = 0;
-    for (; k + kAhead <= nsteps; k += kAhead) {
+    if constexpr (W <= 16) {
+      // 16 samples a slab (~200 VALU instructions a step): three octets in flight, in unrolled
+      // rounds of the ring (the body stays small enough to copy)
+      constexpr int kAhead = 3;
+      SlabOct<M> ring[kAhead];
 #pragma unroll
-      for (int u = 0; u < kAhead; ++u) step(ring[u]);
+      for (int u = 0; u < kAhead; ++u) {
+        ring[u] = slab_oct_load<T, true>(w, ahead);
+        slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+      }
+      int k = 0;
+      for (; k + kAhead <= nsteps; k += kAhead) {
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+          step(ring[u]);
+          ring[u] = slab_oct_load<T, true>(w, ahead);
+          slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kAhead - 1; ++u)
+        if (k + u < nsteps) step(ring[u]);
+    } else {
+      // 32 or 64 samples a slab: one octet in flight (a step's ~300-600 VALU instructions hide
+      // its load) and a single copy of the body: the ring's copies of the epilogue overflowed
+      // the instruction cache (the kernel ran 6x slower at 64 samples)
+      SlabOct<M> nxt = slab_oct_load<T, true>(w, ahead);
+      slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+#pragma unroll 1
+      for (int k = 0; k < nsteps; ++k) {
+        const SlabOct<M> slot = nxt;
+        nxt = slab_oct_load<T, true>(w, ahead);
+        slab_cur_next<T>(ahead, bhi, lane, sgoff, bstride);
+        step(slot);
+      }
     }
-#pragma unroll
-    for (int u = 0; u < kAhead - 1; ++u)
-      if (k + u < nsteps) step(ring[u]);
     // flags: OR over the wave's lanes, then over the work-group
 #pragma unroll
     for (int j = 0; j < NW; ++j) {
