@@ -2134,13 +2134,26 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
               uint32_t pl[6];
 #pragma unroll
               for (int k = 0; k < 6; ++k) pl[k] = j ? cnt[m][k][NW - 1] : cnt[m][k][0];
+              // the group's byte of each plane as the rows of an 8x8 bit matrix (v_perm_b32 picks
+              // byte b of two words), transposed by three delta swaps: byte i of the result is
+              // sample i's count (bit k from plane k)
+              const uint32_t b = static_cast<uint32_t>(b0 >> 3);
+              const uint32_t sel_lo = b | ((4u + b) << 8) | 0x0C0C0000u, sel_hi = 0x0C0Cu | (b << 16) | ((4u + b) << 24);
+              const uint32_t lo = __builtin_amdgcn_perm(pl[1], pl[0], sel_lo) | __builtin_amdgcn_perm(pl[3], pl[2], sel_hi);
+              const uint32_t hi = __builtin_amdgcn_perm(pl[5], pl[4], sel_lo);
+              uint64_t x = (static_cast<uint64_t>(hi) << 32) | lo;
+              uint64_t tt = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+              x = x ^ tt ^ (tt << 7);
+              tt = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+              x = x ^ tt ^ (tt << 14);
+              tt = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+              x = x ^ tt ^ (tt << 28);
+              const uint32_t cw[2] = {static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32)};
               uint32_t d[4] = {0u, 0u, 0u, 0u}, am = 0u, um = 0u;
 #pragma unroll
               for (int i = 0; i < 8; ++i) {
-                uint32_t c = 0;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) c |= __builtin_amdgcn_ubfe(pl[k], b0 + i, 1) << k;
-                const float z = __builtin_fmaf(static_cast<float>(c), wuni, bias);
+                const float c = static_cast<float>((cw[i >> 2] >> (8 * (i & 3))) & 0xFFu);  // v_cvt_f32_ubyte
+                const float z = __builtin_fmaf(c, wuni, bias);
                 const _Float16 yv = static_cast<_Float16>(__builtin_amdgcn_fmed3f(z, 0.0f, 32.0f));
                 const uint32_t h = __builtin_bit_cast(uint16_t, yv);
                 // flags from the FP16 value, as the slab kernel takes them (y in [0, 32]): alive
