@@ -170,7 +170,11 @@ def test_keep_threshold_tuning(engine):
 
 def test_mlp_plan_regret_on_an_unseen_network(engine):
     """Train the cost model on timings of some networks, plan a network it never saw, and
-    compare with exhaustive tuning (the yardstick) and with the default choice (no tuner)."""
+    compare with exhaustive tuning (the yardstick) and with the default choice (no tuner).
+    The plan is the per-layer argmin of the mean prediction of three models (seeds 9-11): on
+    this network several layers sit within 20% of each other on two candidates (the slab and
+    band kernels), and one model's choice there flips with the timing noise of its training
+    tables (regrets of 0-21% across runs for a single model)."""
     train_nets = [(1024, 24, 12000, -0.3, cd.GEN_RANDOM, 31), (4096, 24, 12000, -0.35, cd.GEN_RANDOM, 32),
                   (4096, 24, 12000, -0.35, cd.GEN_BANDED, 33), (2048, 24, 20000, -0.3, cd.GEN_RANDOM, 34),
                   (8192, 16, 8000, -0.4, cd.GEN_BANDED, 35)]
@@ -184,14 +188,23 @@ def test_mlp_plan_regret_on_an_unseen_network(engine):
         net.close()
     X = np.concatenate(Xs)
     y = np.concatenate(ys)
-    m = cd.Mlp(engine, cd.TUNE_FEATURES, 64, seed=9)
-    loss = m.train(X, y, steps=3000, lr=2e-3)
-    say(f"cost model: {len(y)} samples, training loss {loss:.4f} (ln-time units)")
-    assert loss < 0.05
+    models = []
+    for seed in (9, 10, 11):
+        m = cd.Mlp(engine, cd.TUNE_FEATURES, 64, seed=seed)
+        loss = m.train(X, y, steps=3000, lr=2e-3)
+        say(f"cost model (seed {seed}): {len(y)} samples, training loss {loss:.4f} (ln-time units)")
+        assert loss < 0.05
+        models.append(m)
 
     net = make_net(engine, 3072, 24, 16000, -0.33, cd.GEN_RANDOM, 36)
     ex_plan, table, live = cd.tune_exhaustive(net, reps=2)
-    mlp_plan, pred = m.plan(net, live)
+    pred = np.mean([m.plan(net, live)[1] for m in models], axis=0)
+    cands = cd.candidates()
+    # as the library's planner: a layer without a tile image takes only the SIMT candidates
+    for l in range(net.L):
+        if net.layer_features(l, net.S, (cd.PLAN_SIMT, 0, 0))[3] <= 0:
+            pred[[k for k, c in enumerate(cands) if c[0] != cd.PLAN_SIMT], l] = np.inf
+    mlp_plan = [cands[int(k)] for k in np.argmin(pred, axis=0)]
     t_ex = plan_spmm_ms(net, ex_plan)
     t_mlp = plan_spmm_ms(net, mlp_plan)
     t_def = plan_spmm_ms(net, default_plan(net))
