@@ -265,6 +265,7 @@ struct RunCfg {
   int bits = 0;              // FP16 pass: an MFMA layer 0 reads the bit-packed copy of a binary input
   int wmask = 0;             // FP16 pass: MFMA layers read the mask image where the layer has one
   int slab = 0;              // FP16 pass: layers with a slab image run the sample-slab kernel
+  int shared = 0;            // other passes run beside this one from its first layer on (batch chains)
 };
 
 constexpr int64_t kFewRows = 512;  // passes up to this many rows use the few-row SIMT kernel
@@ -583,6 +584,10 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
         }
         // one work-group per CU; with fewer slabs than CUs the kernel splits each slab's outputs
         // over several work-groups (it knows the live row count, the host only its bound)
+        // One sample per slab: the 768-thread kernel on the layers before the checkpoint, where
+        // nothing runs beside the pass (the guard tiers fork at the checkpoint; batch chains
+        // share the device throughout); its work-groups would not share a CU with them.
+        a.order = !cfg.shared && D > 0 && l < D ? kSlabWideOrder : 1;
         HIP_TRY(launch_spmm_slab(a, view, capg(e->cu_count), st));
         if (conv_out) {
           ConvArgs c;
@@ -1440,6 +1445,8 @@ int run_batch(cospdnn_net* const* nets, int n, const cospdnn_opts& o, cospdnn_re
   const int chains = std::max(1, std::min({n, o.batch_concurrent, kMaxSideStreams / 2}));
   if (o.batch_cu_share != 0 && chains > 1)
     for (int i = 0; i < n; ++i) cfg[i].grid_div = chains;
+  if (chains > 1)
+    for (int i = 0; i < n; ++i) cfg[i].shared = 1;
   // chain c runs its networks' main passes on side[c] and their guard tiers on side[chains + c]
   CALL_TRY(ensure_side_streams(e, 2 * chains));
   std::vector<char> dev_guard(n, 0);

@@ -1419,7 +1419,7 @@ __device__ __forceinline__ int64_t slab_index(int p, int r, int R) {
 // Slab sl (positions sl*T ..: input positions = output positions, no compaction) into the LDS
 // planes straight from memory: the bit-packed layer-0 input or T values of every neuron-major
 // row (through registers).  Plane k at half k * PB/2, row r at + r * P.
-template <int T>
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void slab_load_rows(const StepArgs& a, lds_half_t* lds, int sl, int x_rows, int64_t ldx,
                                                int tid) {
   constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
@@ -1427,7 +1427,7 @@ __device__ __forceinline__ void slab_load_rows(const StepArgs& a, lds_half_t* ld
   if (a.xbits != nullptr) {
     // 64-sample tile t, row r: words (t * xbits_rows + r) * 2 + h, bit i of word h = sample 64t + 32h + i
     const int t = p0 >> 6, o = p0 & 63;
-    for (int r = tid; r < x_rows; r += kSlabThreads) {
+    for (int r = tid; r < x_rows; r += NT) {
       const uint32_t* wp = a.xbits + (static_cast<int64_t>(t) * a.xbits_rows + r) * 2;
       uint32_t bits[2];
       if constexpr (T == 64) {
@@ -1462,14 +1462,14 @@ __device__ __forceinline__ void slab_load_rows(const StepArgs& a, lds_half_t* ld
       // 16-byte pieces of a row (8 samples = two planes)
       constexpr int ppr = T / 8;
       const int pieces = x_rows * ppr;
-      for (int i = tid; i < pieces; i += kSlabThreads) {
+      for (int i = tid; i < pieces; i += NT) {
         const int r = i / ppr, part = i % ppr;
         const uint4 v = *reinterpret_cast<const uint4*>(X + static_cast<int64_t>(r) * ldx + p0 + 8 * part);
         lds_put8(lds + (2 * part) * PH + r * 4, v.x, v.y);
         lds_put8(lds + (2 * part + 1) * PH + r * 4, v.z, v.w);
       }
     } else {
-      for (int r = tid; r < x_rows; r += kSlabThreads) {
+      for (int r = tid; r < x_rows; r += NT) {
         const _Float16* src = X + static_cast<int64_t>(r) * ldx + p0;
         if constexpr (T == 4) {
           const uint2 v = *reinterpret_cast<const uint2*>(src);
@@ -1486,12 +1486,12 @@ __device__ __forceinline__ void slab_load_rows(const StepArgs& a, lds_half_t* ld
 
 // A sample-blocked slab (T / P planes of slab_rows(x_rows) rows, contiguous in HBM) straight into
 // the LDS planes, 16 bytes a lane (global_load_lds: the lanes of a wave fill 1 KB, pieces
-// k0 = w * 64 + k * kSlabThreads on), no registers held.  The loads are non-temporal (nt): a
+// k0 = w * 64 + k * NT on, NT threads), no registers held.  The loads are non-temporal (nt): a
 // slab is read once per work-group, and kept out of the L2 it leaves the slab image there, which
 // every slab of the layer reads again (c2 forward 24.3 -> 23.5 ms; non-temporal epilogue stores,
 // partial lines, took that back).  The caller waits for vmcnt(0) before the barrier that
 // publishes it.
-template <int T>
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void slab_load_blocked(const StepArgs& a, lds_half_t* lds, int sl, int x_rows, int wave,
                                                   int lane) {
   constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
@@ -1502,7 +1502,7 @@ __device__ __forceinline__ void slab_load_blocked(const StepArgs& a, lds_half_t*
   for (int k = 0; k < T / P; ++k) {
     const _Float16* sp = src + static_cast<int64_t>(k) * R * P;
     lds_half_t* dp = lds + k * PH;
-    for (int k0 = wave * 64; k0 < pp; k0 += kSlabThreads) {
+    for (int k0 = wave * 64; k0 < pp; k0 += NT) {
       const int i = k0 + lane;
       if (i < pp) __builtin_amdgcn_global_load_lds(sp + 8 * i, (lds_ptr_t)(dp + 8 * k0), 16, 0, 2);
     }
@@ -1512,12 +1512,12 @@ __device__ __forceinline__ void slab_load_blocked(const StepArgs& a, lds_half_t*
 // The slab of output positions q0 = sl*T .. after a compacting scan: sample t is input position
 // psrc[t] (-1: past the live rows, loaded as zeros), gathered value by value from the input
 // (sample-blocked, neuron-major or bit-packed).  Thread (t, r): t = tid % T, rows r = tid / T +
-// k * (kSlabThreads / T); neighbouring threads read neighbouring positions of one row.
-template <int T>
+// k * (NT / T) (NT threads); neighbouring threads read neighbouring positions of one row.
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* lds, const int* psrc, int x_rows,
                                                  int64_t ldx, bool x_slab, int tid) {
   constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
-  constexpr int kRowsPer = kSlabThreads / T;
+  constexpr int kRowsPer = NT / T;
   const int t = tid % T;
   const int p = psrc[t];
   const _Float16* __restrict__ X = static_cast<const _Float16*>(a.x);
@@ -1557,7 +1557,7 @@ __device__ __forceinline__ void slab_load_gather(const StepArgs& a, lds_half_t* 
 // per thread, and the sample's values picked out into its LDS plane.  Rows past x_rows (up to
 // slab_rows) take whatever the input's pad rows hold; the slab image never reads them.  The
 // loads are non-temporal, as in slab_load_blocked (c2 SpMM 21.7 -> 21.5 ms).
-template <int T>
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void slab_load_gather_blocked(const StepArgs& a, lds_half_t* lds, const int* psrc,
                                                          int x_rows, int tid) {
   constexpr int P = SlabShape<T>::kP, PH = SlabShape<T>::kPb / 2;
@@ -1572,7 +1572,7 @@ __device__ __forceinline__ void slab_load_gather_blocked(const StepArgs& a, lds_
 #pragma unroll
     for (int u = 0; u < kU; ++u) src[u] = psrc[t0 + u];
 #pragma unroll 1
-    for (int c = tid; c < pieces; c += kSlabThreads) {
+    for (int c = tid; c < pieces; c += NT) {
       uint4 v[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
@@ -1668,7 +1668,7 @@ __device__ __forceinline__ SlabOct<SlabShape<T>::kM> slab_oct_load(const DevLaye
   return k;
 }
 
-template <int T, bool UNI, bool BAND>
+template <int T, bool UNI, bool BAND, int NT = kSlabThreads>
 __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, lds_half_t* lds) {
   using Sh = SlabShape<T>;
   constexpr int P = Sh::kP, SPL = Sh::kSpl, LPO = Sh::kLpo, OPW = Sh::kOpw, NR = Sh::kReads, PB = Sh::kPb, M = Sh::kM;
@@ -1718,21 +1718,21 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
   const uint32_t lane_base = static_cast<uint32_t>(g * NR * PB);
   // the zero row of every plane (pads of shorter lists)
   const int zr = slab_rows(x_rows);
-  for (int i = tid; i < T; i += kSlabThreads) lds[(i / P) * (PB / 2) + zr * P + i % P] = static_cast<_Float16>(0.0f);
+  for (int i = tid; i < T; i += NT) lds[(i / P) * (PB / 2) + zr * P + i % P] = static_cast<_Float16>(0.0f);
   auto load_slab = [&](int sl) {
     if (ident) {
       if (x_slab)
-        slab_load_blocked<T>(a, lds, sl, x_rows, wave, lane);
+        slab_load_blocked<T, NT>(a, lds, sl, x_rows, wave, lane);
       else
-        slab_load_rows<T>(a, lds, sl, x_rows, ldx, tid);
+        slab_load_rows<T, NT>(a, lds, sl, x_rows, ldx, tid);
     } else if (T == 1 && x_slab) {
       // one sample per slab: the sample's rows are one contiguous block of the sample-blocked
       // input (slab psrc[0] of it), loaded whole instead of value by value
-      slab_load_blocked<T>(a, lds, __builtin_amdgcn_readfirstlane(psrc[0]), x_rows, wave, lane);
+      slab_load_blocked<T, NT>(a, lds, __builtin_amdgcn_readfirstlane(psrc[0]), x_rows, wave, lane);
     } else if (x_slab) {
-      slab_load_gather_blocked<T>(a, lds, psrc, x_rows, tid);
+      slab_load_gather_blocked<T, NT>(a, lds, psrc, x_rows, tid);
     } else {
-      slab_load_gather<T>(a, lds, psrc, x_rows, ldx, x_slab, tid);
+      slab_load_gather<T, NT>(a, lds, psrc, x_rows, ldx, x_slab, tid);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   };
@@ -1754,7 +1754,7 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
     const int64_t ub = static_cast<int64_t>(sl) * nb;
     const int blo = static_cast<int>((u0 > ub ? u0 : ub) - ub);
     const int bhi = static_cast<int>((u1 < ub + nb ? u1 : ub + nb) - ub);
-    const int b0 = blo + wave, bstride = kSlabWaves;
+    const int b0 = blo + wave, bstride = NT / 64;
     // per lane: packed FP16 max (alive: > 0) and min (unsaturated: < 32) of its samples over
     // its slots, reduced to flag bits once per slab
     uint32_t fmx[ND], fmn[ND];
@@ -1765,7 +1765,7 @@ __device__ __forceinline__ void slab_body(const StepArgs& a, const DevLayer& w, 
     }
     uint32_t fb = 0;
     // The wave's steps, one octet (8 nonzeros of each of the lane's M slots, 128 FMAs a lane) per
-    // step: batches b = wave, wave + kSlabWaves, ... (every batch has at least one octet),
+    // step: batches b = wave, wave + NT / 64, ... (every batch has at least one octet),
     // octets in order.  Row ids (and values) sit in a ring of kSlabAhead entries: step k
     // processes entry k % kSlabAhead, then reloads it for step k + kSlabAhead; the loop is
     // unrolled over the ring (no register copies that would wait early).  The epilogue's stores
@@ -1940,6 +1940,14 @@ SLAB_KERNEL(8)
 SLAB_KERNEL(4)
 SLAB_KERNEL(2)
 SLAB_KERNEL(1)
+// One sample per slab with 12 waves (3 per SIMD at 168 VGPRs; same slab image): faster alone
+// (one more wave per SIMD to overlap LDS reads with FMAs), but its work-group then takes its
+// CU's whole register file and cannot share the CU with kernels running beside the pass.
+constexpr int kSlabWideThreads = 768;
+__global__ __launch_bounds__(kSlabWideThreads, 1) void spmm_slab_uni_wide_kernel_1(StepArgs a, DevLayer w) {
+  extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];
+  slab_body<1, true, false, kSlabWideThreads>(a, w, (lds_half_t*)slab_lds);
+}
 #undef SLAB_KERNEL
 
 // --------------------------------------------------------------------------
@@ -2889,7 +2897,16 @@ hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipS
     SLAB_LAUNCH(8)
     SLAB_LAUNCH(4)
     SLAB_LAUNCH(2)
-    SLAB_LAUNCH(1)
+    case 1:
+      if (a.band_next != nullptr)
+        hipLaunchKernelGGL(spmm_slab_band_kernel_1, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);
+      else if (uni && a.order == kSlabWideOrder)
+        hipLaunchKernelGGL(spmm_slab_uni_wide_kernel_1, dim3(grid), dim3(kSlabWideThreads), bytes, st, a, w);
+      else if (uni)
+        hipLaunchKernelGGL(spmm_slab_uni_kernel_1, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);
+      else
+        hipLaunchKernelGGL(spmm_slab_kernel_1, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -3000,6 +3017,7 @@ hipError_t configure_kernels() {
       reinterpret_cast<const void*>(spmm_slab_band_kernel_##T_)
   const void* slab_kernels[] = {SLAB_KERNELS(64), SLAB_KERNELS(32), SLAB_KERNELS(16), SLAB_KERNELS(8),
                                 SLAB_KERNELS(4),  SLAB_KERNELS(2),  SLAB_KERNELS(1),
+                                reinterpret_cast<const void*>(spmm_slab_uni_wide_kernel_1),
                                 reinterpret_cast<const void*>(spmm_bits_kernel_16),
                                 reinterpret_cast<const void*>(spmm_bits_kernel_8),
                                 reinterpret_cast<const void*>(spmm_bits_kernel_4),

@@ -216,6 +216,10 @@ hipError_t launch_spmm_mfma_band(const StepArgs& a, const DevLayer& w, int grid,
 // Sample-slab kernel (FP16 pass; slab image present, slab_samples(a.x_rows) > 0, 512 threads,
 // one work-group per CU); any grid (the work is split into one balanced range per work-group).
 hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st);
+// a.order for launch_spmm_slab: at one sample per slab with one weight value and no guard-band
+// flags, the 768-thread variant (faster alone; its work-group fills its CU's registers, so it
+// suits layers with nothing running beside the pass).  Any other value: the 512-thread kernel.
+constexpr int kSlabWideOrder = 7;
 // Layer 0 of a slab plan on the bit-packed binary input (a.xbits) with one weight value (slab image
 // without values, slab_samples(a.x_rows) <= 16, lists of at most 63 rows): per output the count
 // of set input rows by bit-sliced adders, y written in a.y_layout (sample-blocked by the same slab
