@@ -529,7 +529,8 @@ def test_c4_shape_on_a_sample_slice(engine):
     """The largest network (N=65536, L=480: ~1e9 weights) on the first 12000 samples (188
     64-sample tiles per layer at first, 4 or fewer after layer 7), with the tile images the
     bench uses (waste ~32): the default MFMA launch, the row-gather kernel on every layer,
-    the sample-slab kernel on every layer (one sample per slab), and the tuned per-layer plan
+    the sample-slab kernel on every layer (one sample per slab: the 768-thread variant on the
+    layers before the checkpoint, layer 12, the 512-thread one after), and the tuned per-layer plan
     (the bench's: MFMA on layer 0, the slab kernel after it), eager and graph-replayed;
     categories equal to the FP64 oracle's."""
     spec = cd.config_nets("c4")[0]
