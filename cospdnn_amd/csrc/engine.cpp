@@ -542,6 +542,8 @@ int enqueue_forward(cospdnn_net* net, RunBufs& rb, const _Float16* input, int64_
           a.x_rows = input_rows(net, l);
           a.x_layout = kLayoutRows;
           a.y_layout = l + 1 < L && pick_kernel(net, cfg, l + 1, S_run) == COSPDNN_KERNEL_SLAB ? kLayoutSlab : kLayoutRows;
+          // the wide variant where nothing runs beside the pass (as for the slab kernel below)
+          a.order = !cfg.shared && D > 0 && l < D ? kSlabWideOrder : 1;
           HIP_TRY(launch_spmm_bits(a, view, capg(e->cu_count), st));
           break;
         }

@@ -1983,7 +1983,7 @@ __device__ __forceinline__ void bits_fa(uint32_t a, uint32_t b, uint32_t c, uint
 // W input bits (2 P bytes) of row r for the slab of positions q0.. into LDS, ident layout: the
 // bit-packed input holds, per 64-sample tile t and row r, 64 bits at byte (t * rows + r) * 8
 // (bit i = sample 64 t + i); gathered: bit by bit through psrc.
-template <int T>
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void bits_load(const StepArgs& a, lds_half_t* lds, int q0, const int* psrc, bool ident,
                                           int x_rows, int tid) {
   constexpr int P = SlabShape<T>::kP, W = 16 * P;
@@ -1992,7 +1992,7 @@ __device__ __forceinline__ void bits_load(const StepArgs& a, lds_half_t* lds, in
     const int t = q0 >> 6, k = (q0 & 63) / W;
     const char* base = reinterpret_cast<const char*>(a.xbits) + static_cast<int64_t>(t) * xr * 8 + k * 2 * P;
 #pragma unroll 4
-    for (int r = tid; r < x_rows; r += kSlabThreads) {
+    for (int r = tid; r < x_rows; r += NT) {
       const char* src = base + static_cast<int64_t>(r) * 8;
       if constexpr (P == 4) {
         const uint2 v = *reinterpret_cast<const uint2*>(src);
@@ -2004,7 +2004,7 @@ __device__ __forceinline__ void bits_load(const StepArgs& a, lds_half_t* lds, in
       }
     }
   } else {
-    for (int r = tid; r < x_rows; r += kSlabThreads) {
+    for (int r = tid; r < x_rows; r += NT) {
       uint32_t wd[2] = {0u, 0u};
       for (int i = 0; i < W; ++i) {
         const int p = psrc[i];
@@ -2023,7 +2023,7 @@ __device__ __forceinline__ void bits_load(const StepArgs& a, lds_half_t* lds, in
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-template <int T>
+template <int T, int NT = kSlabThreads>
 __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, lds_half_t* lds) {
   using Sh = SlabShape<T>;
   static_assert(Sh::kLpo == 1, "one lane per output slot");
@@ -2061,7 +2061,7 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
   const int sl_last = u1 > u0 ? static_cast<int>((u1 - 1) / nb) : sl - 1;
   set_src(sl);
   __syncthreads();  // psrc of the first slab, the zero row
-  if (sl <= sl_last) bits_load<T>(a, lds, sl * W, psrc, ident, x_rows, tid);
+  if (sl <= sl_last) bits_load<T, NT>(a, lds, sl * W, psrc, ident, x_rows, tid);
   for (int it = 0; sl <= sl_last; ++sl, ++it) {
     const int par = it & 1;
     if (tid < 4) sflag[par * 4 + tid] = 0u;
@@ -2070,7 +2070,7 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
     const int64_t ub = static_cast<int64_t>(sl) * nb;
     const int blo = static_cast<int>((u0 > ub ? u0 : ub) - ub);
     const int bhi = static_cast<int>((u1 < ub + nb ? u1 : ub + nb) - ub);
-    const int b0 = blo + wave, bstride = kSlabWaves;
+    const int b0 = blo + wave, bstride = NT / 64;
     uint32_t fa[NW], fu[NW];
 #pragma unroll
     for (int j = 0; j < NW; ++j) fa[j] = fu[j] = 0u;
@@ -2284,15 +2284,23 @@ __device__ __forceinline__ void bits_body(const StepArgs& a, const DevLayer& w, 
         set_src(sl + 1);
         __syncthreads();  // psrc of the next slab (gathered loads read it)
       }
-      bits_load<T>(a, lds, (sl + 1) * W, psrc, ident, x_rows, tid);
+      bits_load<T, NT>(a, lds, (sl + 1) * W, psrc, ident, x_rows, tid);
     }
   }
 }
 
+// The wide variants (bits_wide_threads(T): 768 at T = 1, 1024 above, within the registers each
+// needs) overlap LDS reads and adds with more waves, but take whole CUs: for layers with nothing
+// running beside the pass (a.order = kSlabWideOrder).
 #define BITS_KERNEL(T_)                                                                                 \
   __global__ __launch_bounds__(kSlabThreads, 1) void spmm_bits_kernel_##T_(StepArgs a, DevLayer w) {     \
     extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
     bits_body<T_>(a, w, (lds_half_t*)slab_lds);                                                         \
+  }                                                                                                     \
+  __global__ __launch_bounds__(bits_wide_threads(T_), 1) void spmm_bits_wide_kernel_##T_(StepArgs a,    \
+                                                                                     DevLayer w) {     \
+    extern __shared__ __attribute__((aligned(16))) _Float16 slab_lds[];                                 \
+    bits_body<T_, bits_wide_threads(T_)>(a, w, (lds_half_t*)slab_lds);                                  \
   }
 BITS_KERNEL(16)
 BITS_KERNEL(8)
@@ -2919,13 +2927,23 @@ hipError_t launch_spmm_bits(const StepArgs& a, const DevLayer& w, int grid, hipS
       (a.y_layout == kLayoutSlab && slab_samples(w.n_out) != t))
     return hipErrorInvalidValue;
   constexpr size_t bytes = kSlabMaxBytes + kSlabMiscBytes;
+  const bool wide = a.order == kSlabWideOrder;
+#define BITS_LAUNCH(T_)                                                                                       \
+  case T_:                                                                                                    \
+    if (wide)                                                                                                 \
+      hipLaunchKernelGGL(spmm_bits_wide_kernel_##T_, dim3(grid), dim3(bits_wide_threads(T_)), bytes, st, a, w); \
+    else                                                                                                      \
+      hipLaunchKernelGGL(spmm_bits_kernel_##T_, dim3(grid), dim3(kSlabThreads), bytes, st, a, w);            \
+    break;
   switch (t) {
-    case 16: hipLaunchKernelGGL(spmm_bits_kernel_16, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
-    case 8: hipLaunchKernelGGL(spmm_bits_kernel_8, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
-    case 4: hipLaunchKernelGGL(spmm_bits_kernel_4, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
-    case 2: hipLaunchKernelGGL(spmm_bits_kernel_2, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
-    default: hipLaunchKernelGGL(spmm_bits_kernel_1, dim3(grid), dim3(kSlabThreads), bytes, st, a, w); break;
+    BITS_LAUNCH(16)
+    BITS_LAUNCH(8)
+    BITS_LAUNCH(4)
+    BITS_LAUNCH(2)
+    BITS_LAUNCH(1)
+    default: return hipErrorInvalidValue;
   }
+#undef BITS_LAUNCH
   return hipGetLastError();
 }
 hipError_t launch_spmm_wave_rows(const StepArgs& a, const DevLayer& w, int grid, hipStream_t st) {
@@ -3022,7 +3040,12 @@ hipError_t configure_kernels() {
                                 reinterpret_cast<const void*>(spmm_bits_kernel_8),
                                 reinterpret_cast<const void*>(spmm_bits_kernel_4),
                                 reinterpret_cast<const void*>(spmm_bits_kernel_2),
-                                reinterpret_cast<const void*>(spmm_bits_kernel_1)};
+                                reinterpret_cast<const void*>(spmm_bits_kernel_1),
+                                reinterpret_cast<const void*>(spmm_bits_wide_kernel_16),
+                                reinterpret_cast<const void*>(spmm_bits_wide_kernel_8),
+                                reinterpret_cast<const void*>(spmm_bits_wide_kernel_4),
+                                reinterpret_cast<const void*>(spmm_bits_wide_kernel_2),
+                                reinterpret_cast<const void*>(spmm_bits_wide_kernel_1)};
 #undef SLAB_KERNELS
   for (const void* k : slab_kernels) {
     const hipError_t e5 =

@@ -220,6 +220,9 @@ hipError_t launch_spmm_slab(const StepArgs& a, const DevLayer& w, int grid, hipS
 // flags, the 768-thread variant (faster alone; its work-group fills its CU's registers, so it
 // suits layers with nothing running beside the pass).  Any other value: the 512-thread kernel.
 constexpr int kSlabWideOrder = 7;
+// threads of the bit-sliced layer-0 kernel's wide variant (launch_spmm_bits with a.order =
+// kSlabWideOrder): 768 at T = 1 (149 VGPRs at 512 threads), 1024 above (at most 122)
+__host__ __device__ constexpr int bits_wide_threads(int t) { return t == 1 ? 768 : 1024; }
 // Layer 0 of a slab plan on the bit-packed binary input (a.xbits) with one weight value (slab image
 // without values, slab_samples(a.x_rows) <= 16, lists of at most 63 rows): per output the count
 // of set input rows by bit-sliced adders, y written in a.y_layout (sample-blocked by the same slab
