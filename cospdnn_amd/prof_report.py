@@ -141,11 +141,12 @@ def fp16_spmm_name(name):
     and some mangled (`_ZN7cospdnn21spmm_simt_rows_kernelIDF16_DF16_EEv...`)."""
     n = name.replace("void ", "")
     base = n.split("(")[0].split("::")[-1]
-    # the sample-slab kernels: spmm_slab[_uni|_band]_kernel_<T>, and spmm_bits_kernel_<T>
+    # the sample-slab kernels: spmm_slab[_uni|_band|_uni_wide]_kernel_<T>, and
+    # spmm_bits[_wide]_kernel_<T>
+    if base.startswith("spmm_bits_") or "cospdnn" in n and "spmm_bits_" in n:
+        return "spmm_bits_kernel"
     if base.startswith("spmm_slab_") or "cospdnn" in n and "spmm_slab_" in n:
         return "spmm_slab_kernel"
-    if base.startswith("spmm_bits_kernel") or "cospdnn" in n and "spmm_bits_kernel" in n:
-        return "spmm_bits_kernel"
     for k in FP16_SPMM:
         if base == k or f"{len(k)}{k}E" in n:
             return k
