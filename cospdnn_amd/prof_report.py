@@ -31,7 +31,7 @@ def trace_table(db, title, cmd):
         "max(workgroup_x), max(vgpr_count), max(accum_vgpr_count), max(lds_size) from kernels group by name "
         "order by sum(duration) desc"))
     tot = sum(r[2] for r in rows) or 1
-    out = [f"# {title}", "", "Command (from the repo root on the GPU box):", "", "```", cmd, "```", "",
+    out = [f"# {title}", "", "Command (from the repo root, on an MI355X):", "", "```", cmd, "```", "",
            "| kernel | calls | total ms | avg us | min us | max us | % | grid | wg | vgpr | agpr | lds B |",
            "|---|---|---|---|---|---|---|---|---|---|---|---|"]
     for n, k, s, a, mn, mx, gx, wx, vg, ag, lds in rows:
@@ -62,7 +62,7 @@ def traffic_table(kernel, alg_bytes, fetch_db, write_db, cfetch_db, cwrite_db, t
     fetch_b = f_kib * ff * 1024
     write_b = w_kib * fw * 1024
     total = fetch_b + write_b
-    out = [f"# {title}", "", "Commands (from the repo root on the GPU box):", "", "```", cmds, "```", "",
+    out = [f"# {title}", "", "Commands (from the repo root, on an MI355X):", "", "```", cmds, "```", "",
            "| quantity | value |", "|---|---|",
            f"| FETCH_SIZE of hbm_copy_kernel (512 MiB read) | {cf:.0f} KiB -> factor {ff:.3f} |",
            f"| WRITE_SIZE of hbm_copy_kernel (512 MiB written) | {cw:.0f} KiB -> factor {fw:.3f} |",
@@ -112,7 +112,7 @@ def mfma_table(kernel, flops_per_launch, counter_csv, trace_csv, simd_num, title
     mops = mean("SQ_INSTS_VALU_MFMA_MOPS_F16")
     flops = mops * 512.0
     t = sum(dur.get(k, 0.0) for k in per) / max(1, sum(1 for k in per if k in dur))
-    out = [f"# {title}", "", "Commands (from the repo root on the GPU box):", "", "```", cmd, "```", "",
+    out = [f"# {title}", "", "Commands (from the repo root, on an MI355X):", "", "```", cmd, "```", "",
            "| quantity | value |", "|---|---|",
            f"| `{kernel}` dispatches | {n} |",
            f"| mean duration under counter collection | {t * 1e6:.1f} us |",
@@ -223,7 +223,7 @@ def layers_table(bench_json, copy_fetch, copy_write, title, cmds, counter_csvs):
     miss = cnt.get("TCC_MISS_sum")
     wait = cnt.get("SQ_WAIT_INST_ANY")
     wcyc = cnt.get("SQ_WAVE_CYCLES")
-    out = [f"# {title}", "", "Commands (from the repo root on the GPU box):", "", "```", cmds, "```", "",
+    out = [f"# {title}", "", "Commands (from the repo root, on an MI355X):", "", "```", cmds, "```", "",
            f"FETCH_SIZE scaled by {ff:.3f} and WRITE_SIZE by {fw:.3f} (what each reads for hbm_copy.out, "
            "512 MiB read and 512 MiB written per dispatch). Algorithmic bytes of layer l: 2 x live_l x N x 2 B "
            "(FP16 activations read and written) + 32 N x 2 B (weight values). SpMM us: the bench line's "
